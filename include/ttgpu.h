@@ -1,0 +1,87 @@
+/* SPDX-License-Identifier: MIT
+ * ttgpu.h — C ABI of the MI355X (gfx950) filtration front end.
+ *
+ * Every entry point replaces one data-parallel function of the reference
+ * library (dnbaker/turbotops v0.1.0, header-only C++20); the comment above
+ * each declaration cites the reference interface it stands in for.
+ *
+ * Conventions
+ *   - Return value: tt_status. On failure, tt_last_error() returns a
+ *     thread-local message whose text matches the reference's exception
+ *     message where the reference has one. Status codes map to the
+ *     reference's exception classes: TT_EINVAL -> std::invalid_argument,
+ *     TT_ELENGTH -> std::length_error, TT_EOVERFLOW -> std::overflow_error,
+ *     TT_EDEVICE -> std::runtime_error (HIP failure; never a silent CPU
+ *     fallback).
+ *   - Metric kinds follow the reference's Metric::Kind
+ *     (include/turbotops/metric.hpp:215): 0 L1, 1 L2, 2 Linf, 3 Lp(p).
+ *   - Points are row-major n x dim float32.
+ *   - "_dev" entry points take device pointers and run asynchronously on
+ *     `stream` (a hipStream_t; NULL = the null stream). Host entry points
+ *     take host pointers, are synchronous and thread-safe.
+ *   - Results do not depend on tile sizes or launch shapes.
+ */
+#ifndef TTGPU_H
+#define TTGPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+    TT_OK = 0,
+    TT_EINVAL = 1,
+    TT_ELENGTH = 2,
+    TT_EOVERFLOW = 3,
+    TT_EDEVICE = 4
+} tt_status;
+
+enum { TT_METRIC_L1 = 0, TT_METRIC_L2 = 1, TT_METRIC_LINF = 2, TT_METRIC_LP = 3 };
+enum { TT_LAYOUT_FULL = 0, TT_LAYOUT_LOWER = 1 };
+
+/* ---- library ------------------------------------------------------------ */
+
+/* Thread-local message of the last failed call on this thread. */
+const char* tt_last_error(void);
+/* Library version, "MAJOR.MINOR.PATCH" (the reference's
+ * TURBOTOPS_VERSION_STRING it is a drop-in for, core.hpp:8-11). */
+const char* tt_version(void);
+/* Number of visible HIP devices (0 if none); no device is initialised by
+ * loading the library. */
+int tt_device_count(void);
+
+/* Metric::parse (metric.hpp:240-253) and Metric::minkowski (:227-237):
+ * "l1" | "manhattan" | "l2" | "euclidean" | "linf" | "chebyshev" |
+ * "lp:<p>" | "lp:inf"; p < 1 or NaN -> TT_EINVAL "metric: p must be in
+ * [1, inf]"; unknown -> TT_EINVAL "metric: unknown '<s>' (...)". */
+tt_status tt_metric_parse(const char* metric, int* kind, float* p);
+
+/* ---- dense distance matrix + enclosing radius ---------------------------- */
+
+/* dense_bytes (rips.hpp:384-386) and the layout choice of make_dense
+ * (rips.hpp:395, :404, :410-414): full when 4n^2 <= budget, else lower when
+ * 2n(n-1) <= budget, else TT_ELENGTH with the reference's message. */
+tt_status tt_dense_layout(uint32_t n, uint64_t budget_bytes, int* layout);
+
+/* rips::make_dense(n, PointCloud, budget) (rips.hpp:391-416) plus
+ * DenseComplex::enclosing_radius() (rips.hpp:370-380), fused.
+ * mat: host buffer of n*n (full) or n(n-1)/2 (lower) floats, written exactly
+ * as the reference lays them out. radius may be NULL. */
+tt_status tt_make_dense(const float* pts, uint32_t n, uint32_t dim, int kind, float p,
+                        int layout, float* mat, float* radius);
+
+/* Device form of tt_make_dense. d_rowmax (n uint32, may be NULL) receives
+ * max_j d(i, j) as float bits; d_radius (1 float, may be NULL) receives
+ * min_i of those, i.e. the enclosing radius. */
+tt_status tt_dist_matrix_dev(const float* d_pts, uint32_t n, uint32_t dim, int kind, float p,
+                             int layout, float* d_mat, uint32_t* d_rowmax, float* d_radius,
+                             void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* TTGPU_H */

@@ -1,0 +1,266 @@
+// SPDX-License-Identifier: MIT
+// ref_harness.cpp — TEST INFRASTRUCTURE ONLY (oracle side).
+//
+// A C ABI over the reference library's own front-end functions, compiled
+// against the reference headers where they lie (see oracle/Makefile) into
+// oracle/_ref/. Nothing here is shipped or measured as the product: the
+// parity tests call it as the checker, and bench.py times it as the CPU
+// baseline. Every entry point names the reference function it drives.
+//
+// Variable-size outputs are returned in malloc'd buffers released by
+// ref_free().
+#include <chrono>
+#include <cstdlib>
+#include <cstring>
+#include <exception>
+#include <string>
+
+#include "turbotops/turbotops.hpp"
+
+using namespace tops;
+
+namespace {
+thread_local std::string g_err;
+
+template <typename F>
+int guard(F&& f) {
+    try {
+        f();
+        return 0;
+    } catch (const std::invalid_argument& e) {
+        g_err = e.what();
+        return 1;
+    } catch (const std::length_error& e) {
+        g_err = e.what();
+        return 2;
+    } catch (const std::overflow_error& e) {
+        g_err = e.what();
+        return 3;
+    } catch (const std::exception& e) {
+        g_err = e.what();
+        return 4;
+    }
+}
+
+template <typename T>
+T* dup(const std::vector<T>& v) {
+    T* p = static_cast<T*>(std::malloc(std::max<size_t>(1, v.size() * sizeof(T))));
+    if (!v.empty()) std::memcpy(p, v.data(), v.size() * sizeof(T));
+    return p;
+}
+
+void flatten(const rips::Barcode& bc, uint64_t* nbars, uint32_t** dims, float** births,
+             float** deaths) {
+    std::vector<uint32_t> d;
+    std::vector<float> b, e;
+    for (u32 k = 0; k < bc.ndims(); ++k)
+        for (u64 i = 0; i < bc.birth[k].size(); ++i) {
+            d.push_back(k);
+            b.push_back(bc.birth[k][i]);
+            e.push_back(bc.death[k][i]);
+        }
+    *nbars = d.size();
+    *dims = dup(d);
+    *births = dup(b);
+    *deaths = dup(e);
+}
+
+double secs_since(std::chrono::steady_clock::time_point t) {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now() - t).count();
+}
+}  // namespace
+
+extern "C" {
+
+const char* ref_last_error() { return g_err.c_str(); }
+void ref_free(void* p) { std::free(p); }
+void ref_set_threads(uint32_t t) { set_num_threads(t); }
+
+// kern::l1/l2/linf/lp through Metric (include/turbotops/metric.hpp:92-225).
+int ref_metric(const char* metric, const float* a, const float* b, uint32_t dim, float* out) {
+    return guard([&] { *out = Metric::parse(metric)(a, b, dim); });
+}
+
+// rips::make_dense with a PointCloud functor (rips.hpp:391-416). *full
+// reports the chosen layout; out must hold n*n floats (full) or n(n-1)/2.
+int ref_make_dense(const float* pts, uint32_t n, uint32_t dim, const char* metric,
+                   uint64_t budget, int* full, float* out) {
+    return guard([&] {
+        const PointCloud c{pts, n, dim, Metric::parse(metric)};
+        rips::DenseComplex cx = rips::make_dense(n, c, budget);
+        *full = cx.full ? 1 : 0;
+        if (!cx.mat.empty()) std::memcpy(out, cx.mat.data(), cx.mat.size() * sizeof(float));
+    });
+}
+
+// DenseComplex::enclosing_radius over a matrix in either layout (rips.hpp:370-380).
+float ref_enclosing_radius(const float* mat, uint32_t n, int full) {
+    rips::DenseComplex cx;
+    cx.n = n;
+    cx.full = full != 0;
+    cx.mat.assign(mat, mat + (full ? u64(n) * n : rips::dense_bytes(n, false) / 4));
+    return cx.enclosing_radius();
+}
+
+// radius_graph (graph.hpp:170-201).
+int ref_radius_graph(const float* pts, uint32_t n, uint32_t dim, const char* metric, float tau,
+                     uint64_t* row_ptr, uint64_t* nnz, uint32_t** col, float** w) {
+    return guard([&] {
+        const PointCloud c{pts, n, dim, Metric::parse(metric)};
+        const CsrGraph g = radius_graph(c, tau);
+        std::memcpy(row_ptr, g.row_ptr.data(), (u64(n) + 1) * sizeof(u64));
+        *nnz = g.col_idx.size();
+        *col = dup(g.col_idx);
+        *w = dup(g.weights);
+    });
+}
+
+// knn_lists_exact (graph.hpp:211-225); ids holds n * min(k, n-1).
+int ref_knn(const float* pts, uint32_t n, uint32_t dim, const char* metric, uint32_t k,
+            uint32_t* kk, uint32_t* ids) {
+    return guard([&] {
+        const PointCloud c{pts, n, dim, Metric::parse(metric)};
+        const KnnLists L = knn_lists_exact(c, k);
+        *kk = L.k;
+        if (!L.ids.empty()) std::memcpy(ids, L.ids.data(), L.ids.size() * sizeof(u32));
+    });
+}
+
+// knn_graph of exact lists (graph.hpp:286-304).
+int ref_knn_graph(const float* pts, uint32_t n, uint32_t dim, const char* metric, uint32_t k,
+                  uint64_t* row_ptr, uint64_t* nnz, uint32_t** col, float** w) {
+    return guard([&] {
+        const PointCloud c{pts, n, dim, Metric::parse(metric)};
+        const CsrGraph g = knn_graph(c, knn_lists_exact(c, k));
+        std::memcpy(row_ptr, g.row_ptr.data(), (u64(n) + 1) * sizeof(u64));
+        *nnz = g.col_idx.size();
+        *col = dup(g.col_idx);
+        *w = dup(g.weights);
+    });
+}
+
+// mst::h0 (mst.hpp:25-99): bars flattened in discovery order.
+int ref_mst_h0(const float* pts, uint32_t n, uint32_t dim, const char* metric, float threshold,
+               uint64_t* nbars, uint32_t** dims, float** births, float** deaths) {
+    return guard([&] {
+        const PointCloud c{pts, n, dim, Metric::parse(metric)};
+        flatten(mst::h0(c, threshold), nbars, dims, births, deaths);
+    });
+}
+
+// The coboundary reducer's dim-0 inputs on the dense complex at the
+// enclosing radius (rips.hpp:685-849): the kept edges sorted by `precedes`
+// (diameter ascending, index descending) and, per edge, whether
+// zero_apparent_cofacet finds its apparent partner.
+int ref_dense_edges(const float* pts, uint32_t n, uint32_t dim, const char* metric,
+                    uint64_t budget, float threshold, uint64_t* m, uint64_t** idx, float** d,
+                    uint8_t** apparent, float* thr_out) {
+    return guard([&] {
+        const PointCloud c{pts, n, dim, Metric::parse(metric)};
+        rips::DenseComplex cx = rips::make_dense(n, c, budget);
+        cx.threshold = std::isinf(threshold) ? cx.enclosing_radius() : threshold;
+        *thr_out = cx.threshold;
+        using E = rips::Entry<u64>;
+        std::vector<E> edges;
+        for (u32 i = 1; i < n; ++i)
+            for (u32 j = 0; j < i; ++j) {
+                const f32 dd = cx.dist(i, j);
+                if (dd <= cx.threshold) edges.push_back(E{u64(i) * (i - 1) / 2 + j, dd});
+            }
+        std::sort(edges.begin(), edges.end(),
+                  [](const E& a, const E& b) { return rips::precedes(a, b); });
+        rips::Barcode out;
+        rips::RipsStats st;
+        rips::Reducer<u64, rips::DenseComplex> r(cx, 1, out, st);
+        std::vector<u64> ix(edges.size());
+        std::vector<float> dv(edges.size());
+        std::vector<uint8_t> ap(edges.size());
+        for (size_t t = 0; t < edges.size(); ++t) {
+            ix[t] = edges[t].idx;
+            dv[t] = edges[t].d;
+            ap[t] = r.zero_apparent_cofacet(edges[t], 2).idx != rips::kNone<u64>;
+        }
+        *m = edges.size();
+        *idx = dup(ix);
+        *d = dup(dv);
+        *apparent = dup(ap);
+    });
+}
+
+// tops::persistence on points (turbotops.hpp:211-303). graph: 0 full, 1 knn;
+// mode: 0 auto, 1 boundary, 2 coboundary.
+int ref_persistence(const float* pts, uint32_t n, uint32_t dim, const char* metric,
+                    uint32_t maxdim, float threshold, int graph, uint32_t k, int mode,
+                    uint64_t budget, uint32_t threads, uint64_t* nbars, uint32_t** dims,
+                    float** births, float** deaths, double* seconds_total, uint64_t* edges) {
+    return guard([&] {
+        Options o;
+        o.maxdim = maxdim;
+        o.threshold = threshold;
+        o.metric = metric;
+        o.graph = graph ? Options::Graph::Knn : Options::Graph::Full;
+        o.k = k;
+        o.mode = mode == 1 ? Mode::Boundary : mode == 2 ? Mode::Coboundary : Mode::Auto;
+        o.memory_budget = budget;
+        o.threads = threads;
+        const Result r = persistence(pts, n, dim, o);
+        flatten(r.barcode, nbars, dims, births, deaths);
+        *seconds_total = r.stats.seconds_total;
+        *edges = r.stats.edges;
+    });
+}
+
+// tops::persistence on a CsrGraph (turbotops.hpp:179-209).
+int ref_persistence_csr(uint32_t n, const uint64_t* row_ptr, const uint32_t* col,
+                        const float* w, uint32_t maxdim, float threshold, int mode,
+                        uint64_t* nbars, uint32_t** dims, float** births, float** deaths) {
+    return guard([&] {
+        CsrGraph g;
+        g.n = n;
+        g.row_ptr.assign(row_ptr, row_ptr + u64(n) + 1);
+        g.m = g.row_ptr[n] / 2;
+        g.col_idx.assign(col, col + g.row_ptr[n]);
+        g.weights.assign(w, w + g.row_ptr[n]);
+        const Result r = persistence(g, maxdim, threshold, 0,
+                                     mode == 1 ? Mode::Boundary
+                                     : mode == 2 ? Mode::Coboundary : Mode::Auto);
+        flatten(r.barcode, nbars, dims, births, deaths);
+    });
+}
+
+// Stage timer for the CPU baseline: make_dense then enclosing_radius, with
+// steady_clock around each (the reference's own threaded code).
+int ref_time_dense(const float* pts, uint32_t n, uint32_t dim, const char* metric,
+                   uint32_t threads, double* s_make, double* s_radius, float* radius) {
+    return guard([&] {
+        if (threads) set_num_threads(threads);
+        const PointCloud c{pts, n, dim, Metric::parse(metric)};
+        const auto t0 = std::chrono::steady_clock::now();
+        rips::DenseComplex cx = rips::make_dense(n, c);
+        *s_make = secs_since(t0);
+        const auto t1 = std::chrono::steady_clock::now();
+        *radius = cx.enclosing_radius();
+        *s_radius = secs_since(t1);
+    });
+}
+
+// Stage timers for the sparse front ends: radius_graph, knn_lists_exact +
+// knn_graph, mst::h0.
+int ref_time_graph(const float* pts, uint32_t n, uint32_t dim, const char* metric, int which,
+                   float tau, uint32_t k, uint32_t threads, double* secs, uint64_t* size) {
+    return guard([&] {
+        if (threads) set_num_threads(threads);
+        const PointCloud c{pts, n, dim, Metric::parse(metric)};
+        const auto t0 = std::chrono::steady_clock::now();
+        if (which == 0) {
+            *size = radius_graph(c, tau).m;
+        } else if (which == 1) {
+            *size = knn_graph(c, knn_lists_exact(c, k)).m;
+        } else {
+            *size = mst::h0(c, tau).birth[0].size();
+        }
+        *secs = secs_since(t0);
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,169 @@
+"""GPU parity of the distance-matrix kernel (tt_dist_matrix_dev / tt_make_dense)
+against the oracle (oracle/ttoracle.c, itself pinned to the reference).
+
+Bar: L1, L2 and L-inf bit-exact, matrix and enclosing radius alike. Lp
+(every p other than 1, 2, inf) within 2 ulp per distance: the reference
+calls glibc powf, which is not correctly rounded; the kernel rounds a
+double-precision power once (see turbotops_amd/csrc/common.hpp pow_f32).
+"""
+import math
+
+import numpy as np
+import pytest
+
+from conftest import bits, ulp_diff
+
+pytestmark = pytest.mark.gpu
+
+EXACT = ["l1", "l2", "linf"]
+LP = ["lp:1.5", "lp:3", "lp:4", "lp:7.5", "lp:50"]
+SHAPES = [(1, 3), (2, 3), (63, 3), (64, 3), (65, 3), (130, 1), (129, 2), (200, 3), (97, 4), (150, 8),
+          (77, 16), (70, 17), (100, 5), (50, 33)]
+
+
+def _cloud(n, d, seed, kind="gauss"):
+    rng = np.random.default_rng(seed)
+    if kind == "grid":
+        return rng.integers(0, 4, size=(n, d)).astype(np.float32)
+    return rng.normal(size=(n, d)).astype(np.float32)
+
+
+def _dev(metric, x, layout):
+    import torch
+    from turbotops_amd import dist_matrix_dev
+    xt = torch.from_numpy(x).cuda()
+    n = x.shape[0]
+    rowmax = torch.zeros(max(n, 1), dtype=torch.int32, device="cuda")
+    radius = torch.zeros(1, dtype=torch.float32, device="cuda")
+    mat = dist_matrix_dev(xt, metric, layout, rowmax=rowmax, radius=radius)
+    torch.cuda.synchronize()
+    return mat.cpu().numpy(), rowmax.cpu().numpy().view(np.float32)[:n], float(radius.cpu()[0])
+
+
+def _check(got, want, metric, what):
+    if metric in EXACT:
+        assert np.array_equal(bits(got), bits(want)), f"{what}: not bit-exact"
+    else:
+        u = ulp_diff(got, want)
+        assert u.max() <= 2, f"{what}: {u.max()} ulp"
+
+
+@pytest.mark.parametrize("metric", EXACT + LP)
+@pytest.mark.parametrize("shape", SHAPES)
+def test_full_and_lower_match_oracle(gpu, restatement, metric, shape):
+    n, d = shape
+    x = _cloud(n, d, seed=n * 100 + d, kind="grid" if d == 2 else "gauss")
+    print(f"case metric={metric} n={n} d={d}", flush=True)
+    want = restatement.make_dense(x, metric, True)
+    want_r = restatement.enclosing_radius(want, n, True)
+    got, rowmax, radius = _dev(metric, x, "full")
+    _check(got.reshape(n, n), want, metric, "full")
+    _check(np.float32(radius), np.float32(want_r), metric, "radius")
+    _check(rowmax, want.max(axis=1) if n else rowmax, metric, "rowmax")
+    low, _, radius_l = _dev(metric, x, "lower")
+    _check(low, restatement.make_dense(x, metric, False), metric, "lower")
+    assert bits(radius_l) == bits(np.float32(radius))
+
+
+@pytest.mark.parametrize("metric", EXACT + ["lp:1.5"])
+def test_host_entry_point(gpu, restatement, metric):
+    from turbotops_amd import make_dense
+    x = _cloud(300, 3, seed=11)
+    full, mat, r = make_dense(x, metric)
+    assert full
+    want = restatement.make_dense(x, metric, True)
+    _check(mat, want, metric, "host full")
+    _check(np.float32(r), np.float32(restatement.enclosing_radius(want, 300)), metric, "host radius")
+    # a budget between the two layouts forces the lower one (tests/test_cli.sh:143-147)
+    budget = (300 * 300 * 4 + 300 * 299 * 2) // 2
+    full, low, r2 = make_dense(x, metric, budget=budget)
+    assert not full and low.size == 300 * 299 // 2
+    _check(low, restatement.make_dense(x, metric, False), metric, "host lower")
+    assert bits(np.float32(r2)) == bits(np.float32(r))
+    with pytest.raises(ValueError, match="exceeds the memory budget"):
+        make_dense(x, metric, budget=1000)
+
+
+def test_unit_square_goldens(gpu):
+    # tests/test_cli.sh:56-94
+    from turbotops_amd import make_dense
+    sq = np.array([[0, 0], [1, 0], [1, 1], [0, 1]], np.float32)
+    for metric, diag in [("l2", "1.41421354"), ("l1", "2"), ("lp:3", "1.25992107"), ("linf", "1")]:
+        _, mat, r = make_dense(sq, metric)
+        assert "%.9g" % mat[0, 2] == diag and "%.9g" % mat[1, 3] == diag
+        assert mat[0, 1] == 1.0 and mat[2, 2] == 0.0
+        assert "%.9g" % r == diag   # every row's max is the diagonal
+
+
+def test_denormals_and_overflow(gpu, restatement):
+    # subnormal differences must not be flushed; overflowing squares give inf
+    x = np.array([[0.0, 0.0], [1e-40, 0.0], [0.0, 3e-39], [1e20, 1e20], [-1e20, 0.0], [1e-30, 2e-30]],
+                 np.float32)
+    for metric in EXACT + ["lp:3", "lp:1.5"]:
+        want = restatement.make_dense(x, metric, True)
+        got, _, r = _dev(metric, x, "full")
+        got = got.reshape(6, 6)
+        if metric in EXACT:
+            assert np.array_equal(bits(got), bits(want)), metric
+        else:
+            fin = np.isfinite(want)
+            assert np.array_equal(fin, np.isfinite(got)), metric
+            assert ulp_diff(got[fin], want[fin]).max() <= 2, metric
+    got, _, _ = _dev("l2", x[:3], "full")
+    assert got.reshape(3, 3)[0, 1] == np.float32(1e-40)
+
+
+def test_ties_grid_linf_l1(gpu, restatement):
+    x = _cloud(257, 3, seed=5, kind="grid")
+    for metric in ["l1", "linf"]:
+        got, _, r = _dev(metric, x, "full")
+        want = restatement.make_dense(x, metric, True)
+        assert np.array_equal(bits(got.reshape(257, 257)), bits(want))
+        assert bits(np.float32(r)) == bits(restatement.enclosing_radius(want, 257))
+
+
+@pytest.mark.parametrize("metric", ["l2", "l1", "linf", "lp:1.5"])
+def test_full_size_properties(gpu, restatement, metric):
+    """n = 20000 (the single-cloud configuration): symmetry, zero diagonal,
+    lower == full, the radius from the matrix itself, and sampled entries
+    against the oracle's per-pair distance."""
+    import torch
+    from turbotops_amd import dist_matrix_dev
+    n, d = 20000, 3 if metric == "l2" else 16
+    rng = np.random.default_rng(2 if d == 3 else 3)
+    x = (rng.normal(size=(n, d)) if d == 3 else rng.uniform(size=(n, d))).astype(np.float32)
+    print(f"case full-size metric={metric} n={n} d={d}", flush=True)
+    xt = torch.from_numpy(x).cuda()
+    rowmax = torch.zeros(n, dtype=torch.int32, device="cuda")
+    radius = torch.zeros(1, dtype=torch.float32, device="cuda")
+    mat = dist_matrix_dev(xt, metric, "full", rowmax=rowmax, radius=radius).view(n, n)
+    torch.cuda.synchronize()
+    assert torch.equal(mat.view(torch.int32), mat.t().contiguous().view(torch.int32))
+    assert torch.count_nonzero(torch.diagonal(mat)) == 0
+    rm = mat.max(dim=1).values
+    assert torch.equal(rm.view(torch.int32), rowmax)
+    assert radius.item() == rm.min().item()
+    low = dist_matrix_dev(xt, metric, "lower")
+    ii, jj = torch.tril_indices(n, n, offset=-1, device="cuda")
+    assert torch.equal(low.view(torch.int32), mat[ii, jj].contiguous().view(torch.int32))
+    del low, ii, jj
+    si = rng.integers(0, n, size=3000)
+    sj = rng.integers(0, n, size=3000)
+    got = mat[torch.from_numpy(si).cuda(), torch.from_numpy(sj).cuda()].cpu().numpy()
+    want = np.array([restatement.dist(metric, x[i], x[j]) for i, j in zip(si, sj)], np.float32)
+    if metric in EXACT:
+        assert np.array_equal(bits(got), bits(want))
+    else:
+        assert ulp_diff(got, want).max() <= 2
+
+
+def test_invalid_arguments(gpu):
+    import torch
+    from turbotops_amd import capi, make_dense
+    with pytest.raises(ValueError, match="p must be in"):
+        make_dense(np.zeros((3, 2), np.float32), (3, 0.5))
+    with pytest.raises(ValueError, match="unknown layout"):
+        capi.lib().call("tt_dist_matrix_dev", None, 0, 1, 1, 2.0, 7, None, None, None, None)
+    x = torch.zeros((4, 2), device="cuda")
+    with pytest.raises(ValueError, match="null buffer"):
+        capi.lib().call("tt_dist_matrix_dev", x.data_ptr(), 4, 2, 1, 2.0, 0, None, None, None, None)

@@ -1,0 +1,16 @@
+"""turbotops_amd — the MI355X (gfx950) filtration front end of turbotops.
+
+A drop-in for the data-parallel front end of dnbaker/turbotops v0.1.0:
+Minkowski distance matrices, the enclosing radius, exact radius graphs,
+exact kNN and kNN graphs, H0 via Boruvka, and the apparent-pair scan, as
+hand-written HIP kernels behind the C ABI in include/ttgpu.h.
+"""
+from . import capi  # noqa: F401
+from .frontend import (  # noqa: F401
+    dense_layout,
+    dist_matrix_dev,
+    make_dense,
+    parse_metric,
+)
+
+__version__ = "0.1.0"

@@ -1,0 +1,96 @@
+"""Front-end operations of the reference (dnbaker/turbotops) on MI355X.
+
+Host-array forms mirror the reference's C++ functions (same names, argument
+meaning and errors); ``*_dev`` forms take torch device tensors and run on
+the current HIP stream without leaving HBM. Every call goes through the C
+ABI of libttgpu (include/ttgpu.h); nothing here computes on the CPU.
+"""
+import ctypes as C
+import math
+
+import numpy as np
+
+from . import capi
+
+_f32p = C.POINTER(C.c_float)
+
+
+def _addr(a):
+    return C.c_void_p(a.ctypes.data) if a.size else None
+
+
+def _f32(x):
+    return np.ascontiguousarray(x, dtype=np.float32)
+
+
+def parse_metric(metric):
+    """Metric::parse (include/turbotops/metric.hpp:240-253) -> (kind, p)."""
+    k, p = C.c_int(), C.c_float()
+    capi.lib().call("tt_metric_parse", str(metric).encode(), C.byref(k), C.byref(p))
+    return k.value, p.value
+
+
+def _metric(metric):
+    if isinstance(metric, tuple):
+        return metric
+    return parse_metric(metric)
+
+
+def dense_layout(n, budget):
+    """make_dense's layout gate (rips.hpp:395-414): 'full', 'lower' or ValueError."""
+    lay = C.c_int()
+    capi.lib().call("tt_dense_layout", int(n), int(budget), C.byref(lay))
+    return "full" if lay.value == capi.LAYOUT_FULL else "lower"
+
+
+def make_dense(x, metric="l2", budget=2**64 - 1, with_radius=True):
+    """rips::make_dense + DenseComplex::enclosing_radius (rips.hpp:370-416).
+
+    Returns (full, mat, radius): mat is (n, n) for the full layout or the
+    flat n(n-1)/2 lower triangle, written exactly as the reference lays it
+    out; radius is the enclosing radius (None unless with_radius)."""
+    x = _f32(x)
+    if x.ndim != 2:
+        raise ValueError("x must be a 2-d array (n, d)")
+    n, d = x.shape
+    kind, p = _metric(metric)
+    layout = dense_layout(n, budget)
+    full = layout == "full"
+    cells = n * n if full else n * (n - 1) // 2
+    mat = np.empty(cells, np.float32)
+    r = C.c_float(math.inf)
+    capi.lib().call("tt_make_dense", _addr(x), n, d, kind, p,
+                    capi.LAYOUT_FULL if full else capi.LAYOUT_LOWER, _addr(mat),
+                    C.byref(r) if with_radius else None)
+    return full, (mat.reshape(n, n) if full else mat), (r.value if with_radius else None)
+
+
+def _stream(stream):
+    if stream is None:
+        import torch
+        return C.c_void_p(torch.cuda.current_stream().cuda_stream)
+    return C.c_void_p(int(stream))
+
+
+def _dptr(t):
+    return C.c_void_p(t.data_ptr()) if t is not None and t.numel() else None
+
+
+def dist_matrix_dev(x, metric="l2", layout="full", out=None, rowmax=None, radius=None, stream=None):
+    """Device form: x is a (n, d) float32 torch tensor in HBM. Returns the
+    matrix tensor (n*n or n(n-1)/2 floats); fills rowmax (n int32 holding
+    float bits) and radius (1 float) when given."""
+    import torch
+    if x.dtype != torch.float32 or not x.is_contiguous() or x.dim() != 2:
+        raise ValueError("x must be a contiguous (n, d) float32 tensor")
+    n, d = x.shape
+    kind, p = _metric(metric)
+    lay = capi.LAYOUT_FULL if layout == "full" else capi.LAYOUT_LOWER
+    cells = n * n if lay == capi.LAYOUT_FULL else n * (n - 1) // 2
+    if out is None:
+        out = torch.empty(cells, dtype=torch.float32, device=x.device)
+    elif out.numel() < cells:
+        raise ValueError("out is too small")
+    capi.lib().call("tt_dist_matrix_dev", _dptr(x), n, d, kind, p, lay, _dptr(out), _dptr(rowmax),
+                    _dptr(radius), _stream(stream))
+    return out
