@@ -80,6 +80,30 @@ tt_status tt_dist_matrix_dev(const float* d_pts, uint32_t n, uint32_t dim, int k
                              int layout, float* d_mat, uint32_t* d_rowmax, float* d_radius,
                              void* stream);
 
+/* ---- dimension 0 of the coboundary reducer -------------------------------- */
+
+/* Reducer::dim0's kept edges and sort (rips.hpp:685-839) and its parallel
+ * apparent test (rips.hpp:844-849 -> zero_apparent_cofacet, :585-665), on the
+ * dense full-layout complex: every edge j < i with d(i, j) <= threshold as
+ * (idx = i(i-1)/2 + j, d), ordered by `precedes` (rips.hpp:116-118:
+ * diameter ascending, index descending), and apparent[e] = 1 when the edge
+ * is the zero-pivot facet of its zero-pivot cofacet. threshold = +inf means
+ * the enclosing radius (rips.hpp:1073), reported in *threshold_out.
+ * The library mallocs *idx, *d and (if apparent != NULL) *apparent; release
+ * them with tt_free. n <= 92682 (vertex pairs < 2^32). */
+tt_status tt_dense_edges(const float* pts, uint32_t n, uint32_t dim, int kind, float p,
+                         float threshold, uint64_t* m, uint64_t** idx, float** d,
+                         uint8_t** apparent, float* threshold_out);
+
+/* Device form over a full-layout matrix already in HBM. With d_idx == NULL
+ * only *m is computed; otherwise the outputs (capacity entries) are filled. */
+tt_status tt_dense_edges_dev(const float* d_mat, uint32_t n, float threshold, int want_apparent,
+                             uint64_t capacity, uint64_t* m, uint64_t* d_idx, float* d_d,
+                             uint8_t* d_apparent, void* stream);
+
+/* Releases a buffer the library allocated for a host-side result. */
+void tt_free(void* p);
+
 #ifdef __cplusplus
 }
 #endif

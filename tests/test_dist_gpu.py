@@ -41,6 +41,9 @@ def _dev(metric, x, layout):
 
 
 def _check(got, want, metric, what):
+    assert np.shape(got) == np.shape(want), what
+    if np.size(want) == 0:
+        return
     if metric in EXACT:
         assert np.array_equal(bits(got), bits(want)), f"{what}: not bit-exact"
     else:
@@ -109,8 +112,8 @@ def test_denormals_and_overflow(gpu, restatement):
             fin = np.isfinite(want)
             assert np.array_equal(fin, np.isfinite(got)), metric
             assert ulp_diff(got[fin], want[fin]).max() <= 2, metric
-    got, _, _ = _dev("l2", x[:3], "full")
-    assert got.reshape(3, 3)[0, 1] == np.float32(1e-40)
+    got, _, _ = _dev("l1", x[:3], "full")
+    assert got.reshape(3, 3)[0, 1] == np.float32(1e-40)   # a subnormal |difference| survives (no FTZ)
 
 
 def test_ties_grid_linf_l1(gpu, restatement):

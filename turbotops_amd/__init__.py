@@ -7,6 +7,7 @@ hand-written HIP kernels behind the C ABI in include/ttgpu.h.
 """
 from . import capi  # noqa: F401
 from .frontend import (  # noqa: F401
+    dense_edges,
     dense_layout,
     dist_matrix_dev,
     make_dense,

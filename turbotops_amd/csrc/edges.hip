@@ -1,0 +1,307 @@
+// SPDX-License-Identifier: MIT
+// edges.hip — the coboundary reducer's dimension-0 inputs on the dense
+// complex: the kept edges in filtration order and, per edge, the apparent
+// pair test.
+//
+// Replaces Reducer::dim0's edge enumeration, counting and sort
+// (include/turbotops/rips.hpp:685-839) and its parallel apparent test
+// (rips.hpp:844-849 -> zero_apparent_cofacet, :585-665 on the full layout,
+// cofacets_full :281-344).
+//
+// Pipeline (all on one stream, matrix resident in HBM, full layout):
+//   edge_count     per row i: #{j < i : d(i, j) <= threshold}, plus the min /
+//                  max float bits of the kept diameters (exact integer order
+//                  of non-negative floats)
+//   scan           row offsets
+//   edge_compact   one wave per row, ballot + mbcnt compaction of the kept
+//                  edges into 64-bit keys  ((d - dmin) << ib) | (M - 1 - idx),
+//                  M = n(n-1)/2: ascending keys are the reference's
+//                  `precedes` order (diameter ascending, index descending)
+//   radix sort     only the bits that vary
+//   apparent_scan  one wave64 per edge: w runs from n - 1 down in chunks of
+//                  64; a ballot of max(d(i,w), d(j,w)) <= d finds the first
+//                  cofacet, and the zero-pivot-facet rule on the triangle
+//                  decides whether the edge is its apparent partner.
+#include <algorithm>
+
+#include "sort.hpp"
+
+namespace ttg {
+
+namespace {
+
+constexpr u32 EB = 256;
+
+__device__ __forceinline__ float at_full(const float* __restrict__ mat, u32 n, u32 i, u32 j) {
+    return mat[u64(i) * n + j];
+}
+
+// Largest v with v(v-1)/2 <= idx, and the edge (v, idx - v(v-1)/2), as the
+// reference's closed-form decode (rips.hpp:549-556).
+__device__ __forceinline__ void decode_edge(u64 idx, u32& i, u32& j) {
+    u64 v = u64((1.0 + __builtin_sqrt(1.0 + 8.0 * double(idx))) * 0.5);
+    while (v * (v - 1) / 2 > idx) --v;
+    while ((v + 1) * v / 2 <= idx) ++v;
+    i = u32(v);
+    j = u32(idx - v * (v - 1) / 2);
+}
+
+__global__ __launch_bounds__(EB) void edge_count(const float* __restrict__ mat, u32 n, float thr,
+                                                 u32* __restrict__ cnt, u32* __restrict__ mm) {
+    // one wave per row, rows grid-strided
+    const u32 lane = threadIdx.x & 63;
+    const u32 gw = (blockIdx.x * EB + threadIdx.x) >> 6;
+    const u32 nw = (gridDim.x * EB) >> 6;
+    u32 lo = 0xffffffffu, hi = 0u;
+    for (u32 i = gw; i < n; i += nw) {
+        const float* __restrict__ row = mat + u64(i) * n;
+        u32 c = 0;
+        for (u32 j = lane; j < i; j += 64) {
+            const float d = row[j];
+            if (d <= thr) {
+                ++c;
+                const u32 b = __float_as_uint(d);
+                lo = min(lo, b);
+                hi = max(hi, b);
+            }
+        }
+#pragma unroll
+        for (u32 off = 32; off > 0; off >>= 1) c += __shfl_xor(c, off);
+        if (lane == 0) cnt[i] = c;
+    }
+#pragma unroll
+    for (u32 off = 32; off > 0; off >>= 1) {
+        lo = min(lo, __shfl_xor(lo, off));
+        hi = max(hi, __shfl_xor(hi, off));
+    }
+    if (lane == 0) {
+        atomicMin(mm, lo);
+        atomicMax(mm + 1, hi);
+    }
+}
+
+__global__ __launch_bounds__(EB) void edge_compact(const float* __restrict__ mat, u32 n, float thr,
+                                                   const u32* __restrict__ off, u32 dmin, int ib,
+                                                   u64 M, u64* __restrict__ keys) {
+    const u32 lane = threadIdx.x & 63;
+    const u32 gw = (blockIdx.x * EB + threadIdx.x) >> 6;
+    const u32 nw = (gridDim.x * EB) >> 6;
+    for (u32 i = gw; i < n; i += nw) {
+        const float* __restrict__ row = mat + u64(i) * n;
+        u64 o = off[i];
+        const u64 rowbase = u64(i) * (i - 1) / 2;
+        for (u32 j0 = 0; j0 < i; j0 += 64) {
+            const u32 j = j0 + lane;
+            const float d = j < i ? row[j] : INFINITY;
+            const bool keep = j < i && d <= thr;
+            const u64 b = __ballot(keep);
+            if (keep) {
+                const u32 r = __builtin_amdgcn_mbcnt_hi(u32(b >> 32), __builtin_amdgcn_mbcnt_lo(u32(b), 0u));
+                const u64 idx = rowbase + j;
+                keys[o + r] = (u64(__float_as_uint(d) - dmin) << ib) | (M - 1 - idx);
+            }
+            o += __popcll(b);
+        }
+    }
+}
+
+__global__ __launch_bounds__(EB) void edge_unpack(const u64* __restrict__ keys, u64 m, u32 dmin,
+                                                  int ib, u64 M, u64* __restrict__ idx,
+                                                  float* __restrict__ d) {
+    const u64 mask = ib >= 64 ? ~0ull : ((u64(1) << ib) - 1);
+    for (u64 t = blockIdx.x * u64(EB) + threadIdx.x; t < m; t += u64(gridDim.x) * EB) {
+        const u64 k = keys[t];
+        if (idx) idx[t] = M - 1 - (k & mask);
+        if (d) d[t] = __uint_as_float(u32(k >> ib) + dmin);
+    }
+}
+
+// Zero-apparent test of sorted edges, one wave64 per edge.
+__global__ __launch_bounds__(EB) void apparent_scan(const float* __restrict__ mat, u32 n,
+                                                    const u64* __restrict__ idx,
+                                                    const float* __restrict__ dd, u64 m,
+                                                    u8* __restrict__ flags) {
+    const u32 lane = threadIdx.x & 63;
+    const u64 gw = (blockIdx.x * u64(EB) + threadIdx.x) >> 6;
+    const u64 nw = (u64(gridDim.x) * EB) >> 6;
+    for (u64 e = gw; e < m; e += nw) {
+        u32 i, j;
+        decode_edge(idx[e], i, j);
+        const float d = dd[e];
+        const float* __restrict__ ri = mat + u64(i) * n;
+        const float* __restrict__ rj = mat + u64(j) * n;
+        // first cofacet in descending w: max(d(i, w), d(j, w)) <= d, w not i, j
+        // (cofacets_full with cap = diameter, rips.hpp:326-343, :593-598)
+        int wf = -1;
+        float diw = 0.0f, djw = 0.0f;
+        for (int hi = int(n); hi > 0; hi -= 64) {
+            const int w = hi - 1 - int(lane);
+            float a = INFINITY, b = INFINITY;
+            if (w >= 0) {
+                a = ri[w];
+                b = rj[w];
+            }
+            const bool c = w >= 0 && u32(w) != i && u32(w) != j && fmax_ref(a, b) <= d;
+            const u64 bal = __ballot(c);
+            if (bal) {
+                const int src = __builtin_ctzll(bal);
+                wf = hi - 1 - src;
+                diw = __shfl(a, src);
+                djw = __shfl(b, src);
+                break;
+            }
+        }
+        if (lane == 0) {
+            u8 ap = 0;
+            if (wf >= 0) {
+                // zero_pivot_facet_v on the triangle, facets in ascending index
+                // order (rips.hpp:609-636): the first facet holding an edge of
+                // length d must be (i, j) itself.
+                const u32 w = u32(wf);
+                if (w > i) ap = 1;                                  // facet (i, j) comes first
+                else if (w > j) ap = djw != d;                      // (w, j) precedes (i, j)
+                else ap = (djw != d) && (diw != d);                 // (j, w), (i, w) precede
+            }
+            flags[e] = ap;
+        }
+    }
+}
+
+u32 grid_for(u64 waves) {
+    return u32(std::max<u64>(1, std::min<u64>((waves + 3) / 4, 256 * 64)));
+}
+
+}  // namespace
+
+struct DenseEdges {
+    u64 m = 0;
+    u64* idx = nullptr;     // device, sorted
+    float* d = nullptr;     // device, sorted
+    u8* apparent = nullptr; // device (null unless requested)
+};
+
+// Kept edges of a full device matrix, in filtration order, and optionally
+// their apparent flags. Buffers live in ws.
+DenseEdges dense_edges(const float* d_mat, u32 n, float thr, bool want_apparent, Workspace& ws,
+                       hipStream_t s) {
+    DenseEdges out;
+    if (n < 2) return out;
+    if (u64(n) * (n - 1) / 2 >= (u64(1) << 32))
+        fail(TT_ELENGTH, "dense_edges: more than 2^32 vertex pairs (n > 92682)");
+    const u64 M = u64(n) * (n - 1) / 2;
+    u32* cnt = ws.get<u32>(u64(n) + 1);
+    u32* mm = ws.get<u32>(2);
+    const u32 init[2] = {0xffffffffu, 0u};
+    TTG_HIP(hipMemcpyAsync(mm, init, sizeof(init), hipMemcpyHostToDevice, s));
+    edge_count<<<grid_for(n), EB, 0, s>>>(d_mat, n, thr, cnt, mm);
+    TTG_LAUNCH_CHECK();
+    u32* total = ws.get<u32>(1);
+    exclusive_scan_u32(cnt, cnt, n, total, ws, s);
+    u32 hm[3];
+    TTG_HIP(hipMemcpyAsync(hm, mm, 2 * sizeof(u32), hipMemcpyDeviceToHost, s));
+    TTG_HIP(hipMemcpyAsync(hm + 2, total, sizeof(u32), hipMemcpyDeviceToHost, s));
+    TTG_HIP(hipStreamSynchronize(s));
+    out.m = hm[2];
+    if (!out.m) return out;
+    const u32 dmin = hm[0], dmax = hm[1];
+    const int ib = std::max(1, bit_length(M - 1));
+    const int db = bit_length(u64(dmax - dmin));
+    if (ib + db > 64) fail(TT_ELENGTH, "dense_edges: key does not fit 64 bits");
+    u64* keys = ws.get<u64>(out.m);
+    u64* alt = ws.get<u64>(out.m);
+    edge_compact<<<grid_for(n), EB, 0, s>>>(d_mat, n, thr, cnt, dmin, ib, M, keys);
+    TTG_LAUNCH_CHECK();
+    radix_sort_u64(keys, nullptr, out.m, 0, ib + db, alt, nullptr, ws, s);
+    out.idx = alt;   // reuse the ping-pong buffer for the unpacked indices
+    out.d = ws.get<float>(out.m);
+    edge_unpack<<<u32(std::min<u64>((out.m + EB - 1) / EB, 65536)), EB, 0, s>>>(keys, out.m, dmin, ib, M,
+                                                                               out.idx, out.d);
+    TTG_LAUNCH_CHECK();
+    if (want_apparent) {
+        out.apparent = ws.get<u8>(out.m);
+        apparent_scan<<<grid_for(out.m), EB, 0, s>>>(d_mat, n, out.idx, out.d, out.m, out.apparent);
+        TTG_LAUNCH_CHECK();
+    }
+    return out;
+}
+
+void dist_matrix(const float* d_x, u32 n, u32 dim, int kind, float p, int layout, float* d_mat,
+                 u32* d_rowmax, float* d_radius, hipStream_t s);
+
+}  // namespace ttg
+
+using namespace ttg;
+
+extern "C" {
+
+tt_status tt_dense_edges_dev(const float* d_mat, uint32_t n, float threshold, int want_apparent,
+                             uint64_t capacity, uint64_t* m, uint64_t* d_idx, float* d_d,
+                             uint8_t* d_apparent, void* stream) {
+    return guarded([&] {
+        if (!m) fail(TT_EINVAL, "tt_dense_edges_dev: null count");
+        if (n && !d_mat) fail(TT_EINVAL, "tt_dense_edges_dev: null matrix");
+        hipStream_t s = as_stream(stream);
+        Workspace ws;
+        const DenseEdges e = dense_edges(d_mat, n, threshold, want_apparent != 0, ws, s);
+        *m = e.m;
+        if (!d_idx) {
+            TTG_HIP(hipStreamSynchronize(s));
+            return;
+        }
+        if (e.m > capacity) fail(TT_ELENGTH, "tt_dense_edges_dev: output capacity too small");
+        if (e.m) {
+            TTG_HIP(hipMemcpyAsync(d_idx, e.idx, e.m * sizeof(u64), hipMemcpyDeviceToDevice, s));
+            if (d_d) TTG_HIP(hipMemcpyAsync(d_d, e.d, e.m * sizeof(float), hipMemcpyDeviceToDevice, s));
+            if (d_apparent && e.apparent)
+                TTG_HIP(hipMemcpyAsync(d_apparent, e.apparent, e.m, hipMemcpyDeviceToDevice, s));
+        }
+        TTG_HIP(hipStreamSynchronize(s));   // ws is released on return
+    });
+}
+
+tt_status tt_dense_edges(const float* pts, uint32_t n, uint32_t dim, int kind, float p,
+                         float threshold, uint64_t* m, uint64_t** idx, float** d,
+                         uint8_t** apparent, float* threshold_out) {
+    return guarded([&] {
+        check_metric(kind, p);
+        check_points(pts, u64(n) * dim, "tt_dense_edges");
+        if (!m || !idx || !d) fail(TT_EINVAL, "tt_dense_edges: null output");
+        if (std::isnan(threshold) || threshold < 0.0f)
+            fail(TT_EINVAL, "persistence: threshold must be non-negative or +inf");
+        *m = 0;
+        *idx = nullptr;
+        *d = nullptr;
+        if (apparent) *apparent = nullptr;
+        hipStream_t s = nullptr;
+        Workspace ws;
+        float* dx = ws.get<float>(u64(n) * dim);
+        float* dm = ws.get<float>(u64(n) * n);
+        u32* rowmax = ws.get<u32>(n);
+        float* dr = ws.get<float>(1);
+        if (u64(n) * dim)
+            TTG_HIP(hipMemcpyAsync(dx, pts, u64(n) * dim * sizeof(float), hipMemcpyHostToDevice, s));
+        dist_matrix(dx, n, dim, kind, p, TT_LAYOUT_FULL, dm, rowmax, dr, s);
+        float thr = threshold;
+        if (std::isinf(threshold)) {
+            TTG_HIP(hipMemcpyAsync(&thr, dr, sizeof(float), hipMemcpyDeviceToHost, s));
+            TTG_HIP(hipStreamSynchronize(s));
+        }
+        if (threshold_out) *threshold_out = thr;
+        const DenseEdges e = dense_edges(dm, n, thr, apparent != nullptr, ws, s);
+        *m = e.m;
+        *idx = static_cast<uint64_t*>(std::malloc(std::max<u64>(1, e.m) * sizeof(u64)));
+        *d = static_cast<float*>(std::malloc(std::max<u64>(1, e.m) * sizeof(float)));
+        if (apparent) *apparent = static_cast<uint8_t*>(std::malloc(std::max<u64>(1, e.m)));
+        if (!*idx || !*d || (apparent && !*apparent)) fail(TT_EDEVICE, "out of host memory");
+        if (e.m) {
+            TTG_HIP(hipMemcpyAsync(*idx, e.idx, e.m * sizeof(u64), hipMemcpyDeviceToHost, s));
+            TTG_HIP(hipMemcpyAsync(*d, e.d, e.m * sizeof(float), hipMemcpyDeviceToHost, s));
+            if (apparent) TTG_HIP(hipMemcpyAsync(*apparent, e.apparent, e.m, hipMemcpyDeviceToHost, s));
+        }
+        TTG_HIP(hipStreamSynchronize(s));
+    });
+}
+
+void tt_free(void* p) { std::free(p); }
+
+}  // extern "C"

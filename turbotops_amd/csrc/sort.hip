@@ -1,0 +1,292 @@
+// SPDX-License-Identifier: MIT
+// sort.hip — exclusive prefix sums and a stable LSD radix sort (64-bit keys,
+// optional 32-bit payload) for gfx950.
+//
+// Radix sort, one pass per 8-bit digit:
+//   upsweep    per-block digit histogram (LDS atomics, one counter row per
+//              wave64) -> hist[digit][block]
+//   scan       exclusive scan of hist in digit-major order: the global start
+//              of every (digit, block) run
+//   downsweep  each block ranks its 4096 keys stably — wave64 ballots on the
+//              digit bits give the lanes sharing a digit, mbcnt the rank among
+//              them, per-wave running counters keep (t, lane) order — reorders
+//              them in LDS, then writes each digit run contiguously.
+#include <algorithm>
+
+#include "sort.hpp"
+
+namespace ttg {
+namespace {
+
+constexpr u32 SB = 256;        // scan block
+constexpr u32 SI = 8;          // items per thread
+constexpr u32 STILE = SB * SI;
+
+template <typename T>
+__device__ __forceinline__ T wave_inclusive_scan(T v, u32 lane) {
+#pragma unroll
+    for (u32 off = 1; off < 64; off <<= 1) {
+        const T o = __shfl_up(v, off);
+        if (lane >= off) v += o;
+    }
+    return v;
+}
+
+// Exclusive scan of one value per thread across a 256-thread block; returns
+// the exclusive prefix and writes the block total to *total.
+template <typename T>
+__device__ __forceinline__ T block_exclusive_scan(T v, T* total) {
+    __shared__ T wsum[SB / 64];
+    const u32 lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const T inc = wave_inclusive_scan(v, lane);
+    if (lane == 63) wsum[w] = inc;
+    __syncthreads();
+    T before = 0, all = 0;
+#pragma unroll
+    for (u32 t = 0; t < SB / 64; ++t) {
+        if (t < w) before += wsum[t];
+        all += wsum[t];
+    }
+    __syncthreads();
+    *total = all;
+    return before + inc - v;
+}
+
+template <typename T>
+__global__ __launch_bounds__(SB) void scan_reduce(const T* __restrict__ in, u64 count,
+                                                  T* __restrict__ sums) {
+    const u64 base = u64(blockIdx.x) * STILE;
+    T s = 0;
+#pragma unroll
+    for (u32 t = 0; t < SI; ++t) {
+        const u64 i = base + u64(t) * SB + threadIdx.x;
+        if (i < count) s += in[i];
+    }
+    T total;
+    (void)block_exclusive_scan(s, &total);
+    if (threadIdx.x == 0) sums[blockIdx.x] = total;
+}
+
+// in and out may alias: a block reads its whole tile before writing.
+template <typename T>
+__global__ __launch_bounds__(SB) void scan_apply(const T* in, T* out, u64 count,
+                                                 const T* __restrict__ offsets,
+                                                 T* __restrict__ d_total) {
+    __shared__ T tile[STILE];
+    const u64 base = u64(blockIdx.x) * STILE;
+#pragma unroll
+    for (u32 t = 0; t < SI; ++t) {
+        const u64 i = base + u64(t) * SB + threadIdx.x;
+        tile[t * SB + threadIdx.x] = i < count ? in[i] : T(0);
+    }
+    __syncthreads();
+    T v[SI], s = 0;
+#pragma unroll
+    for (u32 t = 0; t < SI; ++t) {
+        v[t] = tile[threadIdx.x * SI + t];
+        s += v[t];
+    }
+    T total;
+    T run = block_exclusive_scan(s, &total) + (offsets ? offsets[blockIdx.x] : T(0));
+    __syncthreads();
+#pragma unroll
+    for (u32 t = 0; t < SI; ++t) {
+        tile[threadIdx.x * SI + t] = run;
+        run += v[t];
+    }
+    __syncthreads();
+#pragma unroll
+    for (u32 t = 0; t < SI; ++t) {
+        const u64 i = base + u64(t) * SB + threadIdx.x;
+        if (i < count) out[i] = tile[t * SB + threadIdx.x];
+    }
+    if (d_total && blockIdx.x == gridDim.x - 1 && threadIdx.x == SB - 1)
+        *d_total = tile[STILE - 1] + v[SI - 1];
+}
+
+template <typename T>
+void exclusive_scan(const T* in, T* out, u64 count, T* d_total, Workspace& ws, hipStream_t s) {
+    if (count == 0) {
+        if (d_total) TTG_HIP(hipMemsetAsync(d_total, 0, sizeof(T), s));
+        return;
+    }
+    const u64 blocks = (count + STILE - 1) / STILE;
+    if (blocks == 1) {
+        scan_apply<T><<<1, SB, 0, s>>>(in, out, count, nullptr, d_total);
+        TTG_LAUNCH_CHECK();
+        return;
+    }
+    T* sums = ws.get<T>(blocks);
+    scan_reduce<T><<<u32(blocks), SB, 0, s>>>(in, count, sums);
+    TTG_LAUNCH_CHECK();
+    exclusive_scan<T>(sums, sums, blocks, nullptr, ws, s);
+    scan_apply<T><<<u32(blocks), SB, 0, s>>>(in, out, count, sums, d_total);
+    TTG_LAUNCH_CHECK();
+}
+
+// ---- radix sort ---------------------------------------------------------
+constexpr u32 RB = 256;             // threads (4 wave64s)
+constexpr u32 RI = 16;              // keys per thread
+constexpr u32 RTILE = RB * RI;      // keys per block
+constexpr u32 RADIX = 256;
+constexpr u32 WSEG = RTILE / (RB / 64);   // keys per wave segment
+
+__device__ __forceinline__ u32 lane_rank(u64 mask) {
+    return __builtin_amdgcn_mbcnt_hi(u32(mask >> 32), __builtin_amdgcn_mbcnt_lo(u32(mask), 0u));
+}
+
+__global__ __launch_bounds__(RB) void rs_upsweep(const u64* __restrict__ keys, u64 count,
+                                                 u32 shift, u32 dmask, u32* __restrict__ hist,
+                                                 u32 nblocks) {
+    __shared__ u32 h[RB / 64][RADIX];
+    for (u32 t = threadIdx.x; t < (RB / 64) * RADIX; t += RB) (&h[0][0])[t] = 0;
+    __syncthreads();
+    const u32 lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const u64 base = u64(blockIdx.x) * RTILE + u64(w) * WSEG;
+#pragma unroll 4
+    for (u32 t = 0; t < RI; ++t) {
+        const u64 i = base + t * 64 + lane;
+        if (i < count) atomicAdd(&h[w][u32(keys[i] >> shift) & dmask], 1u);
+    }
+    __syncthreads();
+    for (u32 d = threadIdx.x; d < RADIX; d += RB) {
+        u32 c = 0;
+#pragma unroll
+        for (u32 v = 0; v < RB / 64; ++v) c += h[v][d];
+        hist[u64(d) * nblocks + blockIdx.x] = c;
+    }
+}
+
+template <bool VALS>
+__global__ __launch_bounds__(RB) void rs_downsweep(const u64* __restrict__ keys,
+                                                   const u32* __restrict__ vals, u64 count,
+                                                   u32 shift, u32 dmask, u32 dbits,
+                                                   const u32* __restrict__ offs, u32 nblocks,
+                                                   u64* __restrict__ keys_out,
+                                                   u32* __restrict__ vals_out) {
+    __shared__ u32 wrun[RB / 64][RADIX];
+    __shared__ u32 bstart[RADIX];
+    __shared__ u64 skeys[RTILE];
+    __shared__ u32 svals[VALS ? RTILE : 1];
+
+    const u32 lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    for (u32 t = threadIdx.x; t < (RB / 64) * RADIX; t += RB) (&wrun[0][0])[t] = 0;
+    __syncthreads();
+
+    const u64 tile0 = u64(blockIdx.x) * RTILE;
+    const u64 base = tile0 + u64(w) * WSEG;
+    u64 k[RI];
+    u32 v[RI];
+#pragma unroll
+    for (u32 t = 0; t < RI; ++t) {
+        const u64 i = base + t * 64 + lane;
+        const bool ok = i < count;
+        k[t] = ok ? keys[i] : 0;
+        if constexpr (VALS) v[t] = ok ? vals[i] : 0;
+        if (ok) atomicAdd(&wrun[w][u32(k[t] >> shift) & dmask], 1u);
+    }
+    __syncthreads();
+
+    // block-local digit starts and each wave's running start per digit
+    {
+        const u32 d = threadIdx.x;   // RB == RADIX
+        u32 c[RB / 64], tot = 0;
+#pragma unroll
+        for (u32 q = 0; q < RB / 64; ++q) {
+            c[q] = wrun[q][d];
+            tot += c[q];
+        }
+        u32 all;
+        const u32 start = block_exclusive_scan(tot, &all);
+        bstart[d] = start;
+        u32 run = start;
+#pragma unroll
+        for (u32 q = 0; q < RB / 64; ++q) {
+            wrun[q][d] = run;
+            run += c[q];
+        }
+    }
+    __syncthreads();
+
+    // stable ranking, (t, lane) order within each wave segment
+#pragma unroll
+    for (u32 t = 0; t < RI; ++t) {
+        const u64 i = base + t * 64 + lane;
+        const bool ok = i < count;
+        const u32 d = u32(k[t] >> shift) & dmask;
+        u64 peers = __ballot(ok);
+        for (u32 b = 0; b < dbits; ++b) {
+            const u64 m = __ballot((d >> b) & 1u);
+            peers &= ((d >> b) & 1u) ? m : ~m;
+        }
+        const u32 rank = lane_rank(peers);
+        u32 pos = 0;
+        if (ok) pos = wrun[w][d] + rank;
+        __builtin_amdgcn_wave_barrier();
+        if (ok && rank == 0) wrun[w][d] = pos + u32(__popcll(peers));
+        __builtin_amdgcn_wave_barrier();
+        if (ok) {
+            skeys[pos] = k[t];
+            if constexpr (VALS) svals[pos] = v[t];
+        }
+    }
+    __syncthreads();
+
+    const u32 tile_count = u32(min<u64>(RTILE, count - tile0));
+#pragma unroll 4
+    for (u32 t = 0; t < RI; ++t) {
+        const u32 q = t * RB + threadIdx.x;
+        if (q < tile_count) {
+            const u64 key = skeys[q];
+            const u32 d = u32(key >> shift) & dmask;
+            const u64 g = u64(offs[u64(d) * nblocks + blockIdx.x]) + (q - bstart[d]);
+            keys_out[g] = key;
+            if constexpr (VALS) vals_out[g] = svals[q];
+        }
+    }
+}
+
+}  // namespace
+
+void exclusive_scan_u32(const u32* in, u32* out, u64 count, u32* d_total, Workspace& ws,
+                        hipStream_t s) {
+    exclusive_scan<u32>(in, out, count, d_total, ws, s);
+}
+
+void exclusive_scan_u64(const u64* in, u64* out, u64 count, u64* d_total, Workspace& ws,
+                        hipStream_t s) {
+    exclusive_scan<u64>(in, out, count, d_total, ws, s);
+}
+
+void radix_sort_u64(u64* keys, u32* vals, u64 count, int lo_bit, int hi_bit, u64* keys_alt,
+                    u32* vals_alt, Workspace& ws, hipStream_t s) {
+    if (count <= 1 || hi_bit <= lo_bit) return;
+    if (count >= (u64(1) << 32)) fail(TT_ELENGTH, "radix_sort: too many keys for one sort");
+    const u32 nblocks = u32((count + RTILE - 1) / RTILE);
+    u32* hist = ws.get<u32>(u64(RADIX) * nblocks);
+    u64 *ks = keys, *kd = keys_alt;
+    u32 *vs = vals, *vd = vals_alt;
+    int passes = 0;
+    for (int shift = lo_bit; shift < hi_bit; shift += 8, ++passes) {
+        const u32 dbits = u32(std::min(8, hi_bit - shift));
+        const u32 dmask = (1u << dbits) - 1u;
+        rs_upsweep<<<nblocks, RB, 0, s>>>(ks, count, u32(shift), dmask, hist, nblocks);
+        TTG_LAUNCH_CHECK();
+        exclusive_scan_u32(hist, hist, u64(RADIX) * nblocks, nullptr, ws, s);
+        if (vals)
+            rs_downsweep<true><<<nblocks, RB, 0, s>>>(ks, vs, count, u32(shift), dmask, dbits,
+                                                      hist, nblocks, kd, vd);
+        else
+            rs_downsweep<false><<<nblocks, RB, 0, s>>>(ks, nullptr, count, u32(shift), dmask,
+                                                       dbits, hist, nblocks, kd, nullptr);
+        TTG_LAUNCH_CHECK();
+        std::swap(ks, kd);
+        std::swap(vs, vd);
+    }
+    if (ks != keys) {
+        TTG_HIP(hipMemcpyAsync(keys, ks, count * sizeof(u64), hipMemcpyDeviceToDevice, s));
+        if (vals) TTG_HIP(hipMemcpyAsync(vals, vs, count * sizeof(u32), hipMemcpyDeviceToDevice, s));
+    }
+}
+
+}  // namespace ttg

@@ -1,0 +1,44 @@
+// SPDX-License-Identifier: MIT
+// sort.hpp — device primitives shared by the front-end kernels: exclusive
+// prefix sums and a stable LSD radix sort of 64-bit keys (optional 32-bit
+// payload). All stream-ordered; temporaries come from a Workspace.
+#pragma once
+
+#include <vector>
+
+#include "common.hpp"
+
+namespace ttg {
+
+// A grow-only bag of device allocations released together.
+struct Workspace {
+    std::vector<void*> ptrs;
+    ~Workspace() {
+        for (void* p : ptrs) (void)hipFree(p);
+    }
+    template <typename T>
+    T* get(u64 count) {
+        void* p = nullptr;
+        if (count) TTG_HIP(hipMalloc(&p, count * sizeof(T)));
+        ptrs.push_back(p);
+        return static_cast<T*>(p);
+    }
+};
+
+// out[i] = sum_{t < i} in[i]; out may alias in. Returns nothing; the total
+// is written to *d_total when non-null.
+void exclusive_scan_u32(const u32* in, u32* out, u64 count, u32* d_total, Workspace& ws,
+                        hipStream_t s);
+void exclusive_scan_u64(const u64* in, u64* out, u64 count, u64* d_total, Workspace& ws,
+                        hipStream_t s);
+
+// Stable ascending sort of keys[0, count) by bits [lo_bit, hi_bit) carrying
+// vals (may be null). Uses keys_alt / vals_alt as ping-pong buffers; the
+// sorted data ends up in keys / vals. count < 2^32.
+void radix_sort_u64(u64* keys, u32* vals, u64 count, int lo_bit, int hi_bit, u64* keys_alt,
+                    u32* vals_alt, Workspace& ws, hipStream_t s);
+
+// Bits needed to represent v (0 for v == 0).
+inline int bit_length(u64 v) { return v ? 64 - __builtin_clzll(v) : 0; }
+
+}  // namespace ttg

@@ -94,3 +94,32 @@ def dist_matrix_dev(x, metric="l2", layout="full", out=None, rowmax=None, radius
     capi.lib().call("tt_dist_matrix_dev", _dptr(x), n, d, kind, p, lay, _dptr(out), _dptr(rowmax),
                     _dptr(radius), _stream(stream))
     return out
+
+
+def _take(lib, ptr, count, ctype, dtype):
+    """Copy a library-malloc'd host buffer into numpy and release it."""
+    if not ptr:
+        return np.zeros(0, dtype)
+    arr = np.ctypeslib.as_array(ptr, shape=(max(count, 1),))[:count].astype(dtype, copy=True)
+    lib.dll.tt_free(C.cast(ptr, C.c_void_p))
+    return arr
+
+
+def dense_edges(x, metric="l2", threshold=math.inf, apparent=True):
+    """Reducer::dim0's kept edges, sort and apparent test on the dense full
+    complex (rips.hpp:685-849). Returns (idx u64, d f32, apparent u8 or None,
+    threshold), edges ordered by (diameter asc, index desc)."""
+    x = _f32(x)
+    n, d = x.shape
+    kind, p = _metric(metric)
+    lib = capi.lib()
+    m = C.c_uint64()
+    pi, pd, pa = C.POINTER(C.c_uint64)(), C.POINTER(C.c_float)(), C.POINTER(C.c_uint8)()
+    thr = C.c_float()
+    lib.call("tt_dense_edges", _addr(x), n, d, kind, p, C.c_float(threshold), C.byref(m), C.byref(pi),
+             C.byref(pd), C.byref(pa) if apparent else None, C.byref(thr))
+    k = m.value
+    idx = _take(lib, pi, k, C.c_uint64, np.uint64)
+    dd = _take(lib, pd, k, C.c_float, np.float32)
+    ap = _take(lib, pa, k, C.c_uint8, np.uint8) if apparent else None
+    return idx, dd, ap, thr.value
