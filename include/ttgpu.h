@@ -80,6 +80,49 @@ tt_status tt_dist_matrix_dev(const float* d_pts, uint32_t n, uint32_t dim, int k
                              int layout, float* d_mat, uint32_t* d_rowmax, float* d_radius,
                              void* stream);
 
+/* ---- neighbourhood graphs ----------------------------------------------- */
+
+/* radius_graph(PointCloud, tau) (graph.hpp:170-201) with
+ * CsrGraph::from_rows (core.hpp:113-134): every pair with d <= tau, each edge
+ * in both rows, rows sorted by neighbour id, no self loops. row_ptr: n + 1
+ * entries (caller's); *col and *w (*nnz = 2m entries) are malloc'd by the
+ * library, release with tt_free. */
+tt_status tt_radius_graph(const float* pts, uint32_t n, uint32_t dim, int kind, float p,
+                          float tau, uint64_t* row_ptr, uint64_t* nnz, uint32_t** col, float** w);
+
+/* knn_lists_exact(PointCloud, k) (graph.hpp:211-225): per point the k
+ * nearest j != i by (distance, id), ascending; k is clamped to n - 1 and
+ * written to *k_out. ids: n * k_out uint32 (caller's). k <= 64. */
+tt_status tt_knn_exact(const float* pts, uint32_t n, uint32_t dim, int kind, float p, uint32_t k,
+                       uint32_t* k_out, uint32_t* ids);
+
+/* knn_graph(PointCloud, KnnLists) (graph.hpp:286-304): the symmetric union
+ * of the lists (ids: n * k, k already clamped) with recomputed weights, as
+ * CSR; output buffers as tt_radius_graph. */
+tt_status tt_knn_graph(const float* pts, uint32_t n, uint32_t dim, int kind, float p,
+                       uint32_t k, const uint32_t* ids, uint64_t* row_ptr, uint64_t* nnz,
+                       uint32_t** col, float** w);
+
+/* knn_graph(cloud, knn_lists_exact(cloud, k)) as persistence() calls them
+ * (turbotops.hpp:275-277), with the lists kept in HBM. k == 0 -> TT_EINVAL
+ * "persistence: k must be >= 1" (turbotops.hpp:174). */
+tt_status tt_knn_graph_exact(const float* pts, uint32_t n, uint32_t dim, int kind, float p,
+                             uint32_t k, uint64_t* row_ptr, uint64_t* nnz, uint32_t** col,
+                             float** w);
+
+/* ---- H0 ------------------------------------------------------------------ */
+
+/* mst::h0(PointCloud, threshold) (mst.hpp:25-99): the n - 1 sorted MST edge
+ * weights (the H0 deaths) by Boruvka; bars follow exactly as mst.hpp:91-97.
+ * Non-finite coordinates -> TT_EINVAL "mst: non-finite coordinate". */
+tt_status tt_mst_deaths(const float* pts, uint32_t n, uint32_t dim, int kind, float p,
+                        float* deaths);
+
+/* Device form; *rounds and *scans (may be NULL) report the Boruvka rounds
+ * and the number of vertex scans (each over n - 1 candidates). */
+tt_status tt_mst_deaths_dev(const float* d_pts, uint32_t n, uint32_t dim, int kind, float p,
+                            float* d_deaths, uint32_t* rounds, uint64_t* scans, void* stream);
+
 /* ---- dimension 0 of the coboundary reducer -------------------------------- */
 
 /* Reducer::dim0's kept edges and sort (rips.hpp:685-839) and its parallel

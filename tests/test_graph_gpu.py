@@ -1,0 +1,160 @@
+"""GPU parity of the neighbourhood-graph and H0 kernels against the oracle:
+radius_graph (graph.hpp:170-201), knn_lists_exact (:211-225), knn_graph
+(:286-304), mst::h0 (mst.hpp:25-99). Index work bit-exact; weights bit-exact
+for L1, L2 and L-inf, within 2 ulp for Lp."""
+import numpy as np
+import pytest
+
+from conftest import bits, ulp_diff
+
+pytestmark = pytest.mark.gpu
+
+METRICS = ["l1", "l2", "linf", "lp:1.5", "lp:4"]
+
+
+def _cloud(n, d, seed, kind="gauss"):
+    rng = np.random.default_rng(seed)
+    if kind == "grid":
+        return rng.integers(0, 4, size=(n, d)).astype(np.float32)
+    if kind == "unif":
+        return rng.uniform(size=(n, d)).astype(np.float32)
+    return rng.normal(size=(n, d)).astype(np.float32)
+
+
+def _same_w(a, b, metric):
+    if metric in ("l1", "l2", "linf"):
+        return np.array_equal(bits(a), bits(b))
+    return a.size == b.size and (a.size == 0 or ulp_diff(a, b).max() <= 2)
+
+
+@pytest.mark.parametrize("metric", METRICS)
+def test_reference_selftest_case(gpu, restatement, metric):
+    """The reference's own check (src/turbotops.cpp:418-441): n=300, d=4
+    gaussian, exact kNN k=7 and radius graph tau=1.2 against brute force."""
+    from turbotops_amd import knn_lists_exact, radius_graph
+    x = _cloud(300, 4, 11)
+    print(f"case selftest metric={metric}", flush=True)
+    ids = knn_lists_exact(x, metric, 7)
+    if metric in ("l1", "l2", "linf"):
+        assert np.array_equal(ids, restatement.knn(x, metric, 7))
+    g = radius_graph(x, metric, 1.2)
+    rp, col, w = restatement.radius_graph(x, metric, 1.2)
+    assert g.valid() and np.array_equal(g.row_ptr, rp) and np.array_equal(g.col_idx, col)
+    assert _same_w(g.weights, w, metric)
+    mat = restatement.make_dense(x, metric)
+    assert g.m == int((mat[np.triu_indices(300, 1)] <= np.float32(1.2)).sum())
+
+
+@pytest.mark.parametrize("metric", ["l1", "l2", "linf"])
+@pytest.mark.parametrize("n,d,kind,k", [(2, 3, "gauss", 5), (50, 3, "grid", 7), (257, 2, "grid", 8),
+                                        (300, 8, "unif", 15), (130, 16, "gauss", 10), (90, 5, "gauss", 33),
+                                        (200, 3, "gauss", 64), (70, 17, "gauss", 4), (1, 3, "gauss", 3)])
+def test_knn_and_knn_graph(gpu, restatement, metric, n, d, kind, k):
+    from turbotops_amd import knn_graph, knn_graph_exact, knn_lists_exact
+    x = _cloud(n, d, n + d, kind)
+    print(f"case knn metric={metric} n={n} d={d} k={k}", flush=True)
+    ids = knn_lists_exact(x, metric, k)
+    want = restatement.knn(x, metric, k)
+    assert ids.shape == want.shape and np.array_equal(ids, want)
+    g = knn_graph(x, metric, ids)
+    rp, col, w = restatement.knn_graph(x, metric, want)
+    assert np.array_equal(g.row_ptr, rp) and np.array_equal(g.col_idx, col)
+    assert np.array_equal(bits(g.weights), bits(w))
+    g2 = knn_graph_exact(x, metric, k)
+    assert np.array_equal(g2.row_ptr, rp) and np.array_equal(g2.col_idx, col)
+    assert g.valid()
+
+
+@pytest.mark.parametrize("metric", ["lp:1.5", "lp:3"])
+def test_knn_lp(gpu, restatement, metric):
+    """Lp distances may differ from glibc powf by an ulp; lists must agree
+    wherever the oracle's k-th and (k+1)-th distances are separated by more."""
+    from turbotops_amd import knn_lists_exact
+    x = _cloud(150, 5, 3)
+    ids = knn_lists_exact(x, metric, 9)
+    mat = restatement.make_dense(x, metric)
+    want = restatement.knn(x, metric, 9)
+    for i in range(150):
+        row = np.sort(np.delete(mat[i], i))
+        if ulp_diff(row[8], row[9]) > 4:
+            assert set(ids[i]) == set(want[i])
+
+
+@pytest.mark.parametrize("metric", METRICS)
+@pytest.mark.parametrize("n,d,kind,tau", [(2, 3, "gauss", 10.0), (64, 3, "grid", 1.0), (65, 3, "grid", 2.0),
+                                          (300, 8, "unif", 0.6), (500, 2, "gauss", 0.2), (120, 17, "gauss", 4.0),
+                                          (150, 3, "gauss", 1e30), (40, 3, "gauss", 0.0)])
+def test_radius_graph(gpu, restatement, metric, n, d, kind, tau):
+    from turbotops_amd import radius_graph
+    x = _cloud(n, d, 7 * n + d, kind)
+    print(f"case radius metric={metric} n={n} d={d} tau={tau}", flush=True)
+    g = radius_graph(x, metric, tau)
+    rp, col, w = restatement.radius_graph(x, metric, tau)
+    if metric in ("l1", "l2", "linf"):
+        assert np.array_equal(g.row_ptr, rp) and np.array_equal(g.col_idx, col)
+        assert np.array_equal(bits(g.weights), bits(w))
+    else:   # an ulp at the threshold may move an edge; away from it the sets agree
+        assert abs(int(g.row_ptr[-1]) - int(rp[-1])) <= 2
+    assert g.valid()
+
+
+def test_radius_graph_large(gpu, restatement):
+    """Edge count beyond the first buffer guess (complete graph, n = 2000):
+    the retry path, then CSR invariants and sampled rows."""
+    from turbotops_amd import radius_graph
+    x = _cloud(2000, 3, 5)
+    g = radius_graph(x, "l2", 1e30)
+    assert g.m == 2000 * 1999 // 2
+    rp = g.row_ptr
+    assert np.all(np.diff(rp.astype(np.int64)) == 1999)
+    for v in (0, 1, 777, 1999):
+        cols = g.col_idx[rp[v]:rp[v + 1]]
+        assert np.array_equal(cols, np.delete(np.arange(2000), v))
+        want = np.array([restatement.dist("l2", x[v], x[u]) for u in cols], np.float32)
+        assert np.array_equal(bits(g.weights[rp[v]:rp[v + 1]]), bits(want))
+
+
+@pytest.mark.parametrize("metric", METRICS)
+@pytest.mark.parametrize("n,d,kind", [(2, 3, "gauss"), (3, 2, "grid"), (100, 3, "gauss"), (257, 2, "grid"),
+                                      (600, 8, "unif"), (333, 17, "gauss"), (1000, 3, "grid")])
+def test_mst_deaths(gpu, restatement, metric, n, d, kind):
+    from turbotops_amd import mst_deaths
+    x = _cloud(n, d, 3 * n + d, kind)
+    print(f"case mst metric={metric} n={n} d={d} {kind}", flush=True)
+    got = mst_deaths(x, metric)
+    want = restatement.mst_deaths(x, metric)
+    if metric in ("l1", "l2", "linf"):
+        assert np.array_equal(bits(got), bits(want))
+    else:
+        assert ulp_diff(got, want).max() <= 2
+
+
+def test_mst_h0_bars_and_threshold(gpu, reference):
+    """Bars against the compiled reference's mst::h0, with and without a cut
+    (src/turbotops.cpp:529-547)."""
+    from turbotops_amd import mst_h0
+    x = _cloud(400, 3, 13)
+    for thr in [np.inf, 0.1, 0.05, 0.0]:
+        b, d = mst_h0(x, "l2", thr)
+        _, rb, rd = reference.mst_h0(x, "l2", thr)
+        assert np.array_equal(bits(d), bits(rd)) and np.array_equal(b, rb)
+
+
+def test_mst_rejects_non_finite(gpu):
+    from turbotops_amd import mst_deaths
+    x = _cloud(10, 3, 1)
+    x[3, 1] = np.nan
+    with pytest.raises(ValueError, match="mst: non-finite coordinate"):
+        mst_deaths(x, "l2")
+
+
+def test_mst_full_size_property(gpu, restatement):
+    """n = 100000 uniform in [0,1]^3: n - 1 sorted non-negative deaths, and
+    an exact oracle match on a 6000-point prefix of the same cloud."""
+    from turbotops_amd import mst_deaths
+    rng = np.random.default_rng(4)
+    x = rng.uniform(size=(100000, 3)).astype(np.float32)
+    d = mst_deaths(x, "l2")
+    assert d.size == 99999 and np.all(np.diff(d) >= 0) and d[0] >= 0
+    small = x[:6000]
+    assert np.array_equal(bits(mst_deaths(small, "l2")), bits(restatement.mst_deaths(small, "l2")))

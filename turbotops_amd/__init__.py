@@ -7,11 +7,18 @@ hand-written HIP kernels behind the C ABI in include/ttgpu.h.
 """
 from . import capi  # noqa: F401
 from .frontend import (  # noqa: F401
+    CsrGraph,
     dense_edges,
     dense_layout,
     dist_matrix_dev,
+    knn_graph,
+    knn_graph_exact,
+    knn_lists_exact,
     make_dense,
+    mst_deaths,
+    mst_h0,
     parse_metric,
+    radius_graph,
 )
 
 __version__ = "0.1.0"

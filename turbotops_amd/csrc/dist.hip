@@ -25,150 +25,14 @@
 // NO_SIMD branches) step by step; the build uses -ffp-contract=off.
 #include <algorithm>
 
-#include "common.hpp"
+#include "pairs.hpp"
 
 namespace ttg {
 namespace {
 
 constexpr u32 TILE = 64;
-constexpr u32 ROWS = 16;      // rows per wave
+
 constexpr u32 BLOCK = 256;    // 4 wave64s
-constexpr u32 KC = 16;        // coordinates per register chunk (runtime dim)
-
-struct MetricArgs {
-    float p;       // Minkowski exponent
-    float inv_p;   // 1.0f / p in float, as the reference computes it (metric.hpp:206)
-    u32 ip;        // integral exponent for square-and-multiply
-};
-
-// ---- per-pair kernels, compile-time dimension -----------------------------
-// a: row point (wave-uniform, scalar loads); b: column point in VGPRs.
-template <int V, int DIM>
-__device__ __forceinline__ float pair_dist(const float* __restrict__ a, const float (&b)[DIM],
-                                           const MetricArgs& ma) {
-    if constexpr (V == MV_L2) {                       // metric.hpp:107-111
-        float s = 0.0f;
-#pragma unroll
-        for (int k = 0; k < DIM; ++k) {
-            const float d = a[k] - b[k];
-            s = __builtin_fmaf(d, d, s);
-        }
-        return __builtin_sqrtf(s);
-    } else if constexpr (V == MV_L1) {                // metric.hpp:157
-        float s = 0.0f;
-#pragma unroll
-        for (int k = 0; k < DIM; ++k) s += __builtin_fabsf(a[k] - b[k]);
-        return s;
-    } else {
-        float m = 0.0f;                               // metric.hpp:172
-#pragma unroll
-        for (int k = 0; k < DIM; ++k) m = fmax_ref(m, __builtin_fabsf(a[k] - b[k]));
-        if constexpr (V == MV_LINF) {
-            return m;
-        } else {                                      // metric.hpp:183-207
-            if (m == 0.0f) return 0.0f;
-            const float inv = 1.0f / m;
-            float s = 0.0f;
-#pragma unroll
-            for (int k = 0; k < DIM; ++k) {
-                const float v = __builtin_fabsf(a[k] - b[k]) * inv;
-                const float x = (v < 1.0f) ? v : 1.0f;
-                if constexpr (V == MV_LP_INT) s += ipow_ref(x, ma.ip);
-                else s += pow_f32(x, ma.p);
-            }
-            return m * pow_f32(s, ma.inv_p);
-        }
-    }
-}
-
-// ---- per-pair kernels, runtime dimension (chunks of KC coordinates) -------
-// The chunking keeps each pair's accumulation strictly sequential in k.
-template <int V>
-__device__ __forceinline__ void rows_runtime(const float* __restrict__ x, u32 n, u32 dim,
-                                             const MetricArgs& ma, u32 ibase, u32 jc,
-                                             float (&val)[ROWS]) {
-    const float* __restrict__ xb = x + u64(jc) * dim;
-    float acc[ROWS];
-#pragma unroll
-    for (u32 r = 0; r < ROWS; ++r) acc[r] = 0.0f;
-    // pass 1: L1 / L2 sums, or the L-inf maximum (also Lp's M)
-    for (u32 k0 = 0; k0 < dim; k0 += KC) {
-        const u32 len = min(KC, dim - k0);
-        float b[KC];
-#pragma unroll
-        for (u32 t = 0; t < KC; ++t) b[t] = t < len ? xb[k0 + t] : 0.0f;
-#pragma unroll
-        for (u32 r = 0; r < ROWS; ++r) {
-            const u32 i = min(ibase + r, n - 1);
-            const float* __restrict__ a = x + u64(i) * dim + k0;
-            float s = acc[r];
-#pragma unroll
-            for (u32 t = 0; t < KC; ++t) {
-                if (t < len) {   // uniform guard: no dynamic register indexing
-                    const float d = a[t] - b[t];
-                    if constexpr (V == MV_L2) s = __builtin_fmaf(d, d, s);
-                    else if constexpr (V == MV_L1) s += __builtin_fabsf(d);
-                    else s = fmax_ref(s, __builtin_fabsf(d));
-                }
-            }
-            acc[r] = s;
-        }
-    }
-    if constexpr (V == MV_L2) {
-#pragma unroll
-        for (u32 r = 0; r < ROWS; ++r) val[r] = __builtin_sqrtf(acc[r]);
-    } else if constexpr (V == MV_L1 || V == MV_LINF) {
-#pragma unroll
-        for (u32 r = 0; r < ROWS; ++r) val[r] = acc[r];
-    } else {
-        // pass 2 (Lp): sum of normalised powers, then M * s^(1/p)
-        float inv[ROWS], s2[ROWS];
-#pragma unroll
-        for (u32 r = 0; r < ROWS; ++r) {
-            inv[r] = acc[r] == 0.0f ? 0.0f : 1.0f / acc[r];
-            s2[r] = 0.0f;
-        }
-        for (u32 k0 = 0; k0 < dim; k0 += KC) {
-            const u32 len = min(KC, dim - k0);
-            float b[KC];
-#pragma unroll
-            for (u32 t = 0; t < KC; ++t) b[t] = t < len ? xb[k0 + t] : 0.0f;
-#pragma unroll
-            for (u32 r = 0; r < ROWS; ++r) {
-                const u32 i = min(ibase + r, n - 1);
-                const float* __restrict__ a = x + u64(i) * dim + k0;
-                float s = s2[r];
-#pragma unroll
-                for (u32 t = 0; t < KC; ++t) {
-                    if (t < len) {
-                        const float v = __builtin_fabsf(a[t] - b[t]) * inv[r];
-                        const float xx = (v < 1.0f) ? v : 1.0f;
-                        if constexpr (V == MV_LP_INT) s += ipow_ref(xx, ma.ip);
-                        else s += pow_f32(xx, ma.p);
-                    }
-                }
-                s2[r] = s;
-            }
-        }
-#pragma unroll
-        for (u32 r = 0; r < ROWS; ++r)
-            val[r] = acc[r] == 0.0f ? 0.0f : acc[r] * pow_f32(s2[r], ma.inv_p);
-    }
-}
-
-template <int V, int DIM>
-__device__ __forceinline__ void rows_static(const float* __restrict__ x, u32 n,
-                                            const MetricArgs& ma, u32 ibase, u32 jc,
-                                            float (&val)[ROWS]) {
-    float b[DIM];
-#pragma unroll
-    for (int k = 0; k < DIM; ++k) b[k] = x[u64(jc) * DIM + k];
-#pragma unroll
-    for (u32 r = 0; r < ROWS; ++r) {
-        const u32 i = min(ibase + r, n - 1);
-        val[r] = pair_dist<V, DIM>(x + u64(i) * DIM, b, ma);
-    }
-}
 
 // Linear block id -> lower-triangle tile (bi >= bj).
 __device__ __forceinline__ void tile_of(u64 b, u32& bi, u32& bj) {

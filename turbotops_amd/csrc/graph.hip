@@ -1,0 +1,496 @@
+// SPDX-License-Identifier: MIT
+// graph.hip — exact neighbourhood graphs over a point cloud.
+//
+// Replaces radius_graph (include/turbotops/graph.hpp:170-201),
+// knn_lists_exact / knn_query (graph.hpp:123-166, :211-225), knn_graph
+// (graph.hpp:286-304) and CsrGraph::from_rows (core.hpp:113-134). The
+// reference prunes with a k-d tree; pruning never changes its result, so an
+// exhaustive tiled scan reproduces it exactly.
+//
+//   radius graph  64x64 tiles of the lower triangle (as the distance matrix);
+//                 per wave and row a ballot of d(i, j) <= tau, one atomic per
+//                 ballot reserves room in an edge list, mbcnt places each edge.
+//                 The CSR comes from a radix sort of (row, col) keys carrying the
+//                 weight bits: rows sorted by neighbour id, as from_rows.
+//   kNN           one query per lane; candidate points stream through LDS in
+//                 tiles of 256; the k best (distance, id) pairs sit in VGPRs as
+//                 a sorted insertion network. Candidates arrive in ascending id,
+//                 so "strictly closer than the current k-th" is the reference's
+//                 (distance, id) lexicographic rule (graph.hpp:112, :145).
+//   kNN graph     both directions of every list entry as (row, col) keys, radix
+//                 sort, adjacent-duplicate removal, weights recomputed as
+//                 c(row, col) (graph.hpp:295-302).
+#include <algorithm>
+
+#include "pairs.hpp"
+#include "sort.hpp"
+
+namespace ttg {
+
+namespace {
+
+constexpr u32 GT = 64;       // tile edge
+constexpr u32 GB = 256;      // threads
+
+__device__ __forceinline__ void tile_of_g(u64 b, u32& bi, u32& bj) {
+    u32 t = u32((__builtin_sqrt(8.0 * double(b) + 1.0) - 1.0) * 0.5);
+    while (u64(t) * (t + 1) / 2 > b) --t;
+    while (u64(t + 1) * (t + 2) / 2 <= b) ++t;
+    bi = t;
+    bj = u32(b - u64(t) * (t + 1) / 2);
+}
+
+__device__ __forceinline__ u32 mbcnt64(u64 m) {
+    return __builtin_amdgcn_mbcnt_hi(u32(m >> 32), __builtin_amdgcn_mbcnt_lo(u32(m), 0u));
+}
+
+// Edge list of every pair j < i with d(i, j) <= tau; entries past `cap`
+// are counted but not stored.
+template <int V, int DIM>
+__global__ __launch_bounds__(GB) void radius_tiles(const float* __restrict__ x, u32 n, u32 dim_rt,
+                                                   MetricArgs ma, float tau, u64 cap,
+                                                   unsigned long long* __restrict__ counter,
+                                                   u32* __restrict__ ei, u32* __restrict__ ej,
+                                                   float* __restrict__ ew) {
+    u32 bi, bj;
+    tile_of_g(blockIdx.x, bi, bj);
+    const u32 lane = threadIdx.x & 63;
+    const u32 wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const u32 j = bj * GT + lane;
+    const u32 jc = j < n ? j : n - 1;
+    const u32 ibase = bi * GT + wave * ROWS;
+    float val[ROWS];
+    if constexpr (DIM > 0) rows_static<V, DIM>(x, n, ma, ibase, jc, val);
+    else rows_runtime<V>(x, n, dim_rt, ma, ibase, jc, val);
+#pragma unroll
+    for (u32 r = 0; r < ROWS; ++r) {
+        const u32 i = ibase + r;
+        const bool keep = i < n && j < i && val[r] <= tau;
+        const u64 b = __ballot(keep);
+        if (b) {
+            unsigned long long base = 0;
+            if (lane == 0) base = atomicAdd(counter, (unsigned long long)__popcll(b));
+            base = __shfl(base, 0);
+            if (keep) {
+                const u64 pos = base + mbcnt64(b);
+                if (pos < cap) {
+                    ei[pos] = i;
+                    ej[pos] = j;
+                    ew[pos] = val[r];
+                }
+            }
+        }
+    }
+}
+
+template <int V, int DIM>
+struct RadiusLaunch {
+    static void run(u32 blocks, hipStream_t s, const float* x, u32 n, u32 dim, MetricArgs ma,
+                    float tau, u64 cap, unsigned long long* counter, u32* ei, u32* ej, float* ew) {
+        radius_tiles<V, DIM><<<blocks, GB, 0, s>>>(x, n, dim, ma, tau, cap, counter, ei, ej, ew);
+    }
+};
+
+// Both directions of every edge as (row << cb | col) keys with the weight bits.
+__global__ void emit_both(const u32* __restrict__ ei, const u32* __restrict__ ej,
+                          const float* __restrict__ ew, u64 m, int cb, u64* __restrict__ keys,
+                          u32* __restrict__ vals, u32* __restrict__ deg) {
+    for (u64 t = blockIdx.x * u64(blockDim.x) + threadIdx.x; t < m; t += u64(gridDim.x) * blockDim.x) {
+        const u32 a = ei[t], b = ej[t];
+        const u32 w = __float_as_uint(ew[t]);
+        keys[2 * t] = (u64(a) << cb) | b;
+        vals[2 * t] = w;
+        keys[2 * t + 1] = (u64(b) << cb) | a;
+        vals[2 * t + 1] = w;
+        atomicAdd(deg + a, 1u);
+        atomicAdd(deg + b, 1u);
+    }
+}
+
+__global__ void split_keys(const u64* __restrict__ keys, const u32* __restrict__ vals, u64 m,
+                           u64 mask, u32* __restrict__ col, float* __restrict__ w) {
+    for (u64 t = blockIdx.x * u64(blockDim.x) + threadIdx.x; t < m; t += u64(gridDim.x) * blockDim.x) {
+        col[t] = u32(keys[t] & mask);
+        w[t] = __uint_as_float(vals[t]);
+    }
+}
+
+__global__ void widen_u32(const u32* __restrict__ in, u64* __restrict__ out, u64 count) {
+    for (u64 t = blockIdx.x * u64(blockDim.x) + threadIdx.x; t < count; t += u64(gridDim.x) * blockDim.x)
+        out[t] = in[t];
+}
+
+// ---- kNN --------------------------------------------------------------------
+constexpr u32 KT = 256;   // candidate tile (points)
+
+template <int V, int DIM, int KMAX>
+__global__ __launch_bounds__(GB) void knn_scan(const float* __restrict__ x, u32 n, u32 dim_rt,
+                                               MetricArgs ma, u32 k, u32* __restrict__ ids) {
+    constexpr u32 DS = DIM > 0 ? DIM : 1;
+    __shared__ float tile[DIM > 0 ? KT * DS : 1];
+    const u32 dim = DIM > 0 ? u32(DIM) : dim_rt;
+    const u32 i = blockIdx.x * GB + threadIdx.x;
+    const u32 qi = i < n ? i : n - 1;
+    float q[DS];
+    const float* __restrict__ qp = x + u64(qi) * dim;
+    if constexpr (DIM > 0) {
+#pragma unroll
+        for (int t = 0; t < DIM; ++t) q[t] = qp[t];
+    }
+    float bd[KMAX];
+    u32 bid[KMAX];
+#pragma unroll
+    for (int t = 0; t < KMAX; ++t) {
+        bd[t] = u32(t) < k ? INFINITY : -INFINITY;   // slots past k never change
+        bid[t] = 0xffffffffu;
+    }
+    float kth = INFINITY;       // current k-th (distance, id); id ~0 while slots are free
+    u32 kthj = 0xffffffffu;
+    for (u32 j0 = 0; j0 < n; j0 += KT) {
+        const u32 len = min(KT, n - j0);
+        if constexpr (DIM > 0) {
+            __syncthreads();
+            for (u32 t = threadIdx.x; t < len * DS; t += GB) tile[t] = x[u64(j0) * DS + t];
+            __syncthreads();
+        }
+        for (u32 jj = 0; jj < len; ++jj) {
+            const u32 j = j0 + jj;
+            float d;
+            if constexpr (DIM > 0) d = pdist<V, DIM>(q, tile + jj * DS, dim, ma);
+            else d = pdist<V, 0>(qp, x + u64(j) * dim, dim, ma);
+            if ((d < kth || (d == kth && j < kthj)) && j != qi) {
+                float cd = d;
+                u32 cj = j;
+#pragma unroll
+                for (int t = 0; t < KMAX; ++t) {
+                    const bool sw = cd < bd[t] || (cd == bd[t] && cj < bid[t]);
+                    const float td = bd[t];
+                    const u32 tj = bid[t];
+                    bd[t] = sw ? cd : td;
+                    bid[t] = sw ? cj : tj;
+                    cd = sw ? td : cd;
+                    cj = sw ? tj : cj;
+                }
+                float nk = bd[0];
+                u32 nj = bid[0];
+#pragma unroll
+                for (int t = 1; t < KMAX; ++t)
+                    if (u32(t) == k - 1) {
+                        nk = bd[t];
+                        nj = bid[t];
+                    }
+                kth = nk;
+                kthj = nj;
+            }
+        }
+    }
+    if (i < n) {
+#pragma unroll
+        for (int t = 0; t < KMAX; ++t)
+            if (u32(t) < k) ids[u64(i) * k + t] = bid[t];
+    }
+}
+
+template <int V, int DIM>
+struct KnnLaunch {
+    static void run(u32 kmax, u32 blocks, hipStream_t s, const float* x, u32 n, u32 dim,
+                    MetricArgs ma, u32 k, u32* ids) {
+        switch (kmax) {
+            case 8: knn_scan<V, DIM, 8><<<blocks, GB, 0, s>>>(x, n, dim, ma, k, ids); break;
+            case 16: knn_scan<V, DIM, 16><<<blocks, GB, 0, s>>>(x, n, dim, ma, k, ids); break;
+            case 32: knn_scan<V, DIM, 32><<<blocks, GB, 0, s>>>(x, n, dim, ma, k, ids); break;
+            default: knn_scan<V, DIM, 64><<<blocks, GB, 0, s>>>(x, n, dim, ma, k, ids); break;
+        }
+    }
+};
+
+// ---- kNN graph ----------------------------------------------------------------
+__global__ void emit_knn_keys(const u32* __restrict__ ids, u32 n, u32 k, int cb,
+                              u64* __restrict__ keys) {
+    const u64 total = u64(n) * k;
+    for (u64 t = blockIdx.x * u64(blockDim.x) + threadIdx.x; t < total; t += u64(gridDim.x) * blockDim.x) {
+        const u32 i = u32(t / k), j = ids[t];
+        if (j == i || j >= n) {
+            keys[2 * t] = keys[2 * t + 1] = ~0ull;   // sorts last, dropped
+        } else {
+            keys[2 * t] = (u64(i) << cb) | j;
+            keys[2 * t + 1] = (u64(j) << cb) | i;
+        }
+    }
+}
+
+__global__ void mark_unique(const u64* __restrict__ keys, u64 m, u32* __restrict__ flag) {
+    for (u64 t = blockIdx.x * u64(blockDim.x) + threadIdx.x; t < m; t += u64(gridDim.x) * blockDim.x)
+        flag[t] = keys[t] != ~0ull && (t == 0 || keys[t] != keys[t - 1]);
+}
+
+template <int V, int DIM>
+__global__ void knn_graph_fill(const float* __restrict__ x, u32 dim_rt, MetricArgs ma,
+                               const u64* __restrict__ keys, const u32* __restrict__ flag,
+                               const u32* __restrict__ pos, u64 m, int cb,
+                               u32* __restrict__ col, float* __restrict__ w,
+                               u32* __restrict__ deg) {
+    const u32 dim = DIM > 0 ? u32(DIM) : dim_rt;
+    const u64 mask = (u64(1) << cb) - 1;
+    for (u64 t = blockIdx.x * u64(blockDim.x) + threadIdx.x; t < m; t += u64(gridDim.x) * blockDim.x) {
+        if (!flag[t]) continue;
+        const u32 r = u32(keys[t] >> cb), c = u32(keys[t] & mask);
+        const u32 o = pos[t];
+        col[o] = c;
+        w[o] = pdist<V, DIM>(x + u64(r) * dim, x + u64(c) * dim, dim, ma);   // c(v, u), graph.hpp:301
+        atomicAdd(deg + r, 1u);
+    }
+}
+
+template <int V, int DIM>
+struct KnnGraphFill {
+    static void run(u32 blocks, hipStream_t s, const float* x, u32 dim, MetricArgs ma,
+                    const u64* keys, const u32* flag, const u32* pos, u64 m, int cb, u32* col,
+                    float* w, u32* deg) {
+        knn_graph_fill<V, DIM><<<blocks, GB, 0, s>>>(x, dim, ma, keys, flag, pos, m, cb, col, w, deg);
+    }
+};
+
+u32 blocks_for(u64 count, u32 per = GB) {
+    return u32(std::max<u64>(1, std::min<u64>((count + per - 1) / per, 65536)));
+}
+
+void check_cloud(const float* pts, u32 n, u32 dim, int kind, float p, const char* what) {
+    check_metric(kind, p);
+    check_points(pts, u64(n) * dim, what);
+    if (n && dim == 0) fail(TT_EINVAL, std::string(what) + ": dim must be >= 1");
+}
+
+}  // namespace
+
+void fill_device_u32(u32* p, u64 count, u32 v, hipStream_t s);
+
+// CSR over n vertices from keys (row << cb | col) sorted ascending; deg[] holds
+// the row counts. Writes row_ptr (n + 1) into ws and returns it.
+static u64* row_ptr_from_deg(u32* deg, u32 n, Workspace& ws, hipStream_t s) {
+    u64* rp = ws.get<u64>(u64(n) + 1);
+    widen_u32<<<blocks_for(n), GB, 0, s>>>(deg, rp, n);
+    TTG_LAUNCH_CHECK();
+    exclusive_scan_u64(rp, rp, u64(n) + 1, nullptr, ws, s);
+    return rp;
+}
+
+struct DevCsr {
+    u64 nnz = 0;
+    u64* row_ptr = nullptr;
+    u32* col = nullptr;
+    float* w = nullptr;
+};
+
+// ---- radius graph --------------------------------------------------------
+DevCsr radius_graph_dev(const float* d_x, u32 n, u32 dim, int kind, float p, float tau,
+                        Workspace& ws, hipStream_t s) {
+    DevCsr g;
+    u32* deg = ws.get<u32>(u64(n) + 1);
+    fill_device_u32(deg, u64(n) + 1, 0u, s);
+    if (n < 2) {
+        g.row_ptr = row_ptr_from_deg(deg, n, ws, s);
+        return g;
+    }
+    const MetricVariant v = variant_of(kind, p);
+    const MetricArgs ma = metric_args(kind, p);
+    const u64 nb = (u64(n) + GT - 1) / GT;
+    const u64 blocks = nb * (nb + 1) / 2;
+    if (blocks > 0x7fffffffull) fail(TT_ELENGTH, "radius_graph: n too large for one launch");
+    unsigned long long* counter = ws.get<unsigned long long>(1);
+    u64 cap = std::min<u64>(u64(n) * (n - 1) / 2, std::max<u64>(u64(1) << 20, u64(n) * 32));
+    u64 m = 0;
+    for (int attempt = 0; attempt < 2; ++attempt) {
+        u32* ei = ws.get<u32>(cap);
+        u32* ej = ws.get<u32>(cap);
+        float* ew = ws.get<float>(cap);
+        fill_device_u32(reinterpret_cast<u32*>(counter), 2, 0u, s);
+        dispatch_metric_dim<RadiusLaunch>(v, dim, u32(blocks), s, d_x, n, dim, ma, tau, cap, counter,
+                                          ei, ej, ew);
+        TTG_LAUNCH_CHECK();
+        TTG_HIP(hipMemcpyAsync(&m, counter, sizeof(u64), hipMemcpyDeviceToHost, s));
+        TTG_HIP(hipStreamSynchronize(s));
+        if (m <= cap) {
+            const int cb = std::max(1, bit_length(n - 1));
+            const u64 nnz = 2 * m;
+            u64* keys = ws.get<u64>(nnz);
+            u64* kalt = ws.get<u64>(nnz);
+            u32* vals = ws.get<u32>(nnz);
+            u32* valt = ws.get<u32>(nnz);
+            if (m) {
+                emit_both<<<blocks_for(m), GB, 0, s>>>(ei, ej, ew, m, cb, keys, vals, deg);
+                TTG_LAUNCH_CHECK();
+                radix_sort_u64(keys, vals, nnz, 0, 2 * cb, kalt, valt, ws, s);
+            }
+            g.nnz = nnz;
+            g.col = ws.get<u32>(nnz);
+            g.w = ws.get<float>(nnz);
+            if (nnz) {
+                split_keys<<<blocks_for(nnz), GB, 0, s>>>(keys, vals, nnz, (u64(1) << cb) - 1, g.col, g.w);
+                TTG_LAUNCH_CHECK();
+            }
+            g.row_ptr = row_ptr_from_deg(deg, n, ws, s);
+            return g;
+        }
+        cap = m;   // exact size known now
+    }
+    fail(TT_EDEVICE, "radius_graph: edge count changed between passes");
+}
+
+// ---- kNN --------------------------------------------------------------------
+u32 knn_dev(const float* d_x, u32 n, u32 dim, int kind, float p, u32 k, u32* d_ids, hipStream_t s) {
+    const u32 kk = n ? std::min<u32>(k, n - 1) : 0;
+    if (!kk) return 0;
+    if (kk > 64) fail(TT_EINVAL, "knn: k > 64 is not supported by the device kernel");
+    const u32 kmax = kk <= 8 ? 8 : kk <= 16 ? 16 : kk <= 32 ? 32 : 64;
+    const MetricVariant v = variant_of(kind, p);
+    const MetricArgs ma = metric_args(kind, p);
+    const u32 blocks = (n + GB - 1) / GB;
+    dispatch_metric_dim<KnnLaunch>(v, dim, kmax, blocks, s, d_x, n, dim, ma, kk, d_ids);
+    TTG_LAUNCH_CHECK();
+    return kk;
+}
+
+// ---- kNN graph --------------------------------------------------------------
+DevCsr knn_graph_dev(const float* d_x, u32 n, u32 dim, int kind, float p, u32 k, const u32* d_ids,
+                     Workspace& ws, hipStream_t s) {
+    DevCsr g;
+    u32* deg = ws.get<u32>(u64(n) + 1);
+    fill_device_u32(deg, u64(n) + 1, 0u, s);
+    const u64 m = 2 * u64(n) * k;
+    if (!m) {
+        g.row_ptr = row_ptr_from_deg(deg, n, ws, s);
+        return g;
+    }
+    const int cb = std::max(1, bit_length(n - 1));
+    u64* keys = ws.get<u64>(m);
+    u64* kalt = ws.get<u64>(m);
+    emit_knn_keys<<<blocks_for(u64(n) * k), GB, 0, s>>>(d_ids, n, k, cb, keys);
+    TTG_LAUNCH_CHECK();
+    radix_sort_u64(keys, nullptr, m, 0, std::min(64, 2 * cb + 1), kalt, nullptr, ws, s);
+    u32* flag = ws.get<u32>(m);
+    u32* pos = ws.get<u32>(m);
+    mark_unique<<<blocks_for(m), GB, 0, s>>>(keys, m, flag);
+    TTG_LAUNCH_CHECK();
+    u32* total = ws.get<u32>(1);
+    exclusive_scan_u32(flag, pos, m, total, ws, s);
+    u32 nnz = 0;
+    TTG_HIP(hipMemcpyAsync(&nnz, total, sizeof(u32), hipMemcpyDeviceToHost, s));
+    TTG_HIP(hipStreamSynchronize(s));
+    g.nnz = nnz;
+    g.col = ws.get<u32>(nnz);
+    g.w = ws.get<float>(nnz);
+    const MetricVariant v = variant_of(kind, p);
+    const MetricArgs ma = metric_args(kind, p);
+    dispatch_metric_dim<KnnGraphFill>(v, dim, blocks_for(m), s, d_x, dim, ma, keys, flag, pos, m, cb,
+                                      g.col, g.w, deg);
+    TTG_LAUNCH_CHECK();
+    g.row_ptr = row_ptr_from_deg(deg, n, ws, s);
+    return g;
+}
+
+}  // namespace ttg
+
+using namespace ttg;
+
+namespace {
+
+// Copies a device CSR to library-malloc'd host arrays (released by tt_free).
+void csr_to_host(const DevCsr& g, u32 n, uint64_t* row_ptr, uint64_t* nnz, uint32_t** col,
+                 float** w, hipStream_t s) {
+    *nnz = g.nnz;
+    TTG_HIP(hipMemcpyAsync(row_ptr, g.row_ptr, (u64(n) + 1) * sizeof(u64), hipMemcpyDeviceToHost, s));
+    *col = static_cast<uint32_t*>(std::malloc(std::max<u64>(1, g.nnz) * sizeof(u32)));
+    *w = static_cast<float*>(std::malloc(std::max<u64>(1, g.nnz) * sizeof(float)));
+    if (!*col || !*w) fail(TT_EDEVICE, "out of host memory");
+    if (g.nnz) {
+        TTG_HIP(hipMemcpyAsync(*col, g.col, g.nnz * sizeof(u32), hipMemcpyDeviceToHost, s));
+        TTG_HIP(hipMemcpyAsync(*w, g.w, g.nnz * sizeof(float), hipMemcpyDeviceToHost, s));
+    }
+    TTG_HIP(hipStreamSynchronize(s));
+}
+
+}  // namespace
+
+extern "C" {
+
+tt_status tt_radius_graph(const float* pts, uint32_t n, uint32_t dim, int kind, float p, float tau,
+                          uint64_t* row_ptr, uint64_t* nnz, uint32_t** col, float** w) {
+    return guarded([&] {
+        check_cloud(pts, n, dim, kind, p, "tt_radius_graph");
+        if (!row_ptr || !nnz || !col || !w) fail(TT_EINVAL, "tt_radius_graph: null output");
+        if (std::isnan(tau) || tau < 0.0f)
+            fail(TT_EINVAL, "persistence: threshold must be non-negative or +inf");
+        hipStream_t s = nullptr;
+        Workspace ws;
+        float* dx = ws.get<float>(u64(n) * dim);
+        if (u64(n) * dim)
+            TTG_HIP(hipMemcpyAsync(dx, pts, u64(n) * dim * sizeof(float), hipMemcpyHostToDevice, s));
+        const DevCsr g = radius_graph_dev(dx, n, dim, kind, p, tau, ws, s);
+        csr_to_host(g, n, row_ptr, nnz, col, w, s);
+    });
+}
+
+tt_status tt_knn_exact(const float* pts, uint32_t n, uint32_t dim, int kind, float p, uint32_t k,
+                       uint32_t* k_out, uint32_t* ids) {
+    return guarded([&] {
+        check_cloud(pts, n, dim, kind, p, "tt_knn_exact");
+        if (!k_out) fail(TT_EINVAL, "tt_knn_exact: null k_out");
+        const u32 kk = n ? std::min<u32>(k, n - 1) : 0;
+        *k_out = kk;
+        if (!kk) return;
+        if (!ids) fail(TT_EINVAL, "tt_knn_exact: null ids");
+        hipStream_t s = nullptr;
+        Workspace ws;
+        float* dx = ws.get<float>(u64(n) * dim);
+        u32* di = ws.get<u32>(u64(n) * kk);
+        TTG_HIP(hipMemcpyAsync(dx, pts, u64(n) * dim * sizeof(float), hipMemcpyHostToDevice, s));
+        knn_dev(dx, n, dim, kind, p, k, di, s);
+        TTG_HIP(hipMemcpyAsync(ids, di, u64(n) * kk * sizeof(u32), hipMemcpyDeviceToHost, s));
+        TTG_HIP(hipStreamSynchronize(s));
+    });
+}
+
+tt_status tt_knn_graph(const float* pts, uint32_t n, uint32_t dim, int kind, float p, uint32_t k,
+                       const uint32_t* ids, uint64_t* row_ptr, uint64_t* nnz, uint32_t** col,
+                       float** w) {
+    return guarded([&] {
+        check_cloud(pts, n, dim, kind, p, "tt_knn_graph");
+        if (!row_ptr || !nnz || !col || !w) fail(TT_EINVAL, "tt_knn_graph: null output");
+        if (u64(n) * k && !ids) fail(TT_EINVAL, "tt_knn_graph: null ids");
+        hipStream_t s = nullptr;
+        Workspace ws;
+        float* dx = ws.get<float>(u64(n) * dim);
+        u32* di = ws.get<u32>(u64(n) * k);
+        if (u64(n) * dim)
+            TTG_HIP(hipMemcpyAsync(dx, pts, u64(n) * dim * sizeof(float), hipMemcpyHostToDevice, s));
+        if (u64(n) * k)
+            TTG_HIP(hipMemcpyAsync(di, ids, u64(n) * k * sizeof(u32), hipMemcpyHostToDevice, s));
+        const DevCsr g = knn_graph_dev(dx, n, dim, kind, p, k, di, ws, s);
+        csr_to_host(g, n, row_ptr, nnz, col, w, s);
+    });
+}
+
+// knn_lists_exact + knn_graph in one device pass (the persistence() path,
+// turbotops.hpp:275-277): the lists never leave HBM.
+tt_status tt_knn_graph_exact(const float* pts, uint32_t n, uint32_t dim, int kind, float p,
+                             uint32_t k, uint64_t* row_ptr, uint64_t* nnz, uint32_t** col,
+                             float** w) {
+    return guarded([&] {
+        check_cloud(pts, n, dim, kind, p, "tt_knn_graph_exact");
+        if (k == 0) fail(TT_EINVAL, "persistence: k must be >= 1");
+        if (!row_ptr || !nnz || !col || !w) fail(TT_EINVAL, "tt_knn_graph_exact: null output");
+        hipStream_t s = nullptr;
+        Workspace ws;
+        const u32 kk = n ? std::min<u32>(k, n - 1) : 0;
+        float* dx = ws.get<float>(u64(n) * dim);
+        u32* di = ws.get<u32>(u64(n) * kk);
+        if (u64(n) * dim)
+            TTG_HIP(hipMemcpyAsync(dx, pts, u64(n) * dim * sizeof(float), hipMemcpyHostToDevice, s));
+        knn_dev(dx, n, dim, kind, p, k, di, s);
+        const DevCsr g = knn_graph_dev(dx, n, dim, kind, p, kk, di, ws, s);
+        csr_to_host(g, n, row_ptr, nnz, col, w, s);
+    });
+}
+
+}  // extern "C"

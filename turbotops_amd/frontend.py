@@ -123,3 +123,105 @@ def dense_edges(x, metric="l2", threshold=math.inf, apparent=True):
     dd = _take(lib, pd, k, C.c_float, np.float32)
     ap = _take(lib, pa, k, C.c_uint8, np.uint8) if apparent else None
     return idx, dd, ap, thr.value
+
+
+class CsrGraph:
+    """CsrGraph (include/turbotops/core.hpp:92-149): n vertices, m undirected
+    edges stored in both rows, rows sorted by neighbour id, no self loops."""
+
+    def __init__(self, n, row_ptr, col_idx, weights):
+        self.n = int(n)
+        self.row_ptr = np.asarray(row_ptr, np.uint64)
+        self.col_idx = np.asarray(col_idx, np.uint32)
+        self.weights = np.asarray(weights, np.float32)
+        self.m = int(self.row_ptr[-1]) // 2 if self.row_ptr.size else 0
+
+    def valid(self):
+        """CsrGraph::valid (core.hpp:137-148)."""
+        n = self.n
+        if self.row_ptr.size != n + 1 or self.row_ptr[0] != 0 or int(self.row_ptr[n]) != 2 * self.m:
+            return False
+        for v in range(n):
+            a, b = int(self.row_ptr[v]), int(self.row_ptr[v + 1])
+            cols = self.col_idx[a:b]
+            if np.any(cols >= n) or np.any(cols == v) or np.any(np.diff(cols.astype(np.int64)) <= 0):
+                return False
+            for t, u in enumerate(cols):
+                ua, ub = int(self.row_ptr[u]), int(self.row_ptr[u + 1])
+                k = np.searchsorted(self.col_idx[ua:ub], v)
+                if k >= ub - ua or self.col_idx[ua + k] != v or self.weights[ua + k] != self.weights[a + t]:
+                    return False
+        return True
+
+
+def _csr_call(name, n, *args):
+    lib = capi.lib()
+    rp = np.zeros(n + 1, np.uint64)
+    nnz = C.c_uint64()
+    pc, pw = C.POINTER(C.c_uint32)(), C.POINTER(C.c_float)()
+    lib.call(name, *args, _addr(rp) if rp.size else None, C.byref(nnz), C.byref(pc), C.byref(pw))
+    k = nnz.value
+    return CsrGraph(n, rp, _take(lib, pc, k, C.c_uint32, np.uint32), _take(lib, pw, k, C.c_float, np.float32))
+
+
+def radius_graph(x, metric, tau):
+    """radius_graph(PointCloud, tau) (graph.hpp:170-201)."""
+    x = _f32(x)
+    n, d = x.shape
+    kind, p = _metric(metric)
+    return _csr_call("tt_radius_graph", n, _addr(x), n, d, kind, p, C.c_float(tau))
+
+
+def knn_lists_exact(x, metric, k):
+    """knn_lists_exact(PointCloud, k) (graph.hpp:211-225): (n, min(k, n-1)) ids."""
+    x = _f32(x)
+    n, d = x.shape
+    kind, p = _metric(metric)
+    kk = min(k, n - 1) if n else 0
+    ids = np.zeros(max(n * kk, 1), np.uint32)
+    kout = C.c_uint32()
+    capi.lib().call("tt_knn_exact", _addr(x), n, d, kind, p, int(k), C.byref(kout), _addr(ids))
+    return ids[: n * kout.value].reshape(n, kout.value)
+
+
+def knn_graph(x, metric, ids):
+    """knn_graph(PointCloud, KnnLists) (graph.hpp:286-304)."""
+    x = _f32(x)
+    n, d = x.shape
+    kind, p = _metric(metric)
+    ids = np.ascontiguousarray(ids, np.uint32)
+    k = ids.shape[1] if ids.ndim == 2 else 0
+    return _csr_call("tt_knn_graph", n, _addr(x), n, d, kind, p, k, _addr(ids) if ids.size else None)
+
+
+def knn_graph_exact(x, metric, k):
+    """knn_graph(cloud, knn_lists_exact(cloud, k)) with the lists kept in HBM."""
+    x = _f32(x)
+    n, d = x.shape
+    kind, p = _metric(metric)
+    return _csr_call("tt_knn_graph_exact", n, _addr(x), n, d, kind, p, int(k))
+
+
+def mst_deaths(x, metric="l2"):
+    """The n - 1 sorted H0 deaths (MST edge weights) by Boruvka."""
+    x = _f32(x)
+    n, d = x.shape
+    kind, p = _metric(metric)
+    out = np.zeros(max(n - 1, 1), np.float32)
+    capi.lib().call("tt_mst_deaths", _addr(x), n, d, kind, p, _addr(out))
+    return out[: max(n - 1, 0)]
+
+
+def mst_h0(x, metric="l2", threshold=math.inf):
+    """mst::h0(PointCloud, threshold) (mst.hpp:25-99): H0 bars (births, deaths)
+    in the reference's order: finite deaths ascending, then the essential
+    classes (one, plus one per MST edge past the threshold)."""
+    deaths = mst_deaths(x, metric)
+    n = _f32(x).shape[0]
+    if n == 0:
+        return np.zeros(0, np.float32), np.zeros(0, np.float32)
+    thr = np.float32(threshold)
+    essential = 1 + int(np.count_nonzero(deaths > thr))
+    fin = deaths[(deaths <= thr) & (deaths > 0)]
+    d = np.concatenate([fin, np.full(essential, np.inf, np.float32)]).astype(np.float32)
+    return np.zeros_like(d), d
