@@ -144,6 +144,16 @@ tt_status tt_dense_edges_dev(const float* d_mat, uint32_t n, float threshold, in
                              uint64_t capacity, uint64_t* m, uint64_t* d_idx, float* d_d,
                              uint8_t* d_apparent, void* stream);
 
+/* The dense front end of rips::compute_dense and persistence()'s dense branch
+ * (rips.hpp:391-416 make_dense, :370-380 enclosing_radius, :1073 threshold,
+ * :685-849 dim0 sort and apparent test) in one device pass: mat (host, may be
+ * NULL) receives the matrix in `layout`; the edges, apparent flags and
+ * threshold are as tt_dense_edges. */
+tt_status tt_dense_filtration(const float* pts, uint32_t n, uint32_t dim, int kind, float p,
+                              float threshold, int layout, float* mat, uint64_t* m,
+                              uint64_t** idx, float** d, uint8_t** apparent,
+                              float* threshold_out);
+
 /* Releases a buffer the library allocated for a host-side result. */
 void tt_free(void* p);
 

@@ -1,0 +1,64 @@
+"""turbo_tops.persistence() on the MI355X front end.
+
+Same signature, coercions, return value and errors as the reference's
+Python entry point (python/turbo_tops/__init__.py:23-71): a point cloud
+(n, d) under a Minkowski metric, or any object with .tocsr() (a sparse
+weighted graph). Returns (dgms, info): dgms is a list of (m, 2) float32
+arrays of (birth, death), one per dimension 0..maxdim (death = inf for
+essential classes); info has method, threshold, edges, index_bits,
+seconds_graph, seconds_total and peak_rss_mb.
+"""
+import math
+
+import numpy as np
+
+from . import capi
+
+
+def _ext():
+    capi.lib()   # libttgpu.so must be there (the module links it)
+    from . import _turbotops
+    return _turbotops
+
+
+def persistence(x, maxdim=1, threshold=math.inf, metric="l2", graph="full", k=15,
+                approximate=False, recall_samples=0, threads=0, memory_budget=0):
+    """Persistence diagrams of the Vietoris-Rips filtration.
+
+    x: array (n, d) or scipy.sparse matrix (n, n); maxdim: highest homology
+    dimension; threshold: filtration cutoff (inclusive; with graph="full"
+    and threshold=inf the filtration stops at the enclosing radius, which
+    changes no bars); metric: "l1", "l2", "linf" or "lp:<p>", p >= 1;
+    graph: "full" or "knn"; k: neighbours for graph="knn"; approximate:
+    HNSW kNN (not part of this build: raises); threads: host worker
+    threads; memory_budget: bytes for the host copy of the dense matrix
+    (0 = half of RAM)."""
+    ext = _ext()
+    if hasattr(x, "tocsr"):
+        g = x.tocsr()
+        g.sort_indices()
+        return ext.persistence_csr(
+            np.ascontiguousarray(g.indptr, dtype=np.int64),
+            np.ascontiguousarray(g.indices, dtype=np.int64),
+            np.ascontiguousarray(g.data, dtype=np.float32),
+            int(g.shape[0]), int(maxdim), float(threshold), int(threads))
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    if x.ndim != 2:
+        raise ValueError("x must be a 2-d array (n, d)")
+    return ext.persistence_points(
+        x, int(maxdim), float(threshold), str(metric), str(graph), int(k),
+        bool(approximate), int(recall_samples), int(threads), int(memory_budget))
+
+
+def persistence_pairs(x, maxdim=1, threshold=math.inf, metric="l2", graph="full", k=15, threads=0,
+                      memory_budget=0):
+    """persistence() plus, per bar, the critical edges (i, j) of its birth
+    and death simplices: the edge whose length is the simplex's filtration
+    value ((-1, -1) where there is none). Returns (dgms, birth_edges,
+    death_edges, info)."""
+    ext = _ext()
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    if x.ndim != 2:
+        raise ValueError("x must be a 2-d array (n, d)")
+    return ext.persistence_pairs(x, int(maxdim), float(threshold), str(metric), str(graph), int(k),
+                                 int(threads), int(memory_budget))

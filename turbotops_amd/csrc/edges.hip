@@ -32,10 +32,6 @@ namespace {
 
 constexpr u32 EB = 256;
 
-__device__ __forceinline__ float at_full(const float* __restrict__ mat, u32 n, u32 i, u32 j) {
-    return mat[u64(i) * n + j];
-}
-
 // Largest v with v(v-1)/2 <= idx, and the edge (v, idx - v(v-1)/2), as the
 // reference's closed-form decode (rips.hpp:549-556).
 __device__ __forceinline__ void decode_edge(u64 idx, u32& i, u32& j) {
@@ -305,3 +301,66 @@ tt_status tt_dense_edges(const float* pts, uint32_t n, uint32_t dim, int kind, f
 void tt_free(void* p) { std::free(p); }
 
 }  // extern "C"
+
+// The whole dense front end of compute_dense / persistence's dense branch
+// (rips.hpp:391-416, :370-380, :1073, :685-849) in one device pass: the
+// matrix (copied to `mat` in the requested host layout), the enclosing radius
+// as threshold when `threshold` is +inf, the kept edges in filtration order
+// and their apparent flags.
+extern "C" tt_status tt_dense_filtration(const float* pts, uint32_t n, uint32_t dim, int kind,
+                                         float p, float threshold, int layout, float* mat,
+                                         uint64_t* m, uint64_t** idx, float** d,
+                                         uint8_t** apparent, float* threshold_out) {
+    using namespace ttg;
+    return guarded([&] {
+        check_metric(kind, p);
+        check_points(pts, u64(n) * dim, "tt_dense_filtration");
+        if (!m || !idx || !d) fail(TT_EINVAL, "tt_dense_filtration: null output");
+        if (std::isnan(threshold) || threshold < 0.0f)
+            fail(TT_EINVAL, "persistence: threshold must be non-negative or +inf");
+        if (layout != TT_LAYOUT_FULL && layout != TT_LAYOUT_LOWER)
+            fail(TT_EINVAL, "tt_dense_filtration: unknown layout");
+        *m = 0;
+        *idx = nullptr;
+        *d = nullptr;
+        if (apparent) *apparent = nullptr;
+        hipStream_t s = nullptr;
+        Workspace ws;
+        float* dx = ws.get<float>(u64(n) * dim);
+        float* dm = ws.get<float>(u64(n) * n);
+        u32* rowmax = ws.get<u32>(n);
+        float* dr = ws.get<float>(1);
+        if (u64(n) * dim)
+            TTG_HIP(hipMemcpyAsync(dx, pts, u64(n) * dim * sizeof(float), hipMemcpyHostToDevice, s));
+        dist_matrix(dx, n, dim, kind, p, TT_LAYOUT_FULL, dm, rowmax, dr, s);
+        float thr = threshold;
+        if (std::isinf(threshold)) {
+            TTG_HIP(hipMemcpyAsync(&thr, dr, sizeof(float), hipMemcpyDeviceToHost, s));
+            TTG_HIP(hipStreamSynchronize(s));
+        }
+        if (threshold_out) *threshold_out = thr;
+        const DenseEdges e = dense_edges(dm, n, thr, apparent != nullptr, ws, s);
+        if (mat) {
+            if (layout == TT_LAYOUT_FULL) {
+                if (u64(n) * n)
+                    TTG_HIP(hipMemcpyAsync(mat, dm, u64(n) * n * sizeof(float), hipMemcpyDeviceToHost, s));
+            } else {
+                const u64 cells = u64(n) * (n ? n - 1 : 0) / 2;
+                float* low = ws.get<float>(cells);
+                dist_matrix(dx, n, dim, kind, p, TT_LAYOUT_LOWER, low, nullptr, nullptr, s);
+                if (cells) TTG_HIP(hipMemcpyAsync(mat, low, cells * sizeof(float), hipMemcpyDeviceToHost, s));
+            }
+        }
+        *m = e.m;
+        *idx = static_cast<uint64_t*>(std::malloc(std::max<u64>(1, e.m) * sizeof(u64)));
+        *d = static_cast<float*>(std::malloc(std::max<u64>(1, e.m) * sizeof(float)));
+        if (apparent) *apparent = static_cast<uint8_t*>(std::malloc(std::max<u64>(1, e.m)));
+        if (!*idx || !*d || (apparent && !*apparent)) fail(TT_EDEVICE, "out of host memory");
+        if (e.m) {
+            TTG_HIP(hipMemcpyAsync(*idx, e.idx, e.m * sizeof(u64), hipMemcpyDeviceToHost, s));
+            TTG_HIP(hipMemcpyAsync(*d, e.d, e.m * sizeof(float), hipMemcpyDeviceToHost, s));
+            if (apparent) TTG_HIP(hipMemcpyAsync(*apparent, e.apparent, e.m, hipMemcpyDeviceToHost, s));
+        }
+        TTG_HIP(hipStreamSynchronize(s));
+    });
+}

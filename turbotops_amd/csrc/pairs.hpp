@@ -162,7 +162,7 @@ __device__ __forceinline__ float pdist(const float* __restrict__ a, const float*
     const u32 dim = DIM > 0 ? u32(DIM) : dim_rt;
     if constexpr (V == MV_L2) {
         float s = 0.0f;
-#pragma unroll (DIM > 0 ? DIM : 4)
+#pragma unroll
         for (u32 k = 0; k < dim; ++k) {
             const float d = a[k] - b[k];
             s = __builtin_fmaf(d, d, s);
@@ -171,12 +171,12 @@ __device__ __forceinline__ float pdist(const float* __restrict__ a, const float*
         else return __builtin_sqrtf(s);
     } else if constexpr (V == MV_L1) {
         float s = 0.0f;
-#pragma unroll (DIM > 0 ? DIM : 4)
+#pragma unroll
         for (u32 k = 0; k < dim; ++k) s += __builtin_fabsf(a[k] - b[k]);
         return s;
     } else {
         float m = 0.0f;
-#pragma unroll (DIM > 0 ? DIM : 4)
+#pragma unroll
         for (u32 k = 0; k < dim; ++k) m = fmax_ref(m, __builtin_fabsf(a[k] - b[k]));
         if constexpr (V == MV_LINF) {
             return m;
@@ -184,7 +184,7 @@ __device__ __forceinline__ float pdist(const float* __restrict__ a, const float*
             if (m == 0.0f) return 0.0f;
             const float inv = 1.0f / m;
             float s = 0.0f;
-#pragma unroll (DIM > 0 ? DIM : 2)
+#pragma unroll
             for (u32 k = 0; k < dim; ++k) {
                 const float v = __builtin_fabsf(a[k] - b[k]) * inv;
                 const float x = (v < 1.0f) ? v : 1.0f;

@@ -1,0 +1,398 @@
+// SPDX-License-Identifier: MIT
+// persistence.cpp — tops::persistence and the front-end functions of the
+// reference's API, with every data-parallel stage on the GPU (include/ttgpu.h)
+// and the residual reduction on the host (reducer.hpp).
+#include <sys/resource.h>
+#include <unistd.h>
+
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <cstdlib>
+#include <cstring>
+#include <thread>
+
+#include "turbotops_gpu.hpp"
+
+namespace tops {
+
+namespace {
+
+std::atomic<u32> g_threads{0};
+
+double since(std::chrono::steady_clock::time_point t) {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now() - t).count();
+}
+
+template <class T>
+std::vector<T> take(T* p, u64 count) {
+    std::vector<T> v;
+    if (p) {
+        v.assign(p, p + count);
+        tt_free(p);
+    }
+    return v;
+}
+
+template <class F>
+void parallel_for(u64 n, F&& fn) {
+    const u32 nt = u32(std::min<u64>(num_threads(), n));
+    if (nt <= 1) {
+        for (u64 i = 0; i < n; ++i) fn(i);
+        return;
+    }
+    std::atomic<u64> next{0};
+    std::vector<std::thread> pool;
+    for (u32 t = 0; t < nt; ++t)
+        pool.emplace_back([&] {
+            for (u64 i; (i = next.fetch_add(1024)) < n;)
+                for (u64 e = i; e < std::min(n, i + 1024); ++e) fn(e);
+        });
+    for (auto& th : pool) th.join();
+}
+
+// validate_options (turbotops.hpp:170-175)
+void validate_options(f32 threshold, u32 k) {
+    if (std::isnan(threshold) || threshold < 0.0f)
+        throw std::invalid_argument("persistence: threshold must be non-negative or +inf");
+    if (k == 0) throw std::invalid_argument("persistence: k must be >= 1");
+}
+
+// redzed::fits (redzed.hpp:1432-1441): 64-bit simplex indices up to maxdim + 2.
+bool boundary_fits(u32 n, u32 maxdim) {
+    try {
+        cob::Binomials<u64> b(std::min<u32>(n, 0x7fffffffu), maxdim + 2);
+        return true;
+    } catch (const std::overflow_error&) {
+        return false;
+    }
+}
+
+// The reference's select_mode (turbotops.hpp:121-162) keeps its checks and
+// errors; both of its engines give the same diagram, and this build reduces
+// with the cohomology engine in every mode.
+std::string select_mode(Mode mode, u32 n, u32 maxdim, double mean_degree) {
+    if (mode == Mode::Coboundary)
+        return "coboundary (--mode coboundary): cohomology reducer, maxdim <= 30, 64/128-bit indices, any input";
+    if (mode == Mode::Boundary) {
+        if (maxdim == 0)
+            return "boundary (--mode boundary) requested, but H0 runs as a minimum spanning tree and uses no "
+                   "reduction engine";
+        if (!boundary_fits(n, maxdim))
+            throw std::invalid_argument(
+                "boundary mode (RedZeD) needs every simplex index in 64 bits (C(n, maxdim + 2) < 2^64, "
+                "implied n < 2^31); this problem does not fit — use the default coboundary mode instead");
+        if (maxdim + 2 > 32) throw std::invalid_argument("redzed: maxdim must be <= 30");
+        return "boundary (--mode boundary): served by the cohomology reducer (identical diagram)";
+    }
+    if (maxdim == 0) return "auto: H0 only, minimum spanning tree (no reduction engine)";
+    (void)mean_degree;
+    return "auto: cohomology reducer on GPU front-end outputs";
+}
+
+// Runs the reducer from dimension 1 after dim0 on the given edges.
+template <class Cx>
+void reduce_all(const Cx& cx, u32 maxdim, const u64* idx, const f32* d, const u8* apparent, u64 m,
+                Barcode& out, Stats& st) {
+    if (maxdim + 2 > 32) throw std::invalid_argument("rips: maxdim must be <= 30");
+    out.ensure(maxdim);
+    if (cx.n == 0) return;
+    cob::Telemetry tel;
+    if (cob::index_fits<u64>(cx.n, maxdim)) {
+        cob::Cohomology<u64, Cx> r(cx, maxdim, out, tel);
+        r.dim0(idx, d, apparent, m);
+        r.run_from_dim1();
+    } else if (cob::index_fits<cob::u128>(cx.n, maxdim)) {
+        tel.wide_index = true;
+        cob::Cohomology<cob::u128, Cx> r(cx, maxdim, out, tel);
+        r.dim0(idx, d, apparent, m);
+        r.run_from_dim1();
+    } else {
+        throw std::overflow_error("rips: C(n, maxdim + 2) exceeds 128-bit simplex indices; lower maxdim");
+    }
+    st.edges = tel.edges;
+    st.wide_index = tel.wide_index;
+    st.columns = tel.columns;
+    st.apparent = tel.apparent;
+    st.emergent = tel.emergent;
+}
+
+// dim0 inputs of a sparse complex: kept edges (u < v, w <= threshold) in
+// filtration order and their apparent flags.
+template <class I>
+void sparse_edges(const cob::SparseView& cx, u32 maxdim, std::vector<u64>& idx, std::vector<f32>& d,
+                  std::vector<u8>& apparent) {
+    struct E {
+        u64 idx;
+        f32 d;
+    };
+    std::vector<E> edges;
+    for (u32 v = 0; v < cx.n; ++v)
+        for (u64 t = cx.row_ptr[v]; t < cx.row_ptr[v + 1]; ++t) {
+            const u32 u = cx.col[t];
+            if (u < v && cx.w[t] <= cx.threshold) edges.push_back(E{u64(v) * (v - 1) / 2 + u, cx.w[t]});
+        }
+    std::sort(edges.begin(), edges.end(),
+              [](const E& a, const E& b) { return a.d < b.d || (a.d == b.d && a.idx > b.idx); });
+    idx.resize(edges.size());
+    d.resize(edges.size());
+    for (size_t e = 0; e < edges.size(); ++e) {
+        idx[e] = edges[e].idx;
+        d[e] = edges[e].d;
+    }
+    apparent.assign(maxdim >= 1 ? edges.size() : 0, 0);
+    if (maxdim >= 1) {
+        cob::Diagram scratch;
+        cob::Telemetry tel;
+        const cob::Cohomology<I, cob::SparseView> r(cx, std::max<u32>(maxdim, 1), scratch, tel);
+        parallel_for(edges.size(), [&](u64 e) { apparent[e] = r.edge_apparent(idx[e], d[e]); });
+    }
+}
+
+}  // namespace
+
+// ---- globals --------------------------------------------------------------
+void set_num_threads(u32 t) { g_threads.store(t); }
+u32 num_threads() {
+    const u32 t = g_threads.load();
+    return t ? t : std::max<u32>(1, std::thread::hardware_concurrency());
+}
+u64 default_memory_budget() {
+    const long pages = sysconf(_SC_PHYS_PAGES), page = sysconf(_SC_PAGE_SIZE);
+    const u64 m = pages > 0 && page > 0 ? u64(pages) * u64(page) : 0;
+    return m ? m / 2 : (u64(8) << 30);
+}
+double peak_rss_mb() {
+    struct rusage ru;
+    getrusage(RUSAGE_SELF, &ru);
+    return double(ru.ru_maxrss) / 1024.0;
+}
+
+Mode parse_mode(const std::string& s) {
+    if (s == "auto") return Mode::Auto;
+    if (s == "boundary") return Mode::Boundary;
+    if (s == "coboundary") return Mode::Coboundary;
+    throw std::invalid_argument("mode must be auto, boundary or coboundary");
+}
+
+// ---- CsrGraph (core.hpp:92-149) ---------------------------------------------
+f32 CsrGraph::weight(u32 u, u32 v) const {
+    const u32* b = col_idx.data() + begin(u);
+    const u32* e = col_idx.data() + end(u);
+    const u32* it = std::lower_bound(b, e, v);
+    return (it != e && *it == v) ? weights[u64(it - col_idx.data())] : INFINITY;
+}
+
+CsrGraph CsrGraph::from_rows(std::vector<std::vector<std::pair<u32, f32>>>& rows) {
+    CsrGraph g;
+    g.n = u32(rows.size());
+    g.row_ptr.assign(u64(g.n) + 1, 0);
+    for (u32 v = 0; v < g.n; ++v) g.row_ptr[v + 1] = g.row_ptr[v] + rows[v].size();
+    const u64 nnz = g.row_ptr[g.n];
+    g.m = nnz / 2;
+    g.col_idx.resize(nnz);
+    g.weights.resize(nnz);
+    for (u32 v = 0; v < g.n; ++v) {
+        auto& r = rows[v];
+        std::sort(r.begin(), r.end());
+        u64 o = g.row_ptr[v];
+        for (const auto& e : r) {
+            g.col_idx[o] = e.first;
+            g.weights[o] = e.second;
+            ++o;
+        }
+    }
+    return g;
+}
+
+bool CsrGraph::valid() const {
+    if (row_ptr.size() != u64(n) + 1 || row_ptr[0] != 0 || row_ptr[n] != 2 * m) return false;
+    for (u32 v = 0; v < n; ++v)
+        for (u64 t = begin(v); t < end(v); ++t) {
+            const u32 u = col_idx[t];
+            if (u >= n || u == v) return false;
+            if (t > begin(v) && col_idx[t - 1] >= u) return false;
+            if (weight(u, v) != weights[t]) return false;
+        }
+    return true;
+}
+
+// ---- front-end functions ------------------------------------------------------
+static CsrGraph graph_from_abi(u32 n, std::vector<u64>& rp, u64 nnz, u32* col, f32* w) {
+    CsrGraph g;
+    g.n = n;
+    g.row_ptr.swap(rp);
+    g.col_idx = take(col, nnz);
+    g.weights = take(w, nnz);
+    g.m = nnz / 2;
+    return g;
+}
+
+CsrGraph radius_graph(const PointCloud& c, f32 tau) {
+    std::vector<u64> rp(u64(c.n) + 1, 0);
+    u64 nnz = 0;
+    u32* col = nullptr;
+    f32* w = nullptr;
+    check(tt_radius_graph(c.x, c.n, c.dim, c.metric.kind, c.metric.p, tau, rp.data(), &nnz, &col, &w));
+    return graph_from_abi(c.n, rp, nnz, col, w);
+}
+
+KnnLists knn_lists_exact(const PointCloud& c, u32 k) {
+    KnnLists L;
+    L.n = c.n;
+    L.k = c.n ? std::min<u32>(k, c.n - 1) : 0;
+    L.ids.resize(u64(L.n) * L.k);
+    u32 kk = 0;
+    check(tt_knn_exact(c.x, c.n, c.dim, c.metric.kind, c.metric.p, k, &kk, L.ids.data()));
+    return L;
+}
+
+CsrGraph knn_graph(const PointCloud& c, const KnnLists& L) {
+    std::vector<u64> rp(u64(c.n) + 1, 0);
+    u64 nnz = 0;
+    u32* col = nullptr;
+    f32* w = nullptr;
+    check(tt_knn_graph(c.x, c.n, c.dim, c.metric.kind, c.metric.p, L.k, L.ids.data(), rp.data(), &nnz,
+                       &col, &w));
+    return graph_from_abi(c.n, rp, nnz, col, w);
+}
+
+namespace mst {
+// mst::h0 (mst.hpp:25-99): bars from the sorted Boruvka deaths.
+Barcode h0(const PointCloud& c, f32 threshold) {
+    Barcode out;
+    out.ensure(0);
+    std::vector<f32> deaths(c.n > 1 ? c.n - 1 : 0);
+    check(tt_mst_deaths(c.x, c.n, c.dim, c.metric.kind, c.metric.p, deaths.data()));
+    if (c.n == 0) return out;
+    u64 essential = 1;
+    for (f32 d : deaths) {
+        if (d > threshold) ++essential;
+        else if (d > 0.0f) out.add(0, 0.0f, d, 0, 0);
+    }
+    for (u64 e = 0; e < essential; ++e) out.add(0, 0.0f, INFINITY, 0, cob::kNone<cob::u128>);
+    return out;
+}
+}  // namespace mst
+
+namespace rips {
+DenseComplex make_dense(const PointCloud& c, u64 budget) {
+    int layout = 0;
+    check(tt_dense_layout(c.n, budget, &layout));
+    DenseComplex cx;
+    cx.n = c.n;
+    cx.full = layout == TT_LAYOUT_FULL;
+    cx.mat.resize(dense_bytes(c.n, cx.full) / sizeof(f32));
+    check(tt_make_dense(c.x, c.n, c.dim, c.metric.kind, c.metric.p, layout, cx.mat.data(), &cx.radius));
+    return cx;
+}
+}  // namespace rips
+
+// ---- persistence ----------------------------------------------------------------
+Result persistence(const CsrGraph& g, u32 maxdim, f32 threshold, u32 threads, Mode mode) {
+    const auto t0 = std::chrono::steady_clock::now();
+    validate_options(threshold, 1);
+    if (!g.valid())
+        throw std::invalid_argument("persistence: CsrGraph invariant violated (see CsrGraph::valid)");
+    if (threads) set_num_threads(threads);
+    Result r;
+    r.stats.mode_reason = select_mode(mode, g.n, maxdim, g.n ? 2.0 * double(g.m) / g.n : 0.0);
+    r.stats.engine = "coboundary";
+    cob::SparseView cx;
+    cx.n = g.n;
+    cx.threshold = threshold;
+    cx.row_ptr = g.row_ptr.data();
+    cx.col = g.col_idx.data();
+    cx.w = g.weights.data();
+    std::vector<u64> idx;
+    std::vector<f32> d;
+    std::vector<u8> ap;
+    if (cob::index_fits<u64>(g.n, std::max<u32>(maxdim, 1))) sparse_edges<u64>(cx, maxdim, idx, d, ap);
+    else sparse_edges<cob::u128>(cx, maxdim, idx, d, ap);
+    reduce_all(cx, maxdim, idx.data(), d.data(), ap.empty() ? nullptr : ap.data(), idx.size(), r.barcode, r.stats);
+    r.barcode.ensure(maxdim);
+    r.stats.method = "sparse";
+    r.stats.threshold = threshold;
+    r.stats.seconds_total = since(t0);
+    r.stats.peak_rss_mb = peak_rss_mb();
+    return r;
+}
+
+Result persistence(const f32* points, u32 n, u32 dim, const Options& o) {
+    const auto t0 = std::chrono::steady_clock::now();
+    validate_options(o.threshold, o.graph == Options::Graph::Knn ? o.k : 1);
+    if (o.threads) set_num_threads(o.threads);
+    const u64 budget = o.memory_budget ? o.memory_budget : default_memory_budget();
+    const PointCloud cloud{points, n, dim, Metric::parse(o.metric)};
+    for (u64 i = 0; i < u64(n) * dim; ++i)
+        if (!std::isfinite(points[i])) throw std::invalid_argument("persistence: non-finite coordinate");
+    Result r;
+    Stats& st = r.stats;
+
+    if (o.graph == Options::Graph::Full && o.maxdim == 0) {
+        r.barcode = mst::h0(cloud, o.threshold);
+        st.method = "mst";
+        st.engine = "mst";
+        st.mode_reason = "H0 only: minimum spanning tree (no reduction engine)";
+        st.threshold = o.threshold;
+        st.edges = n ? n - 1 : 0;
+    } else if (o.graph == Options::Graph::Full && std::isinf(o.threshold)) {
+        st.mode_reason = select_mode(o.mode, n, o.maxdim, INFINITY);
+        st.engine = "coboundary";
+        if (o.maxdim + 2 > 32)
+            throw std::invalid_argument(o.mode == Mode::Boundary ? "redzed: maxdim must be <= 30"
+                                                                 : "rips: maxdim must be <= 30");
+        int layout = 0;
+        check(tt_dense_layout(n, budget, &layout));
+        std::vector<f32> mat(rips::dense_bytes(n, layout == TT_LAYOUT_FULL) / sizeof(f32));
+        u64 m = 0;
+        u64* pidx = nullptr;
+        f32* pd = nullptr;
+        u8* pap = nullptr;
+        f32 thr = INFINITY;
+        const auto tf = std::chrono::steady_clock::now();
+        check(tt_dense_filtration(points, n, dim, cloud.metric.kind, cloud.metric.p, INFINITY, layout,
+                                  mat.data(), &m, &pidx, &pd, o.maxdim >= 1 ? &pap : nullptr, &thr));
+        st.seconds_front = since(tf);
+        std::vector<u64> idx = take(pidx, m);
+        std::vector<f32> d = take(pd, m);
+        std::vector<u8> ap = take(pap, m);
+        cob::DenseView cx;
+        cx.n = n;
+        cx.threshold = thr;
+        cx.full = layout == TT_LAYOUT_FULL;
+        cx.mat = mat.data();
+        reduce_all(cx, o.maxdim, idx.data(), d.data(), ap.empty() ? nullptr : ap.data(), m, r.barcode, st);
+        st.method = "dense";
+        st.threshold = thr;
+    } else {
+        if (o.approximate) throw std::invalid_argument("persistence: built without HNSW support");
+        const auto tg = std::chrono::steady_clock::now();
+        CsrGraph g;
+        std::vector<u64> rp(u64(n) + 1, 0);
+        u64 nnz = 0;
+        u32* col = nullptr;
+        f32* w = nullptr;
+        if (o.graph == Options::Graph::Full)
+            check(tt_radius_graph(points, n, dim, cloud.metric.kind, cloud.metric.p, o.threshold, rp.data(),
+                                  &nnz, &col, &w));
+        else
+            check(tt_knn_graph_exact(points, n, dim, cloud.metric.kind, cloud.metric.p, o.k, rp.data(), &nnz,
+                                     &col, &w));
+        g = graph_from_abi(n, rp, nnz, col, w);
+        st.seconds_graph = since(tg);
+        const Result rg = persistence(g, o.maxdim, o.threshold, 0, o.mode);
+        r.barcode = rg.barcode;
+        const double sg = st.seconds_graph;
+        st = rg.stats;
+        st.seconds_graph = sg;
+        st.method = "sparse";
+        st.threshold = o.threshold;
+    }
+    r.barcode.ensure(o.maxdim);
+    st.seconds_total = since(t0);
+    st.peak_rss_mb = peak_rss_mb();
+    return r;
+}
+
+}  // namespace tops

@@ -1,0 +1,217 @@
+// SPDX-License-Identifier: MIT
+// pymodule.cpp — Python module _turbotops, the counterpart of the reference's
+// nanobind extension (python/turbotops_ext.cpp:20-120): persistence_points
+// and persistence_csr with the same arguments, return values and errors,
+// plus persistence_pairs, which also returns each bar's critical edges.
+#include <pybind11/numpy.h>
+#include <pybind11/pybind11.h>
+#include <pybind11/stl.h>
+
+#include <algorithm>
+
+#include "turbotops_gpu.hpp"
+
+namespace py = pybind11;
+using namespace tops;
+
+namespace {
+
+using F32Array = py::array_t<float, py::array::c_style | py::array::forcecast>;
+using I64Array = py::array_t<int64_t, py::array::c_style | py::array::forcecast>;
+
+// bars of each dimension as an (m, 2) float32 array (turbotops_ext.cpp:20-33)
+py::list to_python(const Barcode& bc) {
+    py::list out;
+    for (size_t d = 0; d < bc.birth.size(); ++d) {
+        const size_t m = bc.birth[d].size();
+        py::array_t<float> a({m, size_t(2)});
+        auto v = a.mutable_unchecked<2>();
+        for (size_t i = 0; i < m; ++i) {
+            v(i, 0) = bc.birth[d][i];
+            v(i, 1) = bc.death[d][i];
+        }
+        out.append(a);
+    }
+    return out;
+}
+
+// stats_dict (turbotops_ext.cpp:35-51)
+py::dict stats_dict(const Stats& s) {
+    py::dict d;
+    d["method"] = s.method;
+    d["threshold"] = s.threshold;
+    d["edges"] = s.edges;
+    d["index_bits"] = s.wide_index ? 128 : 64;
+    d["seconds_graph"] = s.seconds_graph;
+    d["seconds_total"] = s.seconds_total;
+    d["peak_rss_mb"] = s.peak_rss_mb;
+    d["seconds_front"] = s.seconds_front;
+    return d;
+}
+
+Options make_options(u32 maxdim, float threshold, const std::string& metric, const std::string& graph, u32 k,
+                     bool approximate, u64 recall_samples, u32 threads, u64 memory_budget) {
+    Options o;
+    o.maxdim = maxdim;
+    o.threshold = threshold;
+    o.metric = metric;
+    if (graph == "full") o.graph = Options::Graph::Full;
+    else if (graph == "knn") o.graph = Options::Graph::Knn;
+    else throw py::value_error("graph must be 'full' or 'knn'");
+    o.k = k;
+    o.approximate = approximate;
+    o.recall_samples = recall_samples;
+    o.threads = threads;
+    o.memory_budget = memory_budget;
+    return o;
+}
+
+Result run_points(F32Array x, const Options& o) {
+    if (x.ndim() != 2) throw py::value_error("x must be a 2-d array (n, d)");
+    const u32 n = u32(x.shape(0)), dim = u32(x.shape(1));
+    Result r;
+    {
+        py::gil_scoped_release release;
+        r = persistence(x.data(), n, dim, o);
+    }
+    return r;
+}
+
+py::tuple persistence_points(F32Array x, u32 maxdim, float threshold, const std::string& metric,
+                             const std::string& graph, u32 k, bool approximate, u64 recall_samples, u32 threads,
+                             u64 memory_budget) {
+    const Result r = run_points(x, make_options(maxdim, threshold, metric, graph, k, approximate, recall_samples,
+                                                threads, memory_budget));
+    return py::make_tuple(to_python(r.barcode), stats_dict(r.stats));
+}
+
+// persistence_csr (turbotops_ext.cpp:76-106): both directions of every
+// entry, the smallest weight of repeated entries, then persistence(CsrGraph).
+py::tuple persistence_csr(I64Array indptr, I64Array indices, F32Array data, u32 n, u32 maxdim, float threshold,
+                          u32 threads) {
+    if (size_t(indptr.shape(0)) != size_t(n) + 1 || indices.shape(0) != data.shape(0))
+        throw py::value_error("inconsistent CSR arrays");
+    std::vector<std::vector<std::pair<u32, f32>>> rows(n);
+    const int64_t* ip = indptr.data();
+    const int64_t* ix = indices.data();
+    const float* dv = data.data();
+    for (u32 i = 0; i < n; ++i)
+        for (int64_t t = ip[i]; t < ip[i + 1]; ++t) {
+            const int64_t j = ix[t];
+            const float w = dv[t];
+            if (j < 0 || j >= int64_t(n)) throw py::value_error("column index out of range");
+            if (!(w >= 0) || !std::isfinite(w)) throw py::value_error("weights must be finite and >= 0");
+            if (j == i) continue;
+            rows[i].emplace_back(u32(j), w);
+            rows[u32(j)].emplace_back(i, w);
+        }
+    for (auto& r : rows) {
+        std::sort(r.begin(), r.end());
+        r.erase(std::unique(r.begin(), r.end(), [](const auto& a, const auto& b) { return a.first == b.first; }),
+                r.end());
+    }
+    const CsrGraph g = CsrGraph::from_rows(rows);
+    Result r;
+    {
+        py::gil_scoped_release release;
+        r = persistence(g, maxdim, threshold, threads);
+    }
+    return py::make_tuple(to_python(r.barcode), stats_dict(r.stats));
+}
+
+// The longest edge of a simplex (ties: the first pair in vertex order), i.e.
+// the edge whose length is the simplex's filtration value.
+std::pair<int64_t, int64_t> critical_edge(const cob::Binomials<cob::u128>& B, u32 n, cob::u128 idx, u32 nv,
+                                          const float* x, u32 dim, const std::function<float(u32, u32)>& dist) {
+    u32 vd[34];
+    u32 top = n;
+    for (u32 i = 0; i < nv; ++i) {
+        const u32 k = nv - i;
+        u32 lo = k - 1, hi = top;
+        while (hi - lo > 1) {
+            const u32 mid = lo + (hi - lo) / 2;
+            if (B(mid, k) <= idx) lo = mid;
+            else hi = mid;
+        }
+        vd[i] = lo;
+        idx -= B(lo, k);
+        top = lo;
+    }
+    (void)x;
+    (void)dim;
+    float best = -1.0f;
+    std::pair<int64_t, int64_t> e{-1, -1};
+    for (u32 a = 0; a < nv; ++a)
+        for (u32 b = a + 1; b < nv; ++b) {
+            const float dd = dist(vd[a], vd[b]);
+            if (dd > best) {
+                best = dd;
+                e = {vd[a], vd[b]};
+            }
+        }
+    return e;
+}
+
+// persistence on a point cloud, returning per dimension the bars and, per
+// bar, the critical edges of its birth and death simplices ((-1, -1) where
+// there is none: H0 births, essential deaths).
+py::tuple persistence_pairs(F32Array x, u32 maxdim, float threshold, const std::string& metric,
+                            const std::string& graph, u32 k, u32 threads, u64 memory_budget) {
+    const Options o = make_options(maxdim, threshold, metric, graph, k, false, 0, threads, memory_budget);
+    if (o.graph == Options::Graph::Full && maxdim == 0)
+        throw py::value_error("persistence_pairs: maxdim 0 runs as an MST; use maxdim >= 1");
+    const Result r = run_points(x, o);
+    const u32 n = u32(x.shape(0)), dim = u32(x.shape(1));
+    const Metric mt = Metric::parse(metric);
+    // exact pair distances for the tie-break between equally long edges
+    std::vector<float> full(size_t(n) * n);
+    float radius = 0;
+    check(tt_make_dense(x.data(), n, dim, mt.kind, mt.p, TT_LAYOUT_FULL, full.data(), &radius));
+    auto dist = [&](u32 a, u32 b) { return full[size_t(a) * n + b]; };
+    cob::Binomials<cob::u128> B(n, maxdim + 2);
+    py::list bars = to_python(r.barcode), births, deaths;
+    for (size_t d = 0; d < r.barcode.birth.size(); ++d) {
+        const size_t m = r.barcode.birth[d].size();
+        py::array_t<int64_t> cb({m, size_t(2)}), cd({m, size_t(2)});
+        auto vb = cb.mutable_unchecked<2>();
+        auto vdd = cd.mutable_unchecked<2>();
+        for (size_t i = 0; i < m; ++i) {
+            std::pair<int64_t, int64_t> eb{-1, -1}, ed{-1, -1};
+            const cob::u128 bs = r.barcode.birth_simplex[d][i], ds = r.barcode.death_simplex[d][i];
+            if (d == 0) {
+                if (std::isfinite(r.barcode.death[d][i])) ed = critical_edge(B, n, ds, 2, x.data(), dim, dist);
+            } else {
+                eb = critical_edge(B, n, bs, u32(d) + 1, x.data(), dim, dist);
+                if (std::isfinite(r.barcode.death[d][i])) ed = critical_edge(B, n, ds, u32(d) + 2, x.data(), dim, dist);
+            }
+            vb(i, 0) = eb.first;
+            vb(i, 1) = eb.second;
+            vdd(i, 0) = ed.first;
+            vdd(i, 1) = ed.second;
+        }
+        births.append(cb);
+        deaths.append(cd);
+    }
+    return py::make_tuple(bars, births, deaths, stats_dict(r.stats));
+}
+
+}  // namespace
+
+PYBIND11_MODULE(_turbotops, m) {
+    m.doc() = "Vietoris-Rips persistent homology with an MI355X front end";
+    m.attr("__version__") = "0.1.0";
+    m.def("persistence_points", &persistence_points, py::arg("x"), py::arg("maxdim"), py::arg("threshold"),
+          py::arg("metric"), py::arg("graph"), py::arg("k"), py::arg("approximate"), py::arg("recall_samples"),
+          py::arg("threads"), py::arg("memory_budget"));
+    m.def("persistence_csr", &persistence_csr, py::arg("indptr"), py::arg("indices"), py::arg("data"),
+          py::arg("n"), py::arg("maxdim"), py::arg("threshold"), py::arg("threads"));
+    m.def("persistence_pairs", &persistence_pairs, py::arg("x"), py::arg("maxdim"), py::arg("threshold"),
+          py::arg("metric"), py::arg("graph"), py::arg("k"), py::arg("threads"), py::arg("memory_budget"));
+    py::register_exception_translator([](std::exception_ptr p) {
+        try {
+            if (p) std::rethrow_exception(p);
+        } catch (const std::length_error& e) {
+            PyErr_SetString(PyExc_ValueError, e.what());
+        }
+    });
+}

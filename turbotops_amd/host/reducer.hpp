@@ -1,0 +1,613 @@
+// SPDX-License-Identifier: MIT
+// reducer.hpp — host-side persistent cohomology of a Rips filtration from
+// front-end outputs computed on the GPU.
+//
+// The algorithm is Ripser's (U. Bauer, "Ripser: efficient computation of
+// Vietoris-Rips persistence barcodes", J. Appl. Comput. Topol. 2021), the one
+// the reference's coboundary engine implements (include/turbotops/rips.hpp:
+// 489-1032): simplices are addressed by the combinatorial number system;
+// columns are coboundaries enumerated on demand and reduced in a binary heap
+// with mod-2 cancellation; apparent pairs and emergent pairs skip reduction;
+// clearing drops every column that is already a pivot of the previous
+// dimension. Filtration order is (diameter ascending, index descending)
+// (rips.hpp:116-118). The diagram does not depend on the reduction strategy,
+// only on the filtration.
+//
+// Dimension 0 does not run here: the GPU hands over the kept edges in
+// filtration order with their apparent flags (edges.hip), and the union-find
+// below only splits them into H0 deaths and dimension-1 columns.
+//
+// Besides the bars, every finite pair records its birth and death simplices
+// (for the gradient of a diagram with respect to the points).
+#pragma once
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+namespace tops::cob {
+
+using u8 = uint8_t;
+using u32 = uint32_t;
+using u64 = uint64_t;
+using u128 = unsigned __int128;
+using f32 = float;
+
+template <class I>
+constexpr I kNone = ~I(0);
+
+// Bars per dimension in discovery order; birth/death simplex indices of each
+// bar (death simplex kNone for essential classes, birth = vertex for H0).
+struct Diagram {
+    std::vector<std::vector<f32>> birth, death;
+    std::vector<std::vector<u128>> birth_simplex, death_simplex;
+    void ensure(u32 d) {
+        if (birth.size() <= d) {
+            birth.resize(d + 1);
+            death.resize(d + 1);
+            birth_simplex.resize(d + 1);
+            death_simplex.resize(d + 1);
+        }
+    }
+    void add(u32 dim, f32 b, f32 e, u128 bs, u128 ds) {
+        ensure(dim);
+        birth[dim].push_back(b);
+        death[dim].push_back(e);
+        birth_simplex[dim].push_back(bs);
+        death_simplex[dim].push_back(ds);
+    }
+};
+
+struct Telemetry {
+    bool wide_index = false;
+    u64 edges = 0;
+    std::vector<u64> columns, apparent, emergent;
+};
+
+// C(i, k) for i <= n, k <= K with overflow detection.
+template <class I>
+struct Binomials {
+    u32 n = 0, K = 0;
+    std::vector<I> t;   // t[k * (n + 1) + i]
+    Binomials(u32 nn, u32 KK) : n(nn), K(KK), t(u64(nn + 1) * (KK + 1), I(0)) {
+        for (u32 i = 0; i <= n; ++i) {
+            t[i] = 1;
+            for (u32 k = 1; k <= std::min(i, K); ++k) {
+                I s;
+                if (__builtin_add_overflow(at(i - 1, k - 1), at(i - 1, k), &s))
+                    throw std::overflow_error("rips: simplex index overflow");
+                t[u64(k) * (n + 1) + i] = s;
+            }
+        }
+    }
+    I at(u32 i, u32 k) const { return k > K ? I(0) : t[u64(k) * (n + 1) + i]; }
+    I operator()(u32 i, u32 k) const { return at(i, k); }
+};
+
+template <class I>
+bool index_fits(u32 n, u32 maxdim) {
+    try {
+        Binomials<I> b(n, maxdim + 2);
+        return b(n, maxdim + 2) < kNone<I>;
+    } catch (const std::overflow_error&) {
+        return false;
+    }
+}
+
+// ---- complexes ----------------------------------------------------------------
+// Both expose n, threshold, dist(i, j) and
+//   cofacets(vd, k, diam, all, cap, emit): vertices w not in the simplex (vd:
+//   k vertices, descending) with max(diam, d(v, w) for v in vd) <= cap, in
+//   descending w (all == false: only w > vd[0]); emit(w, t, cd) with t the
+//   number of simplex vertices above w; emit returns false to stop.
+
+// Dense distances, full (n x n) or lower (i(i-1)/2 + j) layout.
+struct DenseView {
+    u32 n = 0;
+    f32 threshold = INFINITY;
+    bool full = true;
+    const f32* mat = nullptr;
+
+    f32 dist(u32 i, u32 j) const {
+        if (i == j) return 0.0f;
+        if (full) return mat[u64(i) * n + j];
+        if (i < j) std::swap(i, j);
+        return mat[u64(i) * (i - 1) / 2 + j];
+    }
+
+    template <class Emit>
+    void cofacets(const u32* vd, u32 k, f32 diam, bool all, f32 cap, Emit&& emit) const {
+        const u32 stop = all ? 0 : vd[0] + 1;
+        u32 t = 0;
+        if (full) {
+            constexpr u32 B = 64;
+            f32 mx[B];
+            for (u32 hi = n; hi > stop;) {
+                const u32 lo = hi - std::min<u32>(B, hi - stop);
+                const u32 len = hi - lo;
+                const f32* r0 = mat + u64(vd[0]) * n + lo;
+                for (u32 j = 0; j < len; ++j) mx[j] = r0[j];
+                for (u32 i = 1; i < k; ++i) {
+                    const f32* ri = mat + u64(vd[i]) * n + lo;
+                    for (u32 j = 0; j < len; ++j) mx[j] = mx[j] < ri[j] ? ri[j] : mx[j];
+                }
+                for (u32 j = len; j-- > 0;) {
+                    const f32 m = mx[j] < diam ? diam : mx[j];
+                    if (!(m <= cap)) continue;
+                    const u32 w = lo + j;
+                    while (t < k && vd[t] > w) ++t;
+                    if (t < k && vd[t] == w) continue;
+                    if (!emit(w, t, m)) return;
+                }
+                hi = lo;
+            }
+            return;
+        }
+        for (u32 wi = n; wi-- > stop;) {
+            const u32 w = wi;
+            while (t < k && vd[t] > w) ++t;
+            if (t < k && vd[t] == w) continue;
+            f32 cd = diam;
+            bool ok = true;
+            for (u32 i = 0; i < k; ++i) {
+                const f32 dv = dist(vd[i], w);
+                if (dv > cd) {
+                    cd = dv;
+                    if (!(cd <= cap)) {
+                        ok = false;
+                        break;
+                    }
+                }
+            }
+            if (ok && !(cd <= cap)) ok = false;
+            if (ok && !emit(w, t, cd)) return;
+        }
+    }
+};
+
+// A weighted graph in CSR form (rows sorted by neighbour id, both
+// directions, no loops): cofacets are common neighbours.
+struct SparseView {
+    u32 n = 0;
+    f32 threshold = INFINITY;
+    const u64* row_ptr = nullptr;
+    const u32* col = nullptr;
+    const f32* w = nullptr;
+
+    f32 dist(u32 i, u32 j) const {
+        if (i == j) return 0.0f;
+        const u32* b = col + row_ptr[i];
+        const u32* e = col + row_ptr[i + 1];
+        const u32* it = std::lower_bound(b, e, j);
+        return (it != e && *it == j) ? w[it - col] : INFINITY;
+    }
+
+    template <class Emit>
+    void cofacets(const u32* vd, u32 k, f32 diam, bool all, f32 cap, Emit&& emit) const {
+        constexpr u32 KM = 33;
+        u64 p[KM], lo[KM];
+        for (u32 i = 0; i < k; ++i) {
+            lo[i] = row_ptr[vd[i]];
+            const u64 e = row_ptr[vd[i] + 1];
+            if (e == lo[i]) return;
+            p[i] = e - 1;
+        }
+        const u32 stop = all ? 0 : vd[0] + 1;
+        for (;;) {
+            // the largest id present in every row at or below the cursors
+            u32 x = col[p[0]];
+            bool agree = false;
+            while (!agree) {
+                agree = true;
+                for (u32 i = 0; i < k; ++i) {
+                    while (col[p[i]] > x) {
+                        if (p[i] == lo[i]) return;
+                        --p[i];
+                    }
+                    if (col[p[i]] < x) {
+                        x = col[p[i]];
+                        agree = false;
+                    }
+                }
+            }
+            if (x < stop) return;
+            u32 t = 0;
+            while (t < k && vd[t] > x) ++t;
+            f32 cd = diam;
+            for (u32 i = 0; i < k; ++i) cd = cd < w[p[i]] ? w[p[i]] : cd;
+            if (cd <= cap && !emit(x, t, cd)) return;
+            if (p[0] == lo[0]) return;
+            --p[0];
+        }
+    }
+};
+
+// ---- pivot map (open addressing) -------------------------------------------
+template <class I>
+struct PivotTable {
+    std::vector<I> key;
+    std::vector<u64> val;
+    u64 mask = 0, used = 0;
+    void reset(u64 expected) {
+        u64 c = 64;
+        while (c < 2 * expected + 16) c <<= 1;
+        key.assign(c, kNone<I>);
+        val.assign(c, 0);
+        mask = c - 1;
+        used = 0;
+    }
+    static u64 mix(I k) {
+        u64 x = u64(k) ^ (u64(k >> 32 >> 32) * 0x9E3779B97F4A7C15ull);
+        x ^= x >> 31;
+        x *= 0xBF58476D1CE4E5B9ull;
+        x ^= x >> 29;
+        x *= 0x94D049BB133111EBull;
+        return x ^ (x >> 32);
+    }
+    u64 find(I k) const {
+        for (u64 s = mix(k) & mask;; s = (s + 1) & mask) {
+            if (key[s] == k) return val[s];
+            if (key[s] == kNone<I>) return ~0ull;
+        }
+    }
+    bool has(I k) const { return find(k) != ~0ull; }
+    void put(I k, u64 v) {
+        if (2 * (used + 1) > key.size()) grow();
+        u64 s = mix(k) & mask;
+        while (key[s] != kNone<I> && key[s] != k) s = (s + 1) & mask;
+        if (key[s] == kNone<I>) ++used;
+        key[s] = k;
+        val[s] = v;
+    }
+    void grow() {
+        std::vector<I> ok;
+        std::vector<u64> ov;
+        ok.swap(key);
+        ov.swap(val);
+        reset(ok.size());
+        for (u64 i = 0; i < ok.size(); ++i)
+            if (ok[i] != kNone<I>) put(ok[i], ov[i]);
+    }
+};
+
+// ---- the reducer ---------------------------------------------------------------
+template <class I, class Cx>
+class Cohomology {
+  public:
+    struct S {          // a simplex in the filtration
+        I idx;
+        f32 d;
+    };
+    // a precedes b in the filtration
+    static bool before(const S& a, const S& b) { return a.d < b.d || (a.d == b.d && a.idx > b.idx); }
+
+    Cohomology(const Cx& cx, u32 maxdim, Diagram& out, Telemetry& tel)
+        : cx_(cx), n_(cx.n), maxdim_(maxdim), B_(cx.n, maxdim + 2), out_(out), tel_(tel) {}
+
+    // Dimension 0 from the GPU's edges (filtration order) and apparent flags.
+    void dim0(const u64* idx, const f32* d, const u8* apparent, u64 m) {
+        std::vector<u32> parent(n_), rank(n_, 0);
+        for (u32 v = 0; v < n_; ++v) parent[v] = v;
+        auto find = [&](u32 x) {
+            while (parent[x] != x) {
+                parent[x] = parent[parent[x]];
+                x = parent[x];
+            }
+            return x;
+        };
+        cols_.clear();
+        u64 skipped = 0;
+        for (u64 e = 0; e < m; ++e) {
+            u32 a, b;
+            decode_edge(idx[e], a, b);
+            u32 ra = find(a), rb = find(b);
+            if (ra != rb) {
+                if (rank[ra] < rank[rb]) std::swap(ra, rb);
+                parent[rb] = ra;
+                if (rank[ra] == rank[rb]) ++rank[ra];
+                if (d[e] > 0.0f) out_.add(0, 0.0f, d[e], u128(a), u128(idx[e]));
+            } else if (maxdim_ >= 1) {
+                if (apparent && apparent[e]) ++skipped;
+                else cols_.push_back(S{I(idx[e]), d[e]});
+            }
+        }
+        for (u32 v = 0; v < n_; ++v)
+            if (find(v) == v) out_.add(0, 0.0f, INFINITY, u128(v), u128(kNone<I>));
+        std::reverse(cols_.begin(), cols_.end());
+        if (maxdim_ >= 2) {
+            simplices_.resize(m);
+            for (u64 e = 0; e < m; ++e) simplices_[e] = S{I(idx[e]), d[e]};
+        }
+        tel_.edges = m;
+        tel_.apparent.push_back(skipped);
+    }
+
+    // dim-0 apparent test of one edge (rips.hpp:844-849): the edge is the
+    // zero-pivot facet of its zero-pivot cofacet.
+    bool edge_apparent(u64 idx, f32 d) const {
+        u32 vd[2];
+        decode_edge(idx, vd[0], vd[1]);
+        return apparent_cofacet_of(S{I(idx), d}, vd, 2).idx != kNone<I>;
+    }
+
+    void run_from_dim1() {
+        for (u32 dim = 1; dim <= maxdim_; ++dim) {
+            reduce(dim);
+            if (dim < maxdim_) assemble(dim);
+        }
+    }
+
+    // ---- index arithmetic -------------------------------------------------
+    // largest v < top with C(v, k) <= idx
+    u32 max_vertex(I idx, u32 k, u32 top) const {
+        u32 lo = k - 1, hi = top;   // C(lo, k) = 0 <= idx
+        while (hi - lo > 1) {
+            const u32 mid = lo + (hi - lo) / 2;
+            if (B_(mid, k) <= idx) lo = mid;
+            else hi = mid;
+        }
+        return lo;
+    }
+    void decode(I idx, u32 nv, u32* vd) const {
+        u32 top = n_;
+        for (u32 i = 0; i < nv; ++i) {
+            const u32 k = nv - i;
+            const u32 v = max_vertex(idx, k, top);
+            vd[i] = v;
+            idx -= B_(v, k);
+            top = v;
+        }
+    }
+    static void decode_edge(u64 idx, u32& a, u32& b) {
+        u64 v = u64((1.0 + std::sqrt(1.0 + 8.0 * double(idx))) * 0.5);
+        while (v * (v - 1) / 2 > idx) --v;
+        while ((v + 1) * v / 2 <= idx) ++v;
+        a = u32(v);
+        b = u32(idx - v * (v - 1) / 2);
+    }
+    I index_of(const u32* vd, u32 nv) const {
+        I s = 0;
+        for (u32 i = 0; i < nv; ++i) s += B_(vd[i], nv - i);
+        return s;
+    }
+
+  private:
+    static constexpr u32 KM = 33;
+
+    // cofacets of (vd, nv, diam) as simplices, in descending index order
+    template <class F>
+    void cofacet_simplices(const u32* vd, u32 nv, f32 diam, bool all, f32 cap, F&& f) const {
+        I pre[KM + 1], suf[KM + 1];   // pre[t] = sum_{i<t} C(vd[i], nv+1-i); suf[t] = sum_{i>=t} C(vd[i], nv-i)
+        pre[0] = 0;
+        for (u32 i = 0; i < nv; ++i) pre[i + 1] = pre[i] + B_(vd[i], nv + 1 - i);
+        suf[nv] = 0;
+        for (u32 i = nv; i-- > 0;) suf[i] = suf[i + 1] + B_(vd[i], nv - i);
+        cx_.cofacets(vd, nv, diam, all, cap,
+                     [&](u32 w, u32 t, f32 cd) { return f(S{pre[t] + B_(w, nv + 1 - t) + suf[t], cd}, w, t); });
+    }
+
+    // zero pivot cofacet: the first cofacet (descending index) of diameter d
+    S zero_pivot_cofacet(const u32* vd, u32 nv, f32 d, u32* cv) const {
+        S found{kNone<I>, 0.0f};
+        u32 fw = 0, ft = 0;
+        cofacet_simplices(vd, nv, d, true, d, [&](const S& c, u32 w, u32 t) {
+            found = c;
+            fw = w;
+            ft = t;
+            return false;
+        });
+        if (found.idx != kNone<I>) {
+            for (u32 i = 0; i < ft; ++i) cv[i] = vd[i];
+            cv[ft] = fw;
+            for (u32 i = ft; i < nv; ++i) cv[i + 1] = vd[i];
+        }
+        return found;
+    }
+
+    // zero pivot facet: the first facet (ascending index) of diameter d
+    S zero_pivot_facet(const u32* vd, u32 nv, f32 d, u32* fv) const {
+        if (nv < 2) return S{kNone<I>, 0.0f};
+        for (u32 j = 0; j < nv; ++j) {
+            bool hit = false;
+            for (u32 a = 0; a < nv && !hit; ++a) {
+                if (a == j) continue;
+                for (u32 b = a + 1; b < nv && !hit; ++b)
+                    if (b != j && cx_.dist(vd[a], vd[b]) == d) hit = true;
+            }
+            if (hit) {
+                u32 o = 0;
+                for (u32 i = 0; i < nv; ++i)
+                    if (i != j) fv[o++] = vd[i];
+                return S{index_of(fv, nv - 1), d};
+            }
+        }
+        return S{kNone<I>, 0.0f};
+    }
+
+    // (f, s) is an apparent pair with s the cofacet: returns f, else none
+    S apparent_facet_of(const S& s, const u32* vd, u32 nv) const {
+        u32 fv[KM], cv[KM + 1];
+        const S f = zero_pivot_facet(vd, nv, s.d, fv);
+        if (f.idx == kNone<I>) return f;
+        return zero_pivot_cofacet(fv, nv - 1, f.d, cv).idx == s.idx ? f : S{kNone<I>, 0.0f};
+    }
+    // (s, c) is an apparent pair with s the facet: returns c, else none
+    S apparent_cofacet_of(const S& s, const u32* vd, u32 nv) const {
+        u32 fv[KM + 1], cv[KM + 1];
+        const S c = zero_pivot_cofacet(vd, nv, s.d, cv);
+        if (c.idx == kNone<I>) return c;
+        return zero_pivot_facet(cv, nv + 1, c.d, fv).idx == s.idx ? c : S{kNone<I>, 0.0f};
+    }
+    S apparent_facet_of(const S& s, u32 nv) const {
+        u32 vd[KM];
+        decode(s.idx, nv, vd);
+        return apparent_facet_of(s, vd, nv);
+    }
+
+    // ---- working column: binary min-heap in filtration order -------------
+    struct Heap {
+        std::vector<S> h;
+        static bool later(const S& a, const S& b) { return before(b, a); }
+        void clear() { h.clear(); }
+        void push(const S& e) {
+            h.push_back(e);
+            std::push_heap(h.begin(), h.end(), later);
+        }
+        void load(std::vector<S>& v) {
+            h.swap(v);
+            std::make_heap(h.begin(), h.end(), later);
+        }
+        S pop_top() {
+            std::pop_heap(h.begin(), h.end(), later);
+            const S e = h.back();
+            h.pop_back();
+            return e;
+        }
+        // earliest surviving entry after mod-2 cancellation, left in the heap
+        S pivot() {
+            while (!h.empty()) {
+                const S p = pop_top();
+                if (!h.empty() && h.front().idx == p.idx) {
+                    pop_top();
+                    continue;
+                }
+                push(p);
+                return p;
+            }
+            return S{kNone<I>, 0.0f};
+        }
+    };
+
+    void push_coboundary(const S& s, u32 nv) {
+        u32 vd[KM];
+        decode(s.idx, nv, vd);
+        cofacet_simplices(vd, nv, s.d, true, cx_.threshold, [&](const S& c, u32, u32) {
+            heap_.push(c);
+            return true;
+        });
+    }
+
+    void reduce(u32 dim) {
+        const u32 nv = dim + 1;
+        pivots_.reset(cols_.size());
+        red_off_.assign(1, 0);
+        red_.clear();
+        u64 emergent = 0;
+        std::vector<S> buf;
+        std::vector<S> added;
+        u32 vd[KM];
+        for (u64 c = 0; c < cols_.size(); ++c) {
+            const S col = cols_[c];
+            heap_.clear();
+            added.clear();
+            buf.clear();
+            decode(col.idx, nv, vd);
+            S piv{kNone<I>, 0.0f};
+            bool fresh = false, checking = true;
+            cofacet_simplices(vd, nv, col.d, true, cx_.threshold, [&](const S& cf, u32, u32) {
+                if (checking && cf.d == col.d) {
+                    if (!pivots_.has(cf.idx) && apparent_facet_of(cf, nv + 1).idx == kNone<I>) {
+                        piv = cf;           // emergent pair: the column is reduced already
+                        fresh = true;
+                        return false;
+                    }
+                    checking = false;
+                }
+                buf.push_back(cf);
+                return true;
+            });
+            if (fresh) {
+                ++emergent;
+            } else {
+                heap_.load(buf);
+                piv = heap_.pivot();
+            }
+            for (;;) {
+                if (piv.idx == kNone<I>) {
+                    out_.add(dim, col.d, INFINITY, u128(col.idx), u128(kNone<I>));
+                    break;
+                }
+                if (fresh) {
+                    pivots_.put(piv.idx, c);
+                    break;
+                }
+                const u64 j = pivots_.find(piv.idx);
+                if (j != ~0ull) {
+                    added.push_back(cols_[j]);
+                    push_coboundary(cols_[j], nv);
+                    for (u64 t = red_off_[j]; t < red_off_[j + 1]; ++t) {
+                        added.push_back(red_[t]);
+                        push_coboundary(red_[t], nv);
+                    }
+                    piv = heap_.pivot();
+                    continue;
+                }
+                const S fa = apparent_facet_of(piv, nv + 1);
+                if (fa.idx != kNone<I>) {
+                    added.push_back(fa);
+                    push_coboundary(fa, nv);
+                    piv = heap_.pivot();
+                    continue;
+                }
+                if (piv.d > col.d) out_.add(dim, col.d, piv.d, u128(col.idx), u128(piv.idx));
+                pivots_.put(piv.idx, c);
+                // store the reduction column modulo 2
+                std::sort(added.begin(), added.end(), [](const S& a, const S& b) { return a.idx < b.idx; });
+                for (size_t x = 0; x < added.size();) {
+                    if (x + 1 < added.size() && added[x].idx == added[x + 1].idx) x += 2;
+                    else red_.push_back(added[x++]);
+                }
+                break;
+            }
+            red_off_.push_back(red_.size());
+        }
+        tel_.columns.push_back(cols_.size());
+        tel_.emergent.push_back(emergent);
+    }
+
+    // columns of dimension dim + 1 from the simplices of dimension dim
+    void assemble(u32 dim) {
+        const u32 nv = dim + 1;
+        const bool keep = dim + 1 < maxdim_;
+        std::vector<S> next;
+        cols_.clear();
+        u64 skipped = 0;
+        u32 vd[KM], cv[KM + 1];
+        for (const S& s : simplices_) {
+            decode(s.idx, nv, vd);
+            cofacet_simplices(vd, nv, s.d, false, cx_.threshold, [&](const S& cf, u32 w, u32 t) {
+                if (keep) next.push_back(cf);
+                if (pivots_.has(cf.idx)) return true;          // cleared
+                for (u32 i = 0; i < t; ++i) cv[i] = vd[i];
+                cv[t] = w;
+                for (u32 i = t; i < nv; ++i) cv[i + 1] = vd[i];
+                if (apparent_cofacet_of(cf, cv, nv + 1).idx != kNone<I> ||
+                    apparent_facet_of(cf, cv, nv + 1).idx != kNone<I>) {
+                    ++skipped;
+                    return true;
+                }
+                cols_.push_back(cf);
+                return true;
+            });
+        }
+        simplices_.swap(next);
+        std::sort(cols_.begin(), cols_.end(), [](const S& a, const S& b) { return before(b, a); });
+        tel_.apparent.push_back(skipped);
+    }
+
+    const Cx& cx_;
+    const u32 n_, maxdim_;
+    const Binomials<I> B_;
+    Diagram& out_;
+    Telemetry& tel_;
+    std::vector<S> cols_, simplices_, red_;
+    std::vector<u64> red_off_;
+    PivotTable<I> pivots_;
+    Heap heap_;
+};
+
+}  // namespace tops::cob
