@@ -154,6 +154,35 @@ tt_status tt_dense_filtration(const float* pts, uint32_t n, uint32_t dim, int ki
                               uint64_t** idx, float** d, uint8_t** apparent,
                               float* threshold_out);
 
+/* ---- batches of clouds ---------------------------------------------------- */
+
+/* Many clouds of uneven size and dimension through the dense front end of
+ * persistence() (turbotops.hpp:230-253) in one launch sequence, recorded
+ * once into a HIP graph and replayed. The reference has no batch API: this is
+ * the per-cloud front end (make_dense, enclosing_radius, Reducer::dim0's sort
+ * and apparent test) applied to every cloud of the batch at once.
+ * Opaque handle; create once per set of shapes. threshold = +inf: each
+ * cloud's enclosing radius. The tile shape is autotuned on the first run. */
+typedef struct tt_batch tt_batch;
+tt_status tt_batch_create(uint32_t nclouds, const uint32_t* n, const uint32_t* dim, int kind,
+                          float p, float threshold, tt_batch** out);
+/* pts: every cloud's points, row-major, concatenated in order. Returns once
+ * the device work is queued and the per-cloud host copies are issued. */
+tt_status tt_batch_run(tt_batch* b, const float* pts, int use_graph);
+/* Blocks until cloud `cloud`'s results are in host memory. */
+tt_status tt_batch_wait(tt_batch* b, uint32_t cloud);
+/* Cloud results (library-owned, valid until the next run): the full
+ * matrix, its threshold, and the kept edges that are not in an apparent
+ * pair (idx, d), in filtration order — the union-find and the H1 columns
+ * of dimension 0 need no others. */
+tt_status tt_batch_cloud(tt_batch* b, uint32_t cloud, const float** mat, float* threshold,
+                         uint64_t* m, const uint64_t** idx, const float** d);
+/* Device time of the last run, the autotuned rows per wave (8/16/32), and
+ * the kept / non-apparent edge counts. */
+tt_status tt_batch_info(tt_batch* b, float* device_ms, int* rows_per_wave, uint64_t* edges_total,
+                        uint64_t* edges_kept);
+void tt_batch_destroy(tt_batch* b);
+
 /* Releases a buffer the library allocated for a host-side result. */
 void tt_free(void* p);
 

@@ -6,7 +6,7 @@ exact kNN and kNN graphs, H0 via Boruvka, and the apparent-pair scan, as
 hand-written HIP kernels behind the C ABI in include/ttgpu.h.
 """
 from . import capi  # noqa: F401
-from .api import persistence, persistence_pairs  # noqa: F401
+from .api import BatchPersistence, persistence, persistence_batch, persistence_pairs  # noqa: F401
 from .frontend import (  # noqa: F401
     CsrGraph,
     dense_edges,

@@ -62,3 +62,37 @@ def persistence_pairs(x, maxdim=1, threshold=math.inf, metric="l2", graph="full"
         raise ValueError("x must be a 2-d array (n, d)")
     return ext.persistence_pairs(x, int(maxdim), float(threshold), str(metric), str(graph), int(k),
                                  int(threads), int(memory_budget))
+
+
+class BatchPersistence:
+    """Many clouds of uneven size and dimension through one batched GPU front
+    end (dense full Rips at each cloud's enclosing radius, H0 and H1), with
+    host reductions in parallel. Create once per set of shapes; call with the
+    clouds (a list of (n_i, d_i) arrays) for a list of (dgms, info), each equal
+    to persistence(cloud, maxdim=1, metric=metric)."""
+
+    def __init__(self, shapes, metric="l2", maxdim=1):
+        ext = _ext()
+        self.shapes = [(int(n), int(d)) for n, d in shapes]
+        self._r = ext.BatchRunner([n for n, _ in self.shapes], [d for _, d in self.shapes], str(metric),
+                                  int(maxdim))
+
+    def __call__(self, clouds, use_graph=True, threads=0):
+        clouds = [np.ascontiguousarray(c, dtype=np.float32) for c in clouds]
+        if [c.shape for c in clouds] != self.shapes:
+            raise ValueError("clouds do not match the batch shapes")
+        return self._r.run(clouds, bool(use_graph), int(threads))
+
+    @property
+    def device_ms(self):
+        return self._r.device_ms
+
+    @property
+    def rows_per_wave(self):
+        return self._r.rows_per_wave
+
+
+def persistence_batch(clouds, maxdim=1, metric="l2", threads=0, use_graph=True):
+    """persistence() of every cloud in `clouds` through one batched launch."""
+    shapes = [np.shape(c) for c in clouds]
+    return BatchPersistence(shapes, metric=metric, maxdim=maxdim)(clouds, use_graph=use_graph, threads=threads)

@@ -1,0 +1,616 @@
+// SPDX-License-Identifier: MIT
+// batch.hip — the dense front end of many point clouds of uneven size and
+// dimension in one launch sequence, recorded once into a HIP graph.
+//
+// Per cloud c it produces what persistence()'s dense branch needs
+// (include/turbotops/turbotops.hpp:230-253, rips.hpp:391-416, :370-380,
+// :685-849): the full distance matrix, the enclosing radius, and the kept
+// edges of dimension 0 in filtration order that are NOT in an apparent pair.
+// Apparent edges are never union-find tree edges (both other edges of their
+// triangle precede them), so for H1 the host needs only the rest.
+//
+// All clouds share one set of buffers: points, matrices and rows are packed
+// with per-cloud offsets. Kernels take fixed grids and read every
+// data-dependent count from device memory, so the whole sequence can be
+// replayed from a graph:
+//   batch_dist     persistent: blocks stride over the global tile list; a
+//                  binary search over tile offsets names the cloud, and the
+//                  tile runs the single-cloud tile body (tiles.hpp) for that
+//                  cloud's dimension
+//   batch_radius   one workgroup per cloud: min of the row maxima
+//   batch_count    one wave per global row: kept edges j < i
+//   scan           global row offsets; cloud segments follow
+//   batch_compact  keys (cloud, diameter bits, M_c - 1 - idx): one sort orders
+//                  every cloud by `precedes` and keeps clouds contiguous
+//   counted radix sort over the device-side count
+//   batch_apparent one wave64 per edge (the apparent test of edges.hip)
+//   compaction of the non-apparent edges
+// Host copies then run per cloud, each followed by an event, so the host
+// reducer starts on a cloud as soon as its data has landed.
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <vector>
+
+#include "frontend.hpp"
+#include "tiles.hpp"
+
+namespace ttg {
+
+namespace {
+
+struct BatchShape {
+    u32 nclouds;
+    const u32* n;
+    const u32* dim;
+    const u64* pt_off;
+    const u64* mat_off;
+    const u64* row_base;    // nclouds + 1
+    const u64* tile_base;   // nclouds + 1
+    const u64* pair_base;   // nclouds + 1 (key capacity offsets)
+    u64 total_tiles, total_rows;
+};
+
+__device__ __forceinline__ u32 find_segment(const u64* base, u32 count, u64 t) {
+    u32 lo = 0, hi = count;   // base[lo] <= t < base[hi]
+    while (hi - lo > 1) {
+        const u32 mid = (lo + hi) >> 1;
+        if (base[mid] <= t) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+template <int V, int RPW>
+__global__ __launch_bounds__(64 * (64 / RPW)) void batch_dist(BatchShape b, MetricArgs ma,
+                                                              const float* __restrict__ pts,
+                                                              float* __restrict__ mats,
+                                                              u32* __restrict__ rowmax) {
+    __shared__ TileSmem<RPW> sm;
+    for (u64 t = blockIdx.x; t < b.total_tiles; t += gridDim.x) {
+        const u32 c = __builtin_amdgcn_readfirstlane(find_segment(b.tile_base, b.nclouds, t));
+        const u32 n = b.n[c], dim = b.dim[c];
+        const float* x = pts + b.pt_off[c];
+        float* mat = mats + b.mat_off[c];
+        u32* rm = rowmax + b.row_base[c];
+        u32 bi, bj;
+        lower_tile_of(t - b.tile_base[c], bi, bj);
+        switch (dim) {
+            case 2: dist_tile<V, 2, TT_LAYOUT_FULL, RPW>(x, n, dim, ma, bi, bj, mat, rm, sm); break;
+            case 3: dist_tile<V, 3, TT_LAYOUT_FULL, RPW>(x, n, dim, ma, bi, bj, mat, rm, sm); break;
+            case 8: dist_tile<V, 8, TT_LAYOUT_FULL, RPW>(x, n, dim, ma, bi, bj, mat, rm, sm); break;
+            default: dist_tile<V, 0, TT_LAYOUT_FULL, RPW>(x, n, dim, ma, bi, bj, mat, rm, sm); break;
+        }
+        __syncthreads();
+    }
+}
+
+__global__ __launch_bounds__(256) void batch_radius(BatchShape b, const u32* __restrict__ rowmax,
+                                                    float user_thr, float* __restrict__ thr) {
+    __shared__ u32 red[4];
+    const u32 c = blockIdx.x;
+    const u32 n = b.n[c];
+    const u32* rm = rowmax + b.row_base[c];
+    u32 m = 0x7f800000u;
+    for (u32 t = threadIdx.x; t < n; t += 256) m = min(m, rm[t]);
+    for (int off = 32; off > 0; off >>= 1) m = min(m, __shfl_xor(m, off));
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const u32 r = min(min(red[0], red[1]), min(red[2], red[3]));
+        thr[c] = isinf(user_thr) ? __uint_as_float(r) : user_thr;
+    }
+}
+
+__global__ __launch_bounds__(256) void batch_count(BatchShape b, const float* __restrict__ mats,
+                                                   const float* __restrict__ thr, u32* __restrict__ cnt) {
+    const u32 lane = threadIdx.x & 63;
+    const u64 gw = (blockIdx.x * 256ull + threadIdx.x) >> 6;
+    const u64 nw = (u64(gridDim.x) * 256) >> 6;
+    for (u64 r = gw; r < b.total_rows; r += nw) {
+        const u32 c = find_segment(b.row_base, b.nclouds, r);
+        const u32 n = b.n[c];
+        const u32 i = u32(r - b.row_base[c]);
+        const float* row = mats + b.mat_off[c] + u64(i) * n;
+        const float th = thr[c];
+        u32 k = 0;
+        for (u32 j = lane; j < i; j += 64) k += row[j] <= th;
+#pragma unroll
+        for (u32 off = 32; off > 0; off >>= 1) k += __shfl_xor(k, off);
+        if (lane == 0) cnt[r] = k;
+    }
+}
+
+__global__ void batch_segments(BatchShape b, const u32* __restrict__ off, u32* __restrict__ seg) {
+    for (u32 c = threadIdx.x; c <= b.nclouds; c += blockDim.x) seg[c] = off[b.row_base[c]];
+}
+
+__global__ __launch_bounds__(256) void batch_compact(BatchShape b, const float* __restrict__ mats,
+                                                     const float* __restrict__ thr,
+                                                     const u32* __restrict__ off, int ib,
+                                                     u64* __restrict__ keys) {
+    const u32 lane = threadIdx.x & 63;
+    const u64 gw = (blockIdx.x * 256ull + threadIdx.x) >> 6;
+    const u64 nw = (u64(gridDim.x) * 256) >> 6;
+    for (u64 r = gw; r < b.total_rows; r += nw) {
+        const u32 c = find_segment(b.row_base, b.nclouds, r);
+        const u32 n = b.n[c];
+        const u32 i = u32(r - b.row_base[c]);
+        const float* row = mats + b.mat_off[c] + u64(i) * n;
+        const float th = thr[c];
+        const u64 M = u64(n) * (n - 1) / 2;
+        const u64 cloud_bits = u64(c) << (31 + ib);
+        u64 o = off[r];
+        const u64 rowbase = u64(i) * (i - 1) / 2;
+        for (u32 j0 = 0; j0 < i; j0 += 64) {
+            const u32 j = j0 + lane;
+            const float d = j < i ? row[j] : INFINITY;
+            const bool keep = j < i && d <= th;
+            const u64 bal = __ballot(keep);
+            if (keep) {
+                const u32 rk = __builtin_amdgcn_mbcnt_hi(u32(bal >> 32), __builtin_amdgcn_mbcnt_lo(u32(bal), 0u));
+                keys[o + rk] = cloud_bits | (u64(__float_as_uint(d)) << ib) | (M - 1 - (rowbase + j));
+            }
+            o += __popcll(bal);
+        }
+    }
+}
+
+__device__ __forceinline__ void decode_edge_b(u64 idx, u32& i, u32& j) {
+    u64 v = u64((1.0 + __builtin_sqrt(1.0 + 8.0 * double(idx))) * 0.5);
+    while (v * (v - 1) / 2 > idx) --v;
+    while ((v + 1) * v / 2 <= idx) ++v;
+    i = u32(v);
+    j = u32(idx - v * (v - 1) / 2);
+}
+
+// Apparent test per sorted edge (one wave64 each) and the keep flag of the
+// non-apparent ones.
+__global__ __launch_bounds__(256) void batch_apparent(BatchShape b, const float* __restrict__ mats,
+                                                      const u64* __restrict__ keys,
+                                                      const u32* __restrict__ total, int ib,
+                                                      u32* __restrict__ keep) {
+    const u32 lane = threadIdx.x & 63;
+    const u64 gw = (blockIdx.x * 256ull + threadIdx.x) >> 6;
+    const u64 nw = (u64(gridDim.x) * 256) >> 6;
+    const u64 m = *total;
+    const u64 imask = (u64(1) << ib) - 1;
+    for (u64 e = gw; e < m; e += nw) {
+        const u64 key = keys[e];
+        const u32 c = u32(key >> (31 + ib));
+        const float d = __uint_as_float(u32(key >> ib) & 0x7fffffffu);
+        const u32 n = b.n[c];
+        const u64 M = u64(n) * (n - 1) / 2;
+        u32 i, j;
+        decode_edge_b(M - 1 - (key & imask), i, j);
+        const float* mat = mats + b.mat_off[c];
+        const float* ri = mat + u64(i) * n;
+        const float* rj = mat + u64(j) * n;
+        int wf = -1;
+        float diw = 0.0f, djw = 0.0f;
+        for (int hi = int(n); hi > 0; hi -= 64) {
+            const int w = hi - 1 - int(lane);
+            float a = INFINITY, bb = INFINITY;
+            if (w >= 0) {
+                a = ri[w];
+                bb = rj[w];
+            }
+            const bool ok = w >= 0 && u32(w) != i && u32(w) != j && fmax_ref(a, bb) <= d;
+            const u64 bal = __ballot(ok);
+            if (bal) {
+                const int src = __builtin_ctzll(bal);
+                wf = hi - 1 - src;
+                diw = __shfl(a, src);
+                djw = __shfl(bb, src);
+                break;
+            }
+        }
+        if (lane == 0) {
+            bool ap = false;
+            if (wf >= 0) {
+                const u32 w = u32(wf);
+                if (w > i) ap = true;
+                else if (w > j) ap = djw != d;
+                else ap = (djw != d) && (diw != d);
+            }
+            keep[e] = ap ? 0u : 1u;
+        }
+    }
+}
+
+__global__ void batch_emit(const u64* __restrict__ keys, const u32* __restrict__ keep,
+                           const u32* __restrict__ pos, const u32* __restrict__ total, int ib,
+                           const u32* __restrict__ n_of, u64* __restrict__ out_idx,
+                           float* __restrict__ out_d) {
+    const u64 m = *total;
+    const u64 imask = (u64(1) << ib) - 1;
+    for (u64 e = blockIdx.x * 256ull + threadIdx.x; e < m; e += u64(gridDim.x) * 256) {
+        if (!keep[e]) continue;
+        const u64 key = keys[e];
+        const u32 c = u32(key >> (31 + ib));
+        const u64 n = n_of[c];
+        const u64 M = n * (n - 1) / 2;
+        out_idx[pos[e]] = M - 1 - (key & imask);
+        out_d[pos[e]] = __uint_as_float(u32(key >> ib) & 0x7fffffffu);
+    }
+}
+
+// per cloud: start of its kept (non-apparent) edges = number of kept edges
+// before its first sorted edge
+__global__ void batch_kept_segments(const u32* __restrict__ seg, const u32* __restrict__ pos,
+                                    const u32* __restrict__ keep, const u32* __restrict__ total,
+                                    u32 nclouds, u32* __restrict__ kseg) {
+    for (u32 c = threadIdx.x; c <= nclouds; c += blockDim.x) {
+        const u32 s = seg[c];
+        const u32 m = *total;
+        kseg[c] = s < m ? pos[s] : (m ? pos[m - 1] + keep[m - 1] : 0u);
+    }
+}
+
+template <int V>
+void launch_batch_dist(int rpw, u32 grid, hipStream_t s, const BatchShape& b, MetricArgs ma,
+                       const float* pts, float* mats, u32* rowmax) {
+    switch (rpw) {
+        case 8: batch_dist<V, 8><<<grid, 512, 0, s>>>(b, ma, pts, mats, rowmax); break;
+        case 32: batch_dist<V, 32><<<grid, 128, 0, s>>>(b, ma, pts, mats, rowmax); break;
+        default: batch_dist<V, 16><<<grid, 256, 0, s>>>(b, ma, pts, mats, rowmax); break;
+    }
+}
+
+void launch_batch_dist_v(MetricVariant v, int rpw, u32 grid, hipStream_t s, const BatchShape& b,
+                         MetricArgs ma, const float* pts, float* mats, u32* rowmax) {
+    switch (v) {
+        case MV_L1: launch_batch_dist<MV_L1>(rpw, grid, s, b, ma, pts, mats, rowmax); break;
+        case MV_L2: launch_batch_dist<MV_L2>(rpw, grid, s, b, ma, pts, mats, rowmax); break;
+        case MV_LINF: launch_batch_dist<MV_LINF>(rpw, grid, s, b, ma, pts, mats, rowmax); break;
+        case MV_LP_INT: launch_batch_dist<MV_LP_INT>(rpw, grid, s, b, ma, pts, mats, rowmax); break;
+        default: launch_batch_dist<MV_LP_POW>(rpw, grid, s, b, ma, pts, mats, rowmax); break;
+    }
+}
+
+// Autotuned tile shape per (metric variant, dimension set): the fastest
+// rows-per-wave of {8, 16, 32} measured on the batch itself.
+std::mutex g_tune_mu;
+std::map<std::pair<int, u64>, int> g_tune;
+
+}  // namespace
+
+struct BatchState {
+    u32 nclouds = 0;
+    int kind = 1;
+    float p = 2.0f, threshold = INFINITY;
+    MetricVariant variant = MV_L2;
+    MetricArgs ma{};
+    int rpw = 16, ib = 1, cb = 0;
+    bool tuned = false;
+    std::vector<u32> n, dim;
+    std::vector<u64> pt_off, mat_off, row_base, tile_base, pair_base;
+    u64 total_pts = 0, total_mat = 0, total_rows = 0, total_tiles = 0, capacity = 0, h_cap = 0;
+    u32 grid_dist = 0;
+    // device
+    Workspace ws;
+    float *d_pts = nullptr, *d_mats = nullptr, *d_thr = nullptr, *d_out_d = nullptr;
+    u32 *d_rowmax = nullptr, *d_cnt = nullptr, *d_total = nullptr, *d_seg = nullptr, *d_keep = nullptr,
+        *d_pos = nullptr, *d_kept = nullptr, *d_kseg = nullptr, *d_hist = nullptr, *d_stmp = nullptr,
+        *d_n = nullptr, *d_dim = nullptr;
+    u64 *d_keys = nullptr, *d_alt = nullptr, *d_out_idx = nullptr, *d_pt_off = nullptr, *d_mat_off = nullptr,
+        *d_row_base = nullptr, *d_tile_base = nullptr, *d_pair_base = nullptr;
+    BatchShape shape{};
+    // host (pinned)
+    float* h_pts = nullptr;
+    float* h_mats = nullptr;
+    u64* h_idx = nullptr;
+    float* h_d = nullptr;
+    std::vector<u32> seg, kseg;
+    std::vector<float> thr;
+    hipStream_t stream = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    std::vector<hipEvent_t> ev;
+    hipEvent_t t0 = nullptr, t1 = nullptr;
+    float last_ms = 0.0f;
+    ~BatchState() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        for (auto e : ev) (void)hipEventDestroy(e);
+        if (t0) (void)hipEventDestroy(t0);
+        if (t1) (void)hipEventDestroy(t1);
+        if (stream) (void)hipStreamDestroy(stream);
+        if (h_pts) (void)hipHostFree(h_pts);
+        if (h_mats) (void)hipHostFree(h_mats);
+        if (h_idx) (void)hipHostFree(h_idx);
+        if (h_d) (void)hipHostFree(h_d);
+    }
+
+    // the device part of one batch (recorded into the graph)
+    void enqueue(int rpw_use) {
+        hipStream_t s = stream;
+        fill_device_u32(d_rowmax, std::max<u64>(total_rows, 1), 0u, s);
+        if (total_tiles) launch_batch_dist_v(variant, rpw_use, grid_dist, s, shape, ma, d_pts, d_mats, d_rowmax);
+        TTG_LAUNCH_CHECK();
+        batch_radius<<<nclouds, 256, 0, s>>>(shape, d_rowmax, threshold, d_thr);
+        fill_device_u32(d_cnt, total_rows + 1, 0u, s);
+        const u32 gw = u32(std::min<u64>(std::max<u64>((total_rows + 3) / 4, 1), 16384));
+        batch_count<<<gw, 256, 0, s>>>(shape, d_mats, d_thr, d_cnt);
+        exclusive_scan_u32_buf(d_cnt, d_cnt, total_rows + 1, d_total, d_stmp, s);
+        batch_segments<<<1, 256, 0, s>>>(shape, d_cnt, d_seg);
+        batch_compact<<<gw, 256, 0, s>>>(shape, d_mats, d_thr, d_cnt, ib, d_keys);
+        TTG_LAUNCH_CHECK();
+        radix_sort_u64_counted(d_keys, capacity, d_total, 0, cb + 31 + ib, d_alt, d_hist, d_stmp, s);
+        const u32 ge = u32(std::min<u64>(std::max<u64>(capacity / 4, 1), 65536));
+        fill_device_u32(d_keep, capacity, 0u, s);   // slots past the count stay 0
+        batch_apparent<<<ge, 256, 0, s>>>(shape, d_mats, d_keys, d_total, ib, d_keep);
+        exclusive_scan_u32_buf(d_keep, d_pos, capacity, d_kept, d_stmp, s);
+        batch_emit<<<ge, 256, 0, s>>>(d_keys, d_keep, d_pos, d_total, ib, d_n, d_out_idx, d_out_d);
+        batch_kept_segments<<<1, 256, 0, s>>>(d_seg, d_pos, d_keep, d_total, nclouds, d_kseg);
+        TTG_LAUNCH_CHECK();
+    }
+};
+
+}  // namespace ttg
+
+using namespace ttg;
+
+struct tt_batch {
+    BatchState st;
+};
+
+namespace {
+
+void batch_setup(BatchState& b, u32 nclouds, const uint32_t* n, const uint32_t* dim, int kind, float p,
+                 float threshold) {
+    check_metric(kind, p);
+    if (nclouds == 0) fail(TT_EINVAL, "tt_batch_create: no clouds");
+    if (!n || !dim) fail(TT_EINVAL, "tt_batch_create: null shapes");
+    if (std::isnan(threshold) || threshold < 0.0f)
+        fail(TT_EINVAL, "persistence: threshold must be non-negative or +inf");
+    b.nclouds = nclouds;
+    b.kind = kind;
+    b.p = p;
+    b.threshold = threshold;
+    b.variant = variant_of(kind, p);
+    b.ma = metric_args(kind, p);
+    b.n.assign(n, n + nclouds);
+    b.dim.assign(dim, dim + nclouds);
+    b.pt_off.assign(nclouds + 1, 0);
+    b.mat_off.assign(nclouds + 1, 0);
+    b.row_base.assign(nclouds + 1, 0);
+    b.tile_base.assign(nclouds + 1, 0);
+    b.pair_base.assign(nclouds + 1, 0);
+    u64 maxM = 1;
+    for (u32 c = 0; c < nclouds; ++c) {
+        if (b.n[c] && b.dim[c] == 0) fail(TT_EINVAL, "tt_batch_create: dim must be >= 1");
+        if (b.n[c] > 92682) fail(TT_ELENGTH, "tt_batch_create: a cloud exceeds 92682 points");
+        const u64 nb = (u64(b.n[c]) + DTILE - 1) / DTILE;
+        const u64 M = u64(b.n[c]) * (b.n[c] ? b.n[c] - 1 : 0) / 2;
+        maxM = std::max<u64>(maxM, M);
+        b.pt_off[c + 1] = b.pt_off[c] + u64(b.n[c]) * b.dim[c];
+        b.mat_off[c + 1] = b.mat_off[c] + u64(b.n[c]) * b.n[c];
+        b.row_base[c + 1] = b.row_base[c] + b.n[c];
+        b.tile_base[c + 1] = b.tile_base[c] + nb * (nb + 1) / 2;
+        b.pair_base[c + 1] = b.pair_base[c] + M;
+    }
+    b.total_pts = b.pt_off[nclouds];
+    b.total_mat = b.mat_off[nclouds];
+    b.total_rows = b.row_base[nclouds];
+    b.total_tiles = b.tile_base[nclouds];
+    b.capacity = std::max<u64>(b.pair_base[nclouds], 1);
+    if (b.capacity >= (u64(1) << 32)) fail(TT_ELENGTH, "tt_batch_create: more than 2^32 vertex pairs in one batch");
+    b.ib = std::max(1, bit_length(maxM - 1));
+    b.cb = bit_length(u64(nclouds - 1));
+    if (b.cb + 31 + b.ib > 64) fail(TT_ELENGTH, "tt_batch_create: edge keys exceed 64 bits");
+    b.grid_dist = u32(std::max<u64>(1, std::min<u64>(b.total_tiles, 256 * 16)));
+
+    Workspace& ws = b.ws;
+    b.d_pts = ws.get<float>(std::max<u64>(b.total_pts, 1));
+    b.d_mats = ws.get<float>(std::max<u64>(b.total_mat, 1));
+    b.d_rowmax = ws.get<u32>(std::max<u64>(b.total_rows, 1));
+    b.d_thr = ws.get<float>(nclouds);
+    b.d_cnt = ws.get<u32>(b.total_rows + 1);
+    b.d_total = ws.get<u32>(1);
+    b.d_seg = ws.get<u32>(nclouds + 1);
+    b.d_kseg = ws.get<u32>(nclouds + 1);
+    b.d_keys = ws.get<u64>(b.capacity);
+    b.d_alt = ws.get<u64>(b.capacity);
+    b.d_keep = ws.get<u32>(b.capacity);
+    b.d_pos = ws.get<u32>(b.capacity);
+    b.d_kept = ws.get<u32>(1);
+    b.d_out_idx = ws.get<u64>(b.capacity);
+    b.d_out_d = ws.get<float>(b.capacity);
+    b.d_hist = ws.get<u32>(radix_hist_entries(b.capacity));
+    const u64 stmp = std::max({scan_tmp_entries(radix_hist_entries(b.capacity)), scan_tmp_entries(b.capacity),
+                               scan_tmp_entries(b.total_rows + 1), u64(1)});
+    b.d_stmp = ws.get<u32>(stmp);
+    b.d_n = ws.get<u32>(nclouds);
+    b.d_dim = ws.get<u32>(nclouds);
+    b.d_pt_off = ws.get<u64>(nclouds + 1);
+    b.d_mat_off = ws.get<u64>(nclouds + 1);
+    b.d_row_base = ws.get<u64>(nclouds + 1);
+    b.d_tile_base = ws.get<u64>(nclouds + 1);
+    b.d_pair_base = ws.get<u64>(nclouds + 1);
+    TTG_HIP(hipStreamCreateWithFlags(&b.stream, hipStreamNonBlocking));
+    hipStream_t s = b.stream;
+    TTG_HIP(hipMemcpyAsync(b.d_n, b.n.data(), nclouds * 4, hipMemcpyHostToDevice, s));
+    TTG_HIP(hipMemcpyAsync(b.d_dim, b.dim.data(), nclouds * 4, hipMemcpyHostToDevice, s));
+    TTG_HIP(hipMemcpyAsync(b.d_pt_off, b.pt_off.data(), (nclouds + 1) * 8, hipMemcpyHostToDevice, s));
+    TTG_HIP(hipMemcpyAsync(b.d_mat_off, b.mat_off.data(), (nclouds + 1) * 8, hipMemcpyHostToDevice, s));
+    TTG_HIP(hipMemcpyAsync(b.d_row_base, b.row_base.data(), (nclouds + 1) * 8, hipMemcpyHostToDevice, s));
+    TTG_HIP(hipMemcpyAsync(b.d_tile_base, b.tile_base.data(), (nclouds + 1) * 8, hipMemcpyHostToDevice, s));
+    TTG_HIP(hipMemcpyAsync(b.d_pair_base, b.pair_base.data(), (nclouds + 1) * 8, hipMemcpyHostToDevice, s));
+    TTG_HIP(hipStreamSynchronize(s));
+    b.shape = BatchShape{nclouds, b.d_n, b.d_dim, b.d_pt_off, b.d_mat_off, b.d_row_base, b.d_tile_base,
+                         b.d_pair_base, b.total_tiles, b.total_rows};
+    TTG_HIP(hipHostMalloc(reinterpret_cast<void**>(&b.h_pts), std::max<u64>(b.total_pts, 1) * 4, hipHostMallocDefault));
+    TTG_HIP(hipHostMalloc(reinterpret_cast<void**>(&b.h_mats), std::max<u64>(b.total_mat, 1) * 4, hipHostMallocDefault));
+    b.ev.resize(nclouds);
+    for (auto& e : b.ev) TTG_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    TTG_HIP(hipEventCreate(&b.t0));
+    TTG_HIP(hipEventCreate(&b.t1));
+    b.seg.resize(nclouds + 1);
+    b.kseg.resize(nclouds + 1);
+    b.thr.resize(nclouds);
+}
+
+// Times each tile shape on this batch (points already on the device) and
+// keeps the fastest for this (metric variant, dimension set).
+void batch_autotune(BatchState& b) {
+    u64 dimkey = 0;
+    for (u32 d : b.dim) dimkey |= u64(1) << std::min<u32>(d, 63);
+    const auto key = std::make_pair(int(b.variant), dimkey);
+    {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        auto it = g_tune.find(key);
+        if (it != g_tune.end()) {
+            b.rpw = it->second;
+            b.tuned = true;
+            return;
+        }
+    }
+    float best = 1e30f;
+    int best_rpw = 16;
+    for (int rpw : {8, 16, 32}) {
+        for (int rep = 0; rep < 2; ++rep) {   // first launch warms up
+            fill_device_u32(b.d_rowmax, std::max<u64>(b.total_rows, 1), 0u, b.stream);
+            TTG_HIP(hipEventRecord(b.t0, b.stream));
+            if (b.total_tiles)
+                launch_batch_dist_v(b.variant, rpw, b.grid_dist, b.stream, b.shape, b.ma, b.d_pts, b.d_mats,
+                                    b.d_rowmax);
+            TTG_HIP(hipEventRecord(b.t1, b.stream));
+            TTG_HIP(hipEventSynchronize(b.t1));
+            float ms = 0;
+            TTG_HIP(hipEventElapsedTime(&ms, b.t0, b.t1));
+            if (rep == 1 && ms < best) {
+                best = ms;
+                best_rpw = rpw;
+            }
+        }
+    }
+    b.rpw = best_rpw;
+    b.tuned = true;
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    g_tune[key] = best_rpw;
+}
+
+void batch_run(BatchState& b, const float* pts, int use_graph) {
+    hipStream_t s = b.stream;
+    if (b.total_pts) {
+        std::memcpy(b.h_pts, pts, b.total_pts * 4);
+        TTG_HIP(hipMemcpyAsync(b.d_pts, b.h_pts, b.total_pts * 4, hipMemcpyHostToDevice, s));
+    }
+    if (!b.tuned) batch_autotune(b);
+    TTG_HIP(hipEventRecord(b.t0, s));
+    if (use_graph) {
+        if (!b.exec) {
+            TTG_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+            b.enqueue(b.rpw);
+            TTG_HIP(hipStreamEndCapture(s, &b.graph));
+            TTG_HIP(hipGraphInstantiate(&b.exec, b.graph, nullptr, nullptr, 0));
+        }
+        TTG_HIP(hipGraphLaunch(b.exec, s));
+    } else {
+        b.enqueue(b.rpw);
+    }
+    TTG_HIP(hipEventRecord(b.t1, s));
+    // sizes of the data-dependent outputs
+    u32 kept = 0;
+    TTG_HIP(hipMemcpyAsync(b.kseg.data(), b.d_kseg, (b.nclouds + 1) * 4, hipMemcpyDeviceToHost, s));
+    TTG_HIP(hipMemcpyAsync(b.thr.data(), b.d_thr, b.nclouds * 4, hipMemcpyDeviceToHost, s));
+    TTG_HIP(hipMemcpyAsync(&kept, b.d_kept, 4, hipMemcpyDeviceToHost, s));
+    TTG_HIP(hipStreamSynchronize(s));
+    TTG_HIP(hipEventElapsedTime(&b.last_ms, b.t0, b.t1));
+    b.kseg[b.nclouds] = kept;
+    if (!b.h_idx || b.h_cap < kept) {
+        if (b.h_idx) (void)hipHostFree(b.h_idx);
+        if (b.h_d) (void)hipHostFree(b.h_d);
+        b.h_idx = nullptr;
+        b.h_d = nullptr;
+        const u64 cap = std::max<u64>(kept, 1) + (kept >> 3);
+        TTG_HIP(hipHostMalloc(reinterpret_cast<void**>(&b.h_idx), cap * 8, hipHostMallocDefault));
+        TTG_HIP(hipHostMalloc(reinterpret_cast<void**>(&b.h_d), cap * 4, hipHostMallocDefault));
+        b.h_cap = cap;
+    }
+    // per cloud: matrix + its kept edges, then an event the host waits on
+    for (u32 c = 0; c < b.nclouds; ++c) {
+        const u64 cells = b.mat_off[c + 1] - b.mat_off[c];
+        if (cells)
+            TTG_HIP(hipMemcpyAsync(b.h_mats + b.mat_off[c], b.d_mats + b.mat_off[c], cells * 4,
+                                   hipMemcpyDeviceToHost, s));
+        const u64 e0 = b.kseg[c], e1 = b.kseg[c + 1];
+        if (e1 > e0) {
+            TTG_HIP(hipMemcpyAsync(b.h_idx + e0, b.d_out_idx + e0, (e1 - e0) * 8, hipMemcpyDeviceToHost, s));
+            TTG_HIP(hipMemcpyAsync(b.h_d + e0, b.d_out_d + e0, (e1 - e0) * 4, hipMemcpyDeviceToHost, s));
+        }
+        TTG_HIP(hipEventRecord(b.ev[c], s));
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+tt_status tt_batch_create(uint32_t nclouds, const uint32_t* n, const uint32_t* dim, int kind, float p,
+                          float threshold, tt_batch** out) {
+    return guarded([&] {
+        if (!out) fail(TT_EINVAL, "tt_batch_create: null handle");
+        *out = nullptr;
+        auto* b = new tt_batch;
+        try {
+            batch_setup(b->st, nclouds, n, dim, kind, p, threshold);
+        } catch (...) {
+            delete b;
+            throw;
+        }
+        *out = b;
+    });
+}
+
+tt_status tt_batch_run(tt_batch* b, const float* pts, int use_graph) {
+    return guarded([&] {
+        if (!b) fail(TT_EINVAL, "tt_batch_run: null handle");
+        if (b->st.total_pts && !pts) fail(TT_EINVAL, "tt_batch_run: null points");
+        batch_run(b->st, pts, use_graph);
+    });
+}
+
+tt_status tt_batch_wait(tt_batch* b, uint32_t cloud) {
+    return guarded([&] {
+        if (!b || cloud >= b->st.nclouds) fail(TT_EINVAL, "tt_batch_wait: bad cloud");
+        TTG_HIP(hipEventSynchronize(b->st.ev[cloud]));
+    });
+}
+
+tt_status tt_batch_cloud(tt_batch* b, uint32_t cloud, const float** mat, float* threshold, uint64_t* m,
+                         const uint64_t** idx, const float** d) {
+    return guarded([&] {
+        if (!b || cloud >= b->st.nclouds) fail(TT_EINVAL, "tt_batch_cloud: bad cloud");
+        BatchState& s = b->st;
+        if (mat) *mat = s.h_mats + s.mat_off[cloud];
+        if (threshold) *threshold = s.thr[cloud];
+        const u64 e0 = s.kseg[cloud], e1 = s.kseg[cloud + 1];
+        if (m) *m = e1 - e0;
+        if (idx) *idx = s.h_idx + e0;
+        if (d) *d = s.h_d + e0;
+    });
+}
+
+tt_status tt_batch_info(tt_batch* b, float* device_ms, int* rows_per_wave, uint64_t* edges_total,
+                        uint64_t* edges_kept) {
+    return guarded([&] {
+        if (!b) fail(TT_EINVAL, "tt_batch_info: null handle");
+        if (device_ms) *device_ms = b->st.last_ms;
+        if (rows_per_wave) *rows_per_wave = b->st.rpw;
+        if (edges_total || edges_kept) {
+            u32 total = 0;
+            TTG_HIP(hipMemcpy(&total, b->st.d_total, 4, hipMemcpyDeviceToHost));
+            if (edges_total) *edges_total = total;
+            if (edges_kept) *edges_kept = b->st.kseg[b->st.nclouds];
+        }
+    });
+}
+
+void tt_batch_destroy(tt_batch* b) { delete b; }
+
+}  // extern "C"

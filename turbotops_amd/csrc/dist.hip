@@ -25,93 +25,34 @@
 // NO_SIMD branches) step by step; the build uses -ffp-contract=off.
 #include <algorithm>
 
-#include "pairs.hpp"
+#include <atomic>
+
+#include "frontend.hpp"
+#include "tiles.hpp"
 
 namespace ttg {
-namespace {
 
-constexpr u32 TILE = 64;
+// Rows per wave of the distance tiles (8, 16 or 32; 512, 256 or 128 threads
+// per tile). Set by the autotuner; results never depend on it.
+static std::atomic<int> g_tile_rpw{16};
 
-constexpr u32 BLOCK = 256;    // 4 wave64s
-
-// Linear block id -> lower-triangle tile (bi >= bj).
-__device__ __forceinline__ void tile_of(u64 b, u32& bi, u32& bj) {
-    u32 t = u32((__builtin_sqrt(8.0 * double(b) + 1.0) - 1.0) * 0.5);
-    while (u64(t) * (t + 1) / 2 > b) --t;
-    while (u64(t + 1) * (t + 2) / 2 <= b) ++t;
-    bi = t;
-    bj = u32(b - u64(t) * (t + 1) / 2);
+int tile_rows_per_wave() { return g_tile_rpw.load(); }
+void set_tile_rows_per_wave(int rpw) {
+    if (rpw != 8 && rpw != 16 && rpw != 32) fail(TT_EINVAL, "tile rows per wave must be 8, 16 or 32");
+    g_tile_rpw.store(rpw);
 }
 
-template <int V, int DIM, int LAYOUT>
-__global__ __launch_bounds__(BLOCK) void dist_tiles(const float* __restrict__ x, u32 n,
-                                                    u32 dim_rt, MetricArgs ma,
-                                                    float* __restrict__ mat,
-                                                    u32* __restrict__ rowmax) {
-    __shared__ float tile[TILE][TILE + 1];
-    __shared__ float part_row[BLOCK / 64][TILE];
-    __shared__ float part_col[BLOCK / 64][TILE];
+namespace {
 
+template <int V, int DIM, int LAYOUT, int RPW>
+__global__ __launch_bounds__(64 * (64 / RPW)) void dist_tiles(const float* __restrict__ x, u32 n,
+                                                              u32 dim_rt, MetricArgs ma,
+                                                              float* __restrict__ mat,
+                                                              u32* __restrict__ rowmax) {
+    __shared__ TileSmem<RPW> sm;
     u32 bi, bj;
-    tile_of(blockIdx.x, bi, bj);
-    const u32 lane = threadIdx.x & 63;
-    const u32 wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const u32 j = bj * TILE + lane;
-    const u32 jc = j < n ? j : n - 1;
-    const u32 ibase = bi * TILE + wave * ROWS;
-
-    float val[ROWS];
-    if constexpr (DIM > 0) rows_static<V, DIM>(x, n, ma, ibase, jc, val);
-    else rows_runtime<V>(x, n, dim_rt, ma, ibase, jc, val);
-
-    // Row segments: (i, j) with j in this tile column.
-    float cmax = 0.0f;
-#pragma unroll
-    for (u32 r = 0; r < ROWS; ++r) {
-        const u32 i = ibase + r;
-        const bool in = i < n && j < n;
-        const float v = in ? val[r] : 0.0f;
-        tile[wave * ROWS + r][lane] = v;
-        cmax = fmax_ref(cmax, v);
-        if constexpr (LAYOUT == TT_LAYOUT_FULL) {
-            if (in) __builtin_nontemporal_store(v, mat + u64(i) * n + j);
-        } else {
-            if (in && j < i) __builtin_nontemporal_store(v, mat + u64(i) * (i - 1) / 2 + j);
-        }
-    }
-    part_col[wave][lane] = cmax;
-    __syncthreads();
-
-    // Transposed pass: lane = tile row, 16 tile columns per wave. Stores the
-    // mirror (full layout, off-diagonal tiles) and gathers row maxima.
-    const u32 i = bi * TILE + lane;
-    float rmax = 0.0f;
-#pragma unroll
-    for (u32 c = 0; c < ROWS; ++c) {
-        const u32 cl = wave * ROWS + c;
-        const float v = tile[lane][cl];
-        rmax = fmax_ref(rmax, v);
-        if constexpr (LAYOUT == TT_LAYOUT_FULL) {
-            const u32 jj = bj * TILE + cl;
-            if (bi != bj && i < n && jj < n) __builtin_nontemporal_store(v, mat + u64(jj) * n + i);
-        }
-    }
-    part_row[wave][lane] = rmax;
-    __syncthreads();
-
-    if (rowmax) {
-        if (wave == 0) {
-            float m = part_col[0][lane];
-#pragma unroll
-            for (u32 w = 1; w < BLOCK / 64; ++w) m = fmax_ref(m, part_col[w][lane]);
-            if (j < n) atomicMax(rowmax + j, __float_as_uint(m));
-        } else if (wave == 1) {
-            float m = part_row[0][lane];
-#pragma unroll
-            for (u32 w = 1; w < BLOCK / 64; ++w) m = fmax_ref(m, part_row[w][lane]);
-            if (i < n) atomicMax(rowmax + i, __float_as_uint(m));
-        }
-    }
+    lower_tile_of(blockIdx.x, bi, bj);
+    dist_tile<V, DIM, LAYOUT, RPW>(x, n, dim_rt, ma, bi, bj, mat, rowmax, sm);
 }
 
 __global__ void fill_u32(u32* __restrict__ p, u64 count, u32 v) {
@@ -136,11 +77,12 @@ __global__ __launch_bounds__(1024) void min_rowmax(const u32* __restrict__ rowma
     }
 }
 
-template <int V, int LAYOUT>
+template <int V, int LAYOUT, int RPW>
 void launch_dim(u32 blocks, hipStream_t s, const float* x, u32 n, u32 dim, MetricArgs ma,
                 float* mat, u32* rowmax) {
+    constexpr u32 threads = 64 * (64 / RPW);
 #define TTG_DIST_CASE(D) \
-    case D: dist_tiles<V, D, LAYOUT><<<blocks, BLOCK, 0, s>>>(x, n, dim, ma, mat, rowmax); break;
+    case D: dist_tiles<V, D, LAYOUT, RPW><<<blocks, threads, 0, s>>>(x, n, dim, ma, mat, rowmax); break;
     switch (dim) {
         TTG_DIST_CASE(1)
         TTG_DIST_CASE(2)
@@ -148,20 +90,30 @@ void launch_dim(u32 blocks, hipStream_t s, const float* x, u32 n, u32 dim, Metri
         TTG_DIST_CASE(4)
         TTG_DIST_CASE(8)
         TTG_DIST_CASE(16)
-        default: dist_tiles<V, 0, LAYOUT><<<blocks, BLOCK, 0, s>>>(x, n, dim, ma, mat, rowmax);
+        default: dist_tiles<V, 0, LAYOUT, RPW><<<blocks, threads, 0, s>>>(x, n, dim, ma, mat, rowmax);
     }
 #undef TTG_DIST_CASE
 }
 
-template <int LAYOUT>
+template <int LAYOUT, int RPW>
 void launch_variant(MetricVariant v, u32 blocks, hipStream_t s, const float* x, u32 n, u32 dim,
                     MetricArgs ma, float* mat, u32* rowmax) {
     switch (v) {
-        case MV_L1: launch_dim<MV_L1, LAYOUT>(blocks, s, x, n, dim, ma, mat, rowmax); break;
-        case MV_L2: launch_dim<MV_L2, LAYOUT>(blocks, s, x, n, dim, ma, mat, rowmax); break;
-        case MV_LINF: launch_dim<MV_LINF, LAYOUT>(blocks, s, x, n, dim, ma, mat, rowmax); break;
-        case MV_LP_INT: launch_dim<MV_LP_INT, LAYOUT>(blocks, s, x, n, dim, ma, mat, rowmax); break;
-        default: launch_dim<MV_LP_POW, LAYOUT>(blocks, s, x, n, dim, ma, mat, rowmax); break;
+        case MV_L1: launch_dim<MV_L1, LAYOUT, RPW>(blocks, s, x, n, dim, ma, mat, rowmax); break;
+        case MV_L2: launch_dim<MV_L2, LAYOUT, RPW>(blocks, s, x, n, dim, ma, mat, rowmax); break;
+        case MV_LINF: launch_dim<MV_LINF, LAYOUT, RPW>(blocks, s, x, n, dim, ma, mat, rowmax); break;
+        case MV_LP_INT: launch_dim<MV_LP_INT, LAYOUT, RPW>(blocks, s, x, n, dim, ma, mat, rowmax); break;
+        default: launch_dim<MV_LP_POW, LAYOUT, RPW>(blocks, s, x, n, dim, ma, mat, rowmax); break;
+    }
+}
+
+template <int LAYOUT>
+void launch_rpw(int rpw, MetricVariant v, u32 blocks, hipStream_t s, const float* x, u32 n, u32 dim,
+                MetricArgs ma, float* mat, u32* rowmax) {
+    switch (rpw) {
+        case 8: launch_variant<LAYOUT, 8>(v, blocks, s, x, n, dim, ma, mat, rowmax); break;
+        case 32: launch_variant<LAYOUT, 32>(v, blocks, s, x, n, dim, ma, mat, rowmax); break;
+        default: launch_variant<LAYOUT, 16>(v, blocks, s, x, n, dim, ma, mat, rowmax); break;
     }
 }
 
@@ -184,7 +136,7 @@ void fill_device_u32(u32* p, u64 count, u32 v, hipStream_t s) {
 
 // Stream-ordered distance matrix (+ row maxima, + enclosing radius).
 void dist_matrix(const float* d_x, u32 n, u32 dim, int kind, float p, int layout, float* d_mat,
-                 u32* d_rowmax, float* d_radius, hipStream_t s) {
+                 u32* d_rowmax, float* d_radius, hipStream_t s, int rpw_override) {
     check_metric(kind, p);
     if (layout != TT_LAYOUT_FULL && layout != TT_LAYOUT_LOWER)
         fail(TT_EINVAL, "dist_matrix: unknown layout " + std::to_string(layout));
@@ -194,15 +146,16 @@ void dist_matrix(const float* d_x, u32 n, u32 dim, int kind, float p, int layout
     if (d_radius && !d_rowmax) fail(TT_EINVAL, "dist_matrix: the radius needs a rowmax buffer");
     if (d_rowmax) fill_device_u32(d_rowmax, n, 0u, s);
     if (n) {
-        const u64 nb = (u64(n) + TILE - 1) / TILE;
+        const u64 nb = (u64(n) + DTILE - 1) / DTILE;
         const u64 blocks = nb * (nb + 1) / 2;
         if (blocks > 0x7fffffffull) fail(TT_ELENGTH, "dist_matrix: n too large for one launch");
         const MetricVariant v = variant_of(kind, p);
         const MetricArgs ma = metric_args(kind, p);
+        const int rpw = rpw_override > 0 ? rpw_override : tile_rows_per_wave();
         if (layout == TT_LAYOUT_FULL)
-            launch_variant<TT_LAYOUT_FULL>(v, u32(blocks), s, d_x, n, dim, ma, d_mat, d_rowmax);
+            launch_rpw<TT_LAYOUT_FULL>(rpw, v, u32(blocks), s, d_x, n, dim, ma, d_mat, d_rowmax);
         else
-            launch_variant<TT_LAYOUT_LOWER>(v, u32(blocks), s, d_x, n, dim, ma, d_mat, d_rowmax);
+            launch_rpw<TT_LAYOUT_LOWER>(rpw, v, u32(blocks), s, d_x, n, dim, ma, d_mat, d_rowmax);
         TTG_LAUNCH_CHECK();
     }
     if (d_radius) {

@@ -24,7 +24,7 @@
 //                  decides whether the edge is its apparent partner.
 #include <algorithm>
 
-#include "sort.hpp"
+#include "frontend.hpp"
 
 namespace ttg {
 
@@ -169,13 +169,6 @@ u32 grid_for(u64 waves) {
 
 }  // namespace
 
-struct DenseEdges {
-    u64 m = 0;
-    u64* idx = nullptr;     // device, sorted
-    float* d = nullptr;     // device, sorted
-    u8* apparent = nullptr; // device (null unless requested)
-};
-
 // Kept edges of a full device matrix, in filtration order, and optionally
 // their apparent flags. Buffers live in ws.
 DenseEdges dense_edges(const float* d_mat, u32 n, float thr, bool want_apparent, Workspace& ws,
@@ -220,9 +213,6 @@ DenseEdges dense_edges(const float* d_mat, u32 n, float thr, bool want_apparent,
     }
     return out;
 }
-
-void dist_matrix(const float* d_x, u32 n, u32 dim, int kind, float p, int layout, float* d_mat,
-                 u32* d_rowmax, float* d_radius, hipStream_t s);
 
 }  // namespace ttg
 

@@ -23,7 +23,7 @@
 #include <algorithm>
 
 #include "pairs.hpp"
-#include "sort.hpp"
+#include "frontend.hpp"
 
 namespace ttg {
 
@@ -263,7 +263,6 @@ void check_cloud(const float* pts, u32 n, u32 dim, int kind, float p, const char
 
 }  // namespace
 
-void fill_device_u32(u32* p, u64 count, u32 v, hipStream_t s);
 
 // CSR over n vertices from keys (row << cb | col) sorted ascending; deg[] holds
 // the row counts. Writes row_ptr (n + 1) into ws and returns it.

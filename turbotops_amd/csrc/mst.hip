@@ -27,7 +27,7 @@
 #include <vector>
 
 #include "pairs.hpp"
-#include "sort.hpp"
+#include "frontend.hpp"
 
 namespace ttg {
 
@@ -179,7 +179,6 @@ u32 blocks_of(u64 count) {
 
 }  // namespace
 
-void fill_device_u32(u32* p, u64 count, u32 v, hipStream_t s);
 
 struct MstStats {
     u32 rounds = 0;

@@ -63,14 +63,14 @@ __device__ __forceinline__ float pair_dist(const float* __restrict__ a, const fl
 
 // ---- per-pair kernels, runtime dimension (chunks of KC coordinates) -------
 // The chunking keeps each pair's accumulation strictly sequential in k.
-template <int V>
+template <int V, int R>
 __device__ __forceinline__ void rows_runtime(const float* __restrict__ x, u32 n, u32 dim,
                                              const MetricArgs& ma, u32 ibase, u32 jc,
-                                             float (&val)[ROWS]) {
+                                             float (&val)[R]) {
     const float* __restrict__ xb = x + u64(jc) * dim;
-    float acc[ROWS];
+    float acc[R];
 #pragma unroll
-    for (u32 r = 0; r < ROWS; ++r) acc[r] = 0.0f;
+    for (u32 r = 0; r < R; ++r) acc[r] = 0.0f;
     // pass 1: L1 / L2 sums, or the L-inf maximum (also Lp's M)
     for (u32 k0 = 0; k0 < dim; k0 += KC) {
         const u32 len = min(KC, dim - k0);
@@ -78,7 +78,7 @@ __device__ __forceinline__ void rows_runtime(const float* __restrict__ x, u32 n,
 #pragma unroll
         for (u32 t = 0; t < KC; ++t) b[t] = t < len ? xb[k0 + t] : 0.0f;
 #pragma unroll
-        for (u32 r = 0; r < ROWS; ++r) {
+        for (u32 r = 0; r < R; ++r) {
             const u32 i = min(ibase + r, n - 1);
             const float* __restrict__ a = x + u64(i) * dim + k0;
             float s = acc[r];
@@ -96,15 +96,15 @@ __device__ __forceinline__ void rows_runtime(const float* __restrict__ x, u32 n,
     }
     if constexpr (V == MV_L2) {
 #pragma unroll
-        for (u32 r = 0; r < ROWS; ++r) val[r] = __builtin_sqrtf(acc[r]);
+        for (u32 r = 0; r < R; ++r) val[r] = __builtin_sqrtf(acc[r]);
     } else if constexpr (V == MV_L1 || V == MV_LINF) {
 #pragma unroll
-        for (u32 r = 0; r < ROWS; ++r) val[r] = acc[r];
+        for (u32 r = 0; r < R; ++r) val[r] = acc[r];
     } else {
         // pass 2 (Lp): sum of normalised powers, then M * s^(1/p)
-        float inv[ROWS], s2[ROWS];
+        float inv[R], s2[R];
 #pragma unroll
-        for (u32 r = 0; r < ROWS; ++r) {
+        for (u32 r = 0; r < R; ++r) {
             inv[r] = acc[r] == 0.0f ? 0.0f : 1.0f / acc[r];
             s2[r] = 0.0f;
         }
@@ -114,7 +114,7 @@ __device__ __forceinline__ void rows_runtime(const float* __restrict__ x, u32 n,
 #pragma unroll
             for (u32 t = 0; t < KC; ++t) b[t] = t < len ? xb[k0 + t] : 0.0f;
 #pragma unroll
-            for (u32 r = 0; r < ROWS; ++r) {
+            for (u32 r = 0; r < R; ++r) {
                 const u32 i = min(ibase + r, n - 1);
                 const float* __restrict__ a = x + u64(i) * dim + k0;
                 float s = s2[r];
@@ -131,20 +131,20 @@ __device__ __forceinline__ void rows_runtime(const float* __restrict__ x, u32 n,
             }
         }
 #pragma unroll
-        for (u32 r = 0; r < ROWS; ++r)
+        for (u32 r = 0; r < R; ++r)
             val[r] = acc[r] == 0.0f ? 0.0f : acc[r] * pow_f32(s2[r], ma.inv_p);
     }
 }
 
-template <int V, int DIM>
+template <int V, int DIM, int R>
 __device__ __forceinline__ void rows_static(const float* __restrict__ x, u32 n,
                                             const MetricArgs& ma, u32 ibase, u32 jc,
-                                            float (&val)[ROWS]) {
+                                            float (&val)[R]) {
     float b[DIM];
 #pragma unroll
     for (int k = 0; k < DIM; ++k) b[k] = x[u64(jc) * DIM + k];
 #pragma unroll
-    for (u32 r = 0; r < ROWS; ++r) {
+    for (u32 r = 0; r < R; ++r) {
         const u32 i = min(ibase + r, n - 1);
         val[r] = pair_dist<V, DIM>(x + u64(i) * DIM, b, ma);
     }

@@ -16,6 +16,17 @@
 #include "sort.hpp"
 
 namespace ttg {
+
+u64 scan_tmp_entries(u64 count) {
+    constexpr u64 tile = 256 * 8;
+    u64 t = 0;
+    while (count > tile) {
+        count = (count + tile - 1) / tile;
+        t += count;
+    }
+    return t;
+}
+
 namespace {
 
 constexpr u32 SB = 256;        // scan block
@@ -105,9 +116,16 @@ __global__ __launch_bounds__(SB) void scan_apply(const T* in, T* out, u64 count,
 }
 
 template <typename T>
-void exclusive_scan(const T* in, T* out, u64 count, T* d_total, Workspace& ws, hipStream_t s) {
+__global__ void set_scalar(T* p, T v) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) *p = v;
+}
+
+// Exclusive scan with caller-provided temporaries (scan_tmp_entries(count)
+// entries): no allocation, so it can be recorded into a graph.
+template <typename T>
+void exclusive_scan_buf(const T* in, T* out, u64 count, T* d_total, T* tmp, hipStream_t s) {
     if (count == 0) {
-        if (d_total) TTG_HIP(hipMemsetAsync(d_total, 0, sizeof(T), s));
+        if (d_total) set_scalar<T><<<1, 64, 0, s>>>(d_total, T(0));
         return;
     }
     const u64 blocks = (count + STILE - 1) / STILE;
@@ -116,12 +134,18 @@ void exclusive_scan(const T* in, T* out, u64 count, T* d_total, Workspace& ws, h
         TTG_LAUNCH_CHECK();
         return;
     }
-    T* sums = ws.get<T>(blocks);
+    T* sums = tmp;
     scan_reduce<T><<<u32(blocks), SB, 0, s>>>(in, count, sums);
     TTG_LAUNCH_CHECK();
-    exclusive_scan<T>(sums, sums, blocks, nullptr, ws, s);
+    exclusive_scan_buf<T>(sums, sums, blocks, nullptr, tmp + blocks, s);
     scan_apply<T><<<u32(blocks), SB, 0, s>>>(in, out, count, sums, d_total);
     TTG_LAUNCH_CHECK();
+}
+
+template <typename T>
+void exclusive_scan(const T* in, T* out, u64 count, T* d_total, Workspace& ws, hipStream_t s) {
+    T* tmp = ws.get<T>(std::max<u64>(1, scan_tmp_entries(count)));
+    exclusive_scan_buf<T>(in, out, count, d_total, tmp, s);
 }
 
 // ---- radix sort ---------------------------------------------------------
@@ -136,9 +160,10 @@ __device__ __forceinline__ u32 lane_rank(u64 mask) {
 }
 
 __global__ __launch_bounds__(RB) void rs_upsweep(const u64* __restrict__ keys, u64 count,
-                                                 u32 shift, u32 dmask, u32* __restrict__ hist,
-                                                 u32 nblocks) {
+                                                 const u32* __restrict__ d_count, u32 shift,
+                                                 u32 dmask, u32* __restrict__ hist, u32 nblocks) {
     __shared__ u32 h[RB / 64][RADIX];
+    if (d_count) count = min<u64>(count, *d_count);
     for (u32 t = threadIdx.x; t < (RB / 64) * RADIX; t += RB) (&h[0][0])[t] = 0;
     __syncthreads();
     const u32 lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -160,6 +185,7 @@ __global__ __launch_bounds__(RB) void rs_upsweep(const u64* __restrict__ keys, u
 template <bool VALS>
 __global__ __launch_bounds__(RB) void rs_downsweep(const u64* __restrict__ keys,
                                                    const u32* __restrict__ vals, u64 count,
+                                                   const u32* __restrict__ d_count,
                                                    u32 shift, u32 dmask, u32 dbits,
                                                    const u32* __restrict__ offs, u32 nblocks,
                                                    u64* __restrict__ keys_out,
@@ -170,10 +196,12 @@ __global__ __launch_bounds__(RB) void rs_downsweep(const u64* __restrict__ keys,
     __shared__ u32 svals[VALS ? RTILE : 1];
 
     const u32 lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (d_count) count = min<u64>(count, *d_count);
+    const u64 tile0 = u64(blockIdx.x) * RTILE;
+    if (tile0 >= count) return;   // whole block idle (uniform)
     for (u32 t = threadIdx.x; t < (RB / 64) * RADIX; t += RB) (&wrun[0][0])[t] = 0;
     __syncthreads();
 
-    const u64 tile0 = u64(blockIdx.x) * RTILE;
     const u64 base = tile0 + u64(w) * WSEG;
     u64 k[RI];
     u32 v[RI];
@@ -248,6 +276,11 @@ __global__ __launch_bounds__(RB) void rs_downsweep(const u64* __restrict__ keys,
 
 }  // namespace
 
+void exclusive_scan_u32_buf(const u32* in, u32* out, u64 count, u32* d_total, u32* tmp,
+                            hipStream_t s) {
+    exclusive_scan_buf<u32>(in, out, count, d_total, tmp, s);
+}
+
 void exclusive_scan_u32(const u32* in, u32* out, u64 count, u32* d_total, Workspace& ws,
                         hipStream_t s) {
     exclusive_scan<u32>(in, out, count, d_total, ws, s);
@@ -257,6 +290,42 @@ void exclusive_scan_u64(const u64* in, u64* out, u64 count, u64* d_total, Worksp
                         hipStream_t s) {
     exclusive_scan<u64>(in, out, count, d_total, ws, s);
 }
+
+namespace {
+
+__global__ void copy_counted(const u64* __restrict__ src, u64* __restrict__ dst, u64 cap,
+                             const u32* __restrict__ d_count) {
+    const u64 count = min<u64>(cap, *d_count);
+    for (u64 t = blockIdx.x * u64(blockDim.x) + threadIdx.x; t < count; t += u64(gridDim.x) * blockDim.x)
+        dst[t] = src[t];
+}
+
+}  // namespace
+
+void radix_sort_u64_counted(u64* keys, u64 capacity, const u32* d_count, int lo_bit, int hi_bit,
+                            u64* keys_alt, u32* hist, u32* scan_tmp, hipStream_t s) {
+    if (capacity <= 1 || hi_bit <= lo_bit) return;
+    if (capacity >= (u64(1) << 32)) fail(TT_ELENGTH, "radix_sort: too many keys for one sort");
+    const u32 nblocks = u32((capacity + RTILE - 1) / RTILE);
+    u64 *ks = keys, *kd = keys_alt;
+    for (int shift = lo_bit; shift < hi_bit; shift += 8) {
+        const u32 dbits = u32(std::min(8, hi_bit - shift));
+        const u32 dmask = (1u << dbits) - 1u;
+        rs_upsweep<<<nblocks, RB, 0, s>>>(ks, capacity, d_count, u32(shift), dmask, hist, nblocks);
+        TTG_LAUNCH_CHECK();
+        exclusive_scan_buf<u32>(hist, hist, u64(RADIX) * nblocks, nullptr, scan_tmp, s);
+        rs_downsweep<false><<<nblocks, RB, 0, s>>>(ks, nullptr, capacity, d_count, u32(shift), dmask, dbits,
+                                                   hist, nblocks, kd, nullptr);
+        TTG_LAUNCH_CHECK();
+        std::swap(ks, kd);
+    }
+    if (ks != keys) {
+        copy_counted<<<4096, 256, 0, s>>>(ks, keys, capacity, d_count);
+        TTG_LAUNCH_CHECK();
+    }
+}
+
+u64 radix_hist_entries(u64 capacity) { return u64(RADIX) * ((capacity + RTILE - 1) / RTILE); }
 
 void radix_sort_u64(u64* keys, u32* vals, u64 count, int lo_bit, int hi_bit, u64* keys_alt,
                     u32* vals_alt, Workspace& ws, hipStream_t s) {
@@ -270,14 +339,14 @@ void radix_sort_u64(u64* keys, u32* vals, u64 count, int lo_bit, int hi_bit, u64
     for (int shift = lo_bit; shift < hi_bit; shift += 8, ++passes) {
         const u32 dbits = u32(std::min(8, hi_bit - shift));
         const u32 dmask = (1u << dbits) - 1u;
-        rs_upsweep<<<nblocks, RB, 0, s>>>(ks, count, u32(shift), dmask, hist, nblocks);
+        rs_upsweep<<<nblocks, RB, 0, s>>>(ks, count, nullptr, u32(shift), dmask, hist, nblocks);
         TTG_LAUNCH_CHECK();
         exclusive_scan_u32(hist, hist, u64(RADIX) * nblocks, nullptr, ws, s);
         if (vals)
-            rs_downsweep<true><<<nblocks, RB, 0, s>>>(ks, vs, count, u32(shift), dmask, dbits,
+            rs_downsweep<true><<<nblocks, RB, 0, s>>>(ks, vs, count, nullptr, u32(shift), dmask, dbits,
                                                       hist, nblocks, kd, vd);
         else
-            rs_downsweep<false><<<nblocks, RB, 0, s>>>(ks, nullptr, count, u32(shift), dmask,
+            rs_downsweep<false><<<nblocks, RB, 0, s>>>(ks, nullptr, count, nullptr, u32(shift), dmask,
                                                        dbits, hist, nblocks, kd, nullptr);
         TTG_LAUNCH_CHECK();
         std::swap(ks, kd);

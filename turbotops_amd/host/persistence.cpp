@@ -395,4 +395,77 @@ Result persistence(const f32* points, u32 n, u32 dim, const Options& o) {
     return r;
 }
 
+// ---- batches ---------------------------------------------------------------
+BatchRunner::BatchRunner(const std::vector<u32>& n, const std::vector<u32>& dim, const std::string& metric,
+                         u32 maxdim)
+    : n_(n), dim_(dim), maxdim_(maxdim) {
+    if (n.size() != dim.size()) throw std::invalid_argument("persistence_batch: n and dim differ in length");
+    if (maxdim != 1) throw std::invalid_argument("persistence_batch: maxdim must be 1 (H0 and H1)");
+    const Metric m = Metric::parse(metric);
+    check(tt_batch_create(u32(n.size()), n.data(), dim.data(), m.kind, m.p, INFINITY, &h_));
+}
+
+BatchRunner::~BatchRunner() {
+    if (h_) tt_batch_destroy(h_);
+}
+
+u64 BatchRunner::total_points() const {
+    u64 t = 0;
+    for (size_t c = 0; c < n_.size(); ++c) t += u64(n_[c]) * dim_[c];
+    return t;
+}
+
+std::vector<Result> BatchRunner::run(const f32* pts, bool use_graph, u32 threads) {
+    const auto t0 = std::chrono::steady_clock::now();
+    const u64 total = total_points();
+    for (u64 i = 0; i < total; ++i)
+        if (!std::isfinite(pts[i])) throw std::invalid_argument("persistence: non-finite coordinate");
+    check(tt_batch_run(h_, pts, use_graph ? 1 : 0));
+    check(tt_batch_info(h_, &device_ms_, &rpw_, &edges_total_, &edges_kept_));
+    const u32 nc = u32(n_.size());
+    std::vector<Result> out(nc);
+    std::vector<u32> order(nc);
+    for (u32 c = 0; c < nc; ++c) order[c] = c;
+    std::stable_sort(order.begin(), order.end(), [&](u32 a, u32 b) { return n_[a] > n_[b]; });
+    std::atomic<u32> next{0};
+    std::vector<std::string> errors(nc);
+    auto worker = [&] {
+        for (u32 k; (k = next.fetch_add(1)) < nc;) {
+            const u32 c = order[k];
+            try {
+                check(tt_batch_wait(h_, c));
+                const f32* mat = nullptr;
+                f32 thr = 0;
+                u64 m = 0;
+                const u64* idx = nullptr;
+                const f32* d = nullptr;
+                check(tt_batch_cloud(h_, c, &mat, &thr, &m, &idx, &d));
+                cob::DenseView cx;
+                cx.n = n_[c];
+                cx.threshold = thr;
+                cx.full = true;
+                cx.mat = mat;
+                Result& r = out[c];
+                reduce_all(cx, maxdim_, idx, d, nullptr, m, r.barcode, r.stats);
+                r.barcode.ensure(maxdim_);
+                r.stats.method = "dense-batch";
+                r.stats.engine = "coboundary";
+                r.stats.threshold = thr;
+            } catch (const std::exception& e) {
+                errors[c] = e.what();
+            }
+        }
+    };
+    const u32 nt = std::max<u32>(1, std::min<u32>(threads ? threads : num_threads(), nc));
+    std::vector<std::thread> pool;
+    for (u32 t = 1; t < nt; ++t) pool.emplace_back(worker);
+    worker();
+    for (auto& th : pool) th.join();
+    for (u32 c = 0; c < nc; ++c)
+        if (!errors[c].empty()) throw std::runtime_error("cloud " + std::to_string(c) + ": " + errors[c]);
+    const double secs = since(t0);
+    for (auto& r : out) r.stats.seconds_total = secs;
+    return out;
+}
+
 }  // namespace tops

@@ -195,6 +195,34 @@ py::tuple persistence_pairs(F32Array x, u32 maxdim, float threshold, const std::
     return py::make_tuple(bars, births, deaths, stats_dict(r.stats));
 }
 
+// BatchRunner: points as one packed float32 array (clouds concatenated) or a
+// list of (n_i, d_i) arrays.
+py::list batch_run(BatchRunner& br, py::object clouds, bool use_graph, u32 threads) {
+    std::vector<float> packed;
+    const float* p = nullptr;
+    if (py::isinstance<py::list>(clouds) || py::isinstance<py::tuple>(clouds)) {
+        for (auto h : clouds) {
+            F32Array a = py::cast<F32Array>(h);
+            packed.insert(packed.end(), a.data(), a.data() + a.size());
+        }
+        if (packed.size() != br.total_points()) throw py::value_error("clouds do not match the batch shapes");
+        p = packed.data();
+    } else {
+        F32Array a = py::cast<F32Array>(clouds);
+        if (u64(a.size()) != br.total_points()) throw py::value_error("clouds do not match the batch shapes");
+        packed.assign(a.data(), a.data() + a.size());
+        p = packed.data();
+    }
+    std::vector<Result> rs;
+    {
+        py::gil_scoped_release release;
+        rs = br.run(p, use_graph, threads);
+    }
+    py::list out;
+    for (const auto& r : rs) out.append(py::make_tuple(to_python(r.barcode), stats_dict(r.stats)));
+    return out;
+}
+
 }  // namespace
 
 PYBIND11_MODULE(_turbotops, m) {
@@ -207,6 +235,14 @@ PYBIND11_MODULE(_turbotops, m) {
           py::arg("n"), py::arg("maxdim"), py::arg("threshold"), py::arg("threads"));
     m.def("persistence_pairs", &persistence_pairs, py::arg("x"), py::arg("maxdim"), py::arg("threshold"),
           py::arg("metric"), py::arg("graph"), py::arg("k"), py::arg("threads"), py::arg("memory_budget"));
+    py::class_<BatchRunner>(m, "BatchRunner")
+        .def(py::init<const std::vector<u32>&, const std::vector<u32>&, const std::string&, u32>(), py::arg("n"),
+             py::arg("dim"), py::arg("metric") = "l2", py::arg("maxdim") = 1)
+        .def("run", &batch_run, py::arg("clouds"), py::arg("use_graph") = true, py::arg("threads") = 0)
+        .def_property_readonly("device_ms", &BatchRunner::device_ms)
+        .def_property_readonly("rows_per_wave", &BatchRunner::rows_per_wave)
+        .def_property_readonly("edges_total", &BatchRunner::edges_total)
+        .def_property_readonly("edges_kept", &BatchRunner::edges_kept);
     py::register_exception_translator([](std::exception_ptr p) {
         try {
             if (p) std::rethrow_exception(p);

@@ -27,6 +27,7 @@
 #include <cstring>
 #include <stdexcept>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 namespace tops::cob {
@@ -336,7 +337,7 @@ class Cohomology {
 
     void run_from_dim1() {
         for (u32 dim = 1; dim <= maxdim_; ++dim) {
-            reduce(dim);
+            if (!(dim == 1 && reduce_dim1_lazy())) reduce(dim);
             if (dim < maxdim_) assemble(dim);
         }
     }
@@ -490,6 +491,327 @@ class Cohomology {
             return true;
         });
     }
+
+
+    // ---- dimension 1 on a dense full matrix: lazy cofacet streams --------
+    // The same column reduction as reduce(1) — same columns, same pivots,
+    // same pairs — with the working coboundary held as a merge of sorted
+    // streams, one per added edge, instead of a heap of every cofacet. A
+    // stream yields the cofacets of edge (i, j) in filtration order:
+    //   phase A  diameter d(i, j): w with max(d(i,w), d(j,w)) <= d(i, j), by
+    //            descending w (descending cofacet index);
+    //   phase B  diameter max(d(i,w), d(j,w)) > d(i, j), ascending, from the
+    //            rows of i and j sorted by distance; equal diameters by
+    //            descending w; stops past the threshold.
+    // Only the entries up to the current pivot are ever generated.
+    struct EdgeStream {
+        u32 i = 0, j = 0;
+        f32 d = 0;
+        u32 w = 0;                 // phase A: candidates below w remain
+        u32 pi = 0, pj = 0;        // phase B cursors into sorted rows
+        int phase = 0;             // 0 A, 1 B, 2 done
+        bool has_pending = false;  // phase B one-entry lookahead
+        u32 pend_w = 0;
+        f32 pend_k = 0;
+        std::vector<u32> tie;      // phase B group of equal diameters (descending w)
+        u32 tie_pos = 0;
+        f32 tie_k = 0;
+        S head{};
+    };
+
+    const std::vector<u32>& sorted_row(u32 v) {
+        if (srow_.empty()) srow_.resize(n_);
+        std::vector<u32>& r = srow_[v];
+        if (r.empty() && n_ > 1) {
+            r.reserve(n_ - 1);
+            for (u32 w = 0; w < n_; ++w)
+                if (w != v) r.push_back(w);
+            const f32* row = cx_.mat + u64(v) * n_;
+            std::sort(r.begin(), r.end(), [row](u32 a, u32 b) { return row[a] < row[b] || (row[a] == row[b] && a < b); });
+        }
+        return r;
+    }
+
+    I tri_index(u32 i, u32 j, u32 w) const {   // i > j
+        if (w > i) return B_(w, 3) + B_(i, 2) + B_(j, 1);
+        if (w > j) return B_(i, 3) + B_(w, 2) + B_(j, 1);
+        return B_(i, 3) + B_(j, 2) + B_(w, 1);
+    }
+
+    // next phase-B candidate (w, key) in ascending key, or false
+    bool next_b(EdgeStream& s, u32& w_out, f32& k_out) {
+        const std::vector<u32>& ri = sorted_row(s.i);
+        const std::vector<u32>& rj = sorted_row(s.j);
+        const f32* di = cx_.mat + u64(s.i) * n_;
+        const f32* dj = cx_.mat + u64(s.j) * n_;
+        for (;;) {
+            const f32 ai = s.pi < ri.size() ? di[ri[s.pi]] : INFINITY;
+            const f32 aj = s.pj < rj.size() ? dj[rj[s.pj]] : INFINITY;
+            if (s.pi >= ri.size() && s.pj >= rj.size()) return false;
+            if (s.pi < ri.size() && (s.pj >= rj.size() || ai <= aj)) {
+                const u32 w = ri[s.pi++];
+                if (w == s.j) continue;
+                if (dj[w] <= ai) {
+                    w_out = w;
+                    k_out = ai;
+                    return true;
+                }
+            } else {
+                const u32 w = rj[s.pj++];
+                if (w == s.i) continue;
+                if (di[w] < aj) {
+                    w_out = w;
+                    k_out = aj;
+                    return true;
+                }
+            }
+        }
+    }
+
+    // first position in a sorted row with distance > d
+    u32 first_above(u32 v, f32 d) {
+        const std::vector<u32>& r = sorted_row(v);
+        const f32* row = cx_.mat + u64(v) * n_;
+        return u32(std::upper_bound(r.begin(), r.end(), d, [row](f32 x, u32 w) { return x < row[w]; }) - r.begin());
+    }
+
+    bool advance(EdgeStream& s) {
+        const f32* di = cx_.mat + u64(s.i) * n_;
+        const f32* dj = cx_.mat + u64(s.j) * n_;
+        if (s.phase == 0) {
+            while (s.w > 0) {
+                const u32 w = --s.w;
+                if (w == s.i || w == s.j) continue;
+                const f32 a = di[w], b = dj[w];
+                if ((a < b ? b : a) <= s.d) {
+                    s.head = S{tri_index(s.i, s.j, w), s.d};
+                    return true;
+                }
+            }
+            s.phase = 1;
+            s.pi = first_above(s.i, s.d);
+            s.pj = first_above(s.j, s.d);
+            s.has_pending = false;
+            s.tie.clear();
+            s.tie_pos = 0;
+        }
+        if (s.phase == 1) {
+            if (s.tie_pos < s.tie.size()) {
+                s.head = S{tri_index(s.i, s.j, s.tie[s.tie_pos++]), s.tie_k};
+                return true;
+            }
+            u32 w;
+            f32 k;
+            if (s.has_pending) {
+                w = s.pend_w;
+                k = s.pend_k;
+                s.has_pending = false;
+            } else if (!next_b(s, w, k)) {
+                s.phase = 2;
+                return false;
+            }
+            if (!(k <= cx_.threshold)) {
+                s.phase = 2;
+                return false;
+            }
+            s.tie.clear();
+            s.tie.push_back(w);
+            u32 w2;
+            f32 k2;
+            while (next_b(s, w2, k2)) {
+                if (k2 == k) {
+                    s.tie.push_back(w2);
+                } else {
+                    s.has_pending = true;
+                    s.pend_w = w2;
+                    s.pend_k = k2;
+                    break;
+                }
+            }
+            if (s.tie.size() > 1) std::sort(s.tie.begin(), s.tie.end(), [](u32 a, u32 b) { return a > b; });
+            s.tie_k = k;
+            s.tie_pos = 1;
+            s.head = S{tri_index(s.i, s.j, s.tie[0]), k};
+            return true;
+        }
+        return false;
+    }
+
+    struct HeapItem {
+        S e;
+        u32 stream;
+    };
+    static bool item_later(const HeapItem& a, const HeapItem& b) { return before(b.e, a.e); }
+
+    void lazy_push_stream(const S& edge) {
+        u32 id;
+        if (free_.empty()) {
+            id = u32(pool_.size());
+            pool_.emplace_back();
+        } else {
+            id = free_.back();
+            free_.pop_back();
+        }
+        EdgeStream& s = pool_[id];
+        decode_edge(u64(edge.idx), s.i, s.j);
+        s.d = edge.d;
+        s.w = n_;
+        s.phase = 0;
+        s.has_pending = false;
+        s.tie.clear();
+        s.tie_pos = 0;
+        if (advance(s)) {
+            lheap_.push_back(HeapItem{s.head, id});
+            std::push_heap(lheap_.begin(), lheap_.end(), item_later);
+        } else {
+            free_.push_back(id);
+        }
+    }
+
+    void lazy_step(u32 id) {
+        if (advance(pool_[id])) {
+            lheap_.push_back(HeapItem{pool_[id].head, id});
+            std::push_heap(lheap_.begin(), lheap_.end(), item_later);
+        } else {
+            free_.push_back(id);
+        }
+    }
+
+    HeapItem lazy_pop() {
+        std::pop_heap(lheap_.begin(), lheap_.end(), item_later);
+        const HeapItem it = lheap_.back();
+        lheap_.pop_back();
+        return it;
+    }
+
+    // earliest surviving entry after mod-2 cancellation (left in the merge)
+    S lazy_pivot() {
+        while (!lheap_.empty()) {
+            const HeapItem top = lazy_pop();
+            group_.clear();
+            while (!lheap_.empty() && lheap_.front().e.idx == top.e.idx) group_.push_back(lazy_pop().stream);
+            if (group_.size() % 2 == 0) {   // top survives
+                for (u32 g : group_) lazy_step(g);
+                lheap_.push_back(top);
+                std::push_heap(lheap_.begin(), lheap_.end(), item_later);
+                return top.e;
+            }
+            lazy_step(top.stream);
+            for (u32 g : group_) lazy_step(g);
+        }
+        return S{kNone<I>, 0.0f};
+    }
+
+    // zero_pivot_cofacet of an edge, cached by edge index
+    I edge_zero_cofacet(u64 eidx, f32 d) {
+        if (zc_.key.empty()) zc_.reset(1024);
+        const u64 hit = zc_.find(I(eidx));
+        if (hit != ~0ull) return I(hit);
+        u32 i, j;
+        decode_edge(eidx, i, j);
+        const f32* di = cx_.mat + u64(i) * n_;
+        const f32* dj = cx_.mat + u64(j) * n_;
+        I found = kNone<I>;
+        for (u32 w = n_; w-- > 0;) {
+            if (w == i || w == j) continue;
+            const f32 a = di[w], b = dj[w];
+            if ((a < b ? b : a) <= d) {
+                found = tri_index(i, j, w);
+                break;
+            }
+        }
+        zc_.put(I(eidx), u64(found));
+        return found;
+    }
+
+    // (f, t) apparent with triangle t the cofacet: f, else none
+    S apparent_edge_of_triangle(const S& t) {
+        u32 v[3];
+        decode(t.idx, 3, v);
+        const f32 d01 = cx_.dist(v[0], v[1]), d02 = cx_.dist(v[0], v[2]), d12 = cx_.dist(v[1], v[2]);
+        // zero pivot facet: facets in ascending index: drop v0 (v1 v2), drop v1 (v0 v2), drop v2 (v0 v1)
+        u32 a, b;
+        if (d12 == t.d) { a = v[1]; b = v[2]; }
+        else if (d02 == t.d) { a = v[0]; b = v[2]; }
+        else if (d01 == t.d) { a = v[0]; b = v[1]; }
+        else return S{kNone<I>, 0.0f};
+        const u64 e = u64(a) * (a - 1) / 2 + b;
+        return edge_zero_cofacet(e, t.d) == t.idx ? S{I(e), t.d} : S{kNone<I>, 0.0f};
+    }
+
+    bool reduce_dim1_lazy() {
+        if constexpr (!std::is_same_v<Cx, DenseView> || !std::is_same_v<I, u64>) {
+            return false;
+        } else {
+            if (!cx_.full) return false;
+            pivots_.reset(cols_.size());
+            red_off_.assign(1, 0);
+            red_.clear();
+            u64 emergent = 0;
+            std::vector<S> added;
+            for (u64 c = 0; c < cols_.size(); ++c) {
+                const S col = cols_[c];
+                lheap_.clear();
+                free_.clear();
+                for (u32 id = 0; id < pool_.size(); ++id) free_.push_back(id);
+                added.clear();
+                // emergent pair: the first cofacet of equal diameter (phase A head)
+                lazy_push_stream(col);
+                if (!lheap_.empty() && lheap_.front().e.d == col.d) {
+                    const S cf = lheap_.front().e;
+                    if (!pivots_.has(cf.idx) && apparent_edge_of_triangle(cf).idx == kNone<I>) {
+                        pivots_.put(cf.idx, c);
+                        ++emergent;
+                        red_off_.push_back(red_.size());
+                        continue;
+                    }
+                }
+                S piv = lazy_pivot();
+                for (;;) {
+                    if (piv.idx == kNone<I>) {
+                        out_.add(1, col.d, INFINITY, u128(col.idx), u128(kNone<I>));
+                        break;
+                    }
+                    const u64 jc = pivots_.find(piv.idx);
+                    if (jc != ~0ull) {
+                        added.push_back(cols_[jc]);
+                        lazy_push_stream(cols_[jc]);
+                        for (u64 t = red_off_[jc]; t < red_off_[jc + 1]; ++t) {
+                            added.push_back(red_[t]);
+                            lazy_push_stream(red_[t]);
+                        }
+                        piv = lazy_pivot();
+                        continue;
+                    }
+                    const S fa = apparent_edge_of_triangle(piv);
+                    if (fa.idx != kNone<I>) {
+                        added.push_back(fa);
+                        lazy_push_stream(fa);
+                        piv = lazy_pivot();
+                        continue;
+                    }
+                    if (piv.d > col.d) out_.add(1, col.d, piv.d, u128(col.idx), u128(piv.idx));
+                    pivots_.put(piv.idx, c);
+                    std::sort(added.begin(), added.end(), [](const S& a, const S& b) { return a.idx < b.idx; });
+                    for (size_t x = 0; x < added.size();) {
+                        if (x + 1 < added.size() && added[x].idx == added[x + 1].idx) x += 2;
+                        else red_.push_back(added[x++]);
+                    }
+                    break;
+                }
+                red_off_.push_back(red_.size());
+            }
+            tel_.columns.push_back(cols_.size());
+            tel_.emergent.push_back(emergent);
+            return true;
+        }
+    }
+
+    std::vector<std::vector<u32>> srow_;
+    std::vector<EdgeStream> pool_;
+    std::vector<u32> free_, group_;
+    std::vector<HeapItem> lheap_;
+    PivotTable<I> zc_;
 
     void reduce(u32 dim) {
         const u32 nv = dim + 1;

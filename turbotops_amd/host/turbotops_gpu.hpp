@@ -155,6 +155,34 @@ Result persistence(const f32* points, u32 n, u32 dim, const Options& o);
 Result persistence(const CsrGraph& g, u32 maxdim, f32 threshold = INFINITY, u32 threads = 0,
                    Mode mode = Mode::Auto);
 
+// Many clouds through one batched front end (include/ttgpu.h tt_batch_*):
+// dense full Rips at each cloud's enclosing radius, maxdim 1. Create once per
+// set of shapes; run() reduces the clouds on host threads (largest first),
+// each as soon as its data has landed. Results equal persistence() on each
+// cloud alone.
+class BatchRunner {
+  public:
+    BatchRunner(const std::vector<u32>& n, const std::vector<u32>& dim, const std::string& metric,
+                u32 maxdim = 1);
+    ~BatchRunner();
+    BatchRunner(const BatchRunner&) = delete;
+    BatchRunner& operator=(const BatchRunner&) = delete;
+    std::vector<Result> run(const f32* packed_points, bool use_graph = true, u32 threads = 0);
+    float device_ms() const { return device_ms_; }
+    int rows_per_wave() const { return rpw_; }
+    u64 edges_total() const { return edges_total_; }
+    u64 edges_kept() const { return edges_kept_; }
+    u64 total_points() const;
+
+  private:
+    tt_batch* h_ = nullptr;
+    std::vector<u32> n_, dim_;
+    u32 maxdim_ = 1;
+    float device_ms_ = 0;
+    int rpw_ = 0;
+    u64 edges_total_ = 0, edges_kept_ = 0;
+};
+
 void set_num_threads(u32 t);
 u32 num_threads();
 u64 default_memory_budget();
