@@ -183,6 +183,28 @@ tt_status tt_batch_info(tt_batch* b, float* device_ms, int* rows_per_wave, uint6
                         uint64_t* edges_kept);
 void tt_batch_destroy(tt_batch* b);
 
+/* ---- differentiation ------------------------------------------------------- */
+
+/* Backward of a diagram with respect to the points. The reference computes
+ * diagrams only (rips.hpp:52-67: Barcode keeps bar values, not simplices);
+ * every finite bar value is the length of a critical edge (i, j) — the
+ * longest edge of the bar's birth or death simplex — so
+ *   dx[a] += grad[e] * d dist(x_a, x_b) / d x_a   for edge e = (a, b), both ends,
+ * with the derivative of the metric `kind`/`p` (metric.hpp:92-207 define the
+ * distances): L2 (x_a - x_b)/d (0 at d = 0); L1 sign(x_a - x_b); L-inf the sign
+ * on the first coordinate of largest |x_a - x_b|; Lp sign * |diff|^(p-1)/d^(p-1).
+ * edges: m pairs (2m u32, endpoints < n); grad: m floats; dx: n*dim floats,
+ * overwritten. Deterministic: contributions are grouped per point by a sort
+ * and summed in FP64, rounded once. */
+tt_status tt_diagram_backward(const float* pts, uint32_t n, uint32_t dim, int kind, float p,
+                              const uint32_t* edges, const float* grad, uint32_t m, float* dx);
+
+/* Device form (HBM pointers, stream as hipStream_t); returns after the
+ * stream has finished the backward. */
+tt_status tt_diagram_backward_dev(const float* d_pts, uint32_t n, uint32_t dim, int kind, float p,
+                                  const uint32_t* d_edges, const float* d_grad, uint32_t m,
+                                  float* d_dx, void* stream);
+
 /* Releases a buffer the library allocated for a host-side result. */
 void tt_free(void* p);
 

@@ -272,3 +272,78 @@ def test_planted_mistakes_are_caught(restatement, golden):
     nofma = np.sqrt(((x[:, None, :] - x[None, :, :]) ** 2).sum(-1, dtype=np.float32)).astype(np.float32)
     assert np.array_equal(bits(got), bits(want))
     assert (bits(nofma) != bits(want)).sum() > 0   # a non-fma sum is caught bitwise
+
+
+# ---- the CPU reference gradient (oracle/grad.py) -----------------------------
+
+GRAD_METRICS = ["l2", "l1", "linf", "lp:1.5", "lp:3", "lp:7.5"]
+
+
+def _grad_case(seed, n=40, m=60, dim=4):
+    rng = np.random.default_rng(seed)
+    x = rng.normal(size=(n, dim)).astype(np.float32)
+    e = rng.integers(0, n, size=(m, 2))
+    e = e[e[:, 0] != e[:, 1]]
+    e = np.concatenate([e, e[:3], e[:2, ::-1]])      # repeated and reversed edges
+    g = rng.normal(size=e.shape[0])
+    return x, e, g
+
+
+def _fd64(x, e, g, metric, h=1e-6):
+    from oracle.grad import dist64
+    x64 = x.astype(np.float64)
+    out = np.zeros_like(x64)
+    loss = lambda z: float((g * dist64(z, e[:, 0], e[:, 1], metric)).sum())
+    for i in range(x.shape[0]):
+        for k in range(x.shape[1]):
+            zp, zm = x64.copy(), x64.copy()
+            zp[i, k] += h
+            zm[i, k] -= h
+            out[i, k] = (loss(zp) - loss(zm)) / (2 * h)
+    return out
+
+
+@pytest.mark.parametrize("metric", GRAD_METRICS)
+def test_grad_oracle_central_differences(metric):
+    """Central differences in float64 of L = sum g_e d(x_a, x_b) pin every
+    metric's derivative (generic points: no ties, no zero distances)."""
+    from oracle.grad import edge_grad64
+    x, e, g = _grad_case(21)
+    want = _fd64(x, e, g, metric)
+    got = edge_grad64(x, e, g, metric)
+    np.testing.assert_allclose(got, want, rtol=1e-6, atol=1e-6)
+
+
+@pytest.mark.parametrize("metric,order", [("l2", 2), ("l1", 1), ("linf", math.inf)])
+def test_grad_oracle_matches_torch_autograd(metric, order):
+    import torch
+    from oracle.grad import edge_grad64
+    x, e, g = _grad_case(22)
+    xt = torch.tensor(x, dtype=torch.float64, requires_grad=True)
+    d = torch.linalg.vector_norm(xt[e[:, 0]] - xt[e[:, 1]], ord=order, dim=1)
+    (torch.tensor(g) * d).sum().backward()
+    np.testing.assert_allclose(edge_grad64(x, e, g, metric), xt.grad.numpy(), rtol=1e-12, atol=1e-12)
+
+
+def test_grad_oracle_degenerate_and_planted():
+    """A zero-length edge contributes nothing; a planted sign flip, a dropped
+    endpoint or a wrong Lp exponent each fail the difference check."""
+    from oracle.grad import edge_grad64
+    x, e, g = _grad_case(23)
+    x[1] = x[0]
+    z = edge_grad64(x, np.array([[0, 1], [1, 1]]), np.array([2.0, 3.0]), "l2")
+    assert not z.any()
+    want = _fd64(x, e[e[:, 0] + e[:, 1] > 1], g[e[:, 0] + e[:, 1] > 1], "lp:3")
+    ok = edge_grad64(x, e[e[:, 0] + e[:, 1] > 1], g[e[:, 0] + e[:, 1] > 1], "lp:3")
+    np.testing.assert_allclose(ok, want, rtol=1e-6, atol=1e-6)
+    keep = e[:, 0] + e[:, 1] > 1
+    ek, gk = e[keep], g[keep]
+    flipped = edge_grad64(x, ek[:, ::-1], -gk, "lp:3")          # sign error
+    assert not np.allclose(flipped, want, rtol=1e-6, atol=1e-6)
+    dropped = np.zeros_like(ok)                                  # endpoint b forgotten
+    for (a, b), gv in zip(ek, gk):
+        one = edge_grad64(x, np.array([[a, b]]), np.array([gv]), "lp:3")
+        dropped[a] += one[a] if a != b else 0
+    assert not np.allclose(dropped, want, rtol=1e-6, atol=1e-6)
+    wrong_p = edge_grad64(x, ek, gk, "lp:2.9")
+    assert not np.allclose(wrong_p, want, rtol=1e-6, atol=1e-6)

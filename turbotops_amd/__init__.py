@@ -2,8 +2,9 @@
 
 A drop-in for the data-parallel front end of dnbaker/turbotops v0.1.0:
 Minkowski distance matrices, the enclosing radius, exact radius graphs,
-exact kNN and kNN graphs, H0 via Boruvka, and the apparent-pair scan, as
-hand-written HIP kernels behind the C ABI in include/ttgpu.h.
+exact kNN and kNN graphs, H0 via Boruvka, the apparent-pair scan, a batched
+form for many clouds and the backward of a diagram, as hand-written HIP
+kernels behind the C ABI in include/ttgpu.h.
 """
 from . import capi  # noqa: F401
 from .api import BatchPersistence, persistence, persistence_batch, persistence_pairs  # noqa: F401
@@ -21,5 +22,6 @@ from .frontend import (  # noqa: F401
     parse_metric,
     radius_graph,
 )
+from .grad import DiagramBatch, diagram, diagram_backward, top_k_persistence_loss  # noqa: F401
 
 __version__ = "0.1.0"

@@ -77,11 +77,13 @@ class BatchPersistence:
         self._r = ext.BatchRunner([n for n, _ in self.shapes], [d for _, d in self.shapes], str(metric),
                                   int(maxdim))
 
-    def __call__(self, clouds, use_graph=True, threads=0):
+    def __call__(self, clouds, use_graph=True, threads=0, pairs=False):
+        """With pairs=True each entry is (dgms, birth_edges, death_edges, info)
+        as persistence_pairs returns it."""
         clouds = [np.ascontiguousarray(c, dtype=np.float32) for c in clouds]
         if [c.shape for c in clouds] != self.shapes:
             raise ValueError("clouds do not match the batch shapes")
-        return self._r.run(clouds, bool(use_graph), int(threads))
+        return self._r.run(clouds, bool(use_graph), int(threads), bool(pairs))
 
     @property
     def device_ms(self):

@@ -47,6 +47,9 @@ _SIGS = {
     "tt_dense_edges": ([_vp, C.c_uint32, C.c_uint32, C.c_int, C.c_float, C.c_float, _u64p, _vp, _vp, _vp,
                         _f32p], None),
     "tt_dense_edges_dev": ([_vp, C.c_uint32, C.c_float, C.c_int, C.c_uint64, _u64p, _vp, _vp, _vp, _vp], None),
+    "tt_diagram_backward": ([_vp, C.c_uint32, C.c_uint32, C.c_int, C.c_float, _vp, _vp, C.c_uint32, _vp], None),
+    "tt_diagram_backward_dev": ([_vp, C.c_uint32, C.c_uint32, C.c_int, C.c_float, _vp, _vp, C.c_uint32, _vp,
+                                 _vp], None),
     "tt_free": ([_vp], C.c_int),
 }
 

@@ -415,7 +415,66 @@ u64 BatchRunner::total_points() const {
     return t;
 }
 
-std::vector<Result> BatchRunner::run(const f32* pts, bool use_graph, u32 threads) {
+namespace {
+
+// vertices of simplex `idx` with nv vertices, descending (combinatorial
+// number system, as Binomials indexes them)
+void simplex_vertices(const cob::Binomials<cob::u128>& B, u32 n, cob::u128 idx, u32 nv, u32* out) {
+    u32 top = n;
+    for (u32 i = 0; i < nv; ++i) {
+        const u32 k = nv - i;
+        u32 lo = k - 1, hi = top;
+        while (hi - lo > 1) {
+            const u32 mid = lo + (hi - lo) / 2;
+            if (B(mid, k) <= idx) lo = mid;
+            else hi = mid;
+        }
+        out[i] = lo;
+        idx -= B(lo, k);
+        top = lo;
+    }
+}
+
+void longest_edge(const cob::Binomials<cob::u128>& B, u32 n, cob::u128 idx, u32 nv,
+                  const std::function<f32(u32, u32)>& dist, int64_t* e) {
+    u32 vd[34];
+    simplex_vertices(B, n, idx, nv, vd);
+    f32 best = -1.0f;
+    for (u32 a = 0; a < nv; ++a)
+        for (u32 b = a + 1; b < nv; ++b) {
+            const f32 dd = dist(vd[a], vd[b]);
+            if (dd > best) {
+                best = dd;
+                e[0] = vd[a];
+                e[1] = vd[b];
+            }
+        }
+}
+
+}  // namespace
+
+CriticalEdges critical_edges(const Barcode& bc, u32 n, const std::function<f32(u32, u32)>& dist) {
+    CriticalEdges ce;
+    const u32 dims = u32(bc.birth.size());
+    ce.birth.resize(dims);
+    ce.death.resize(dims);
+    if (!dims) return ce;
+    cob::Binomials<cob::u128> B(std::max<u32>(n, 1), dims + 1);
+    for (u32 d = 0; d < dims; ++d) {
+        const size_t m = bc.birth[d].size();
+        ce.birth[d].assign(2 * m, -1);
+        ce.death[d].assign(2 * m, -1);
+        for (size_t i = 0; i < m; ++i) {
+            if (d > 0) longest_edge(B, n, bc.birth_simplex[d][i], d + 1, dist, &ce.birth[d][2 * i]);
+            if (std::isfinite(bc.death[d][i]))
+                longest_edge(B, n, bc.death_simplex[d][i], d + 2, dist, &ce.death[d][2 * i]);
+        }
+    }
+    return ce;
+}
+
+std::vector<Result> BatchRunner::run(const f32* pts, bool use_graph, u32 threads,
+                                     std::vector<CriticalEdges>* edges) {
     const auto t0 = std::chrono::steady_clock::now();
     const u64 total = total_points();
     for (u64 i = 0; i < total; ++i)
@@ -424,6 +483,7 @@ std::vector<Result> BatchRunner::run(const f32* pts, bool use_graph, u32 threads
     check(tt_batch_info(h_, &device_ms_, &rpw_, &edges_total_, &edges_kept_));
     const u32 nc = u32(n_.size());
     std::vector<Result> out(nc);
+    if (edges) edges->assign(nc, {});
     std::vector<u32> order(nc);
     for (u32 c = 0; c < nc; ++c) order[c] = c;
     std::stable_sort(order.begin(), order.end(), [&](u32 a, u32 b) { return n_[a] > n_[b]; });
@@ -451,6 +511,11 @@ std::vector<Result> BatchRunner::run(const f32* pts, bool use_graph, u32 threads
                 r.stats.method = "dense-batch";
                 r.stats.engine = "coboundary";
                 r.stats.threshold = thr;
+                if (edges) {
+                    const u64 nn = n_[c];
+                    (*edges)[c] = critical_edges(r.barcode, n_[c],
+                                                 [&](u32 a, u32 b) { return mat[u64(a) * nn + b]; });
+                }
             } catch (const std::exception& e) {
                 errors[c] = e.what();
             }

@@ -119,37 +119,21 @@ py::tuple persistence_csr(I64Array indptr, I64Array indices, F32Array data, u32 
     return py::make_tuple(to_python(r.barcode), stats_dict(r.stats));
 }
 
-// The longest edge of a simplex (ties: the first pair in vertex order), i.e.
-// the edge whose length is the simplex's filtration value.
-std::pair<int64_t, int64_t> critical_edge(const cob::Binomials<cob::u128>& B, u32 n, cob::u128 idx, u32 nv,
-                                          const float* x, u32 dim, const std::function<float(u32, u32)>& dist) {
-    u32 vd[34];
-    u32 top = n;
-    for (u32 i = 0; i < nv; ++i) {
-        const u32 k = nv - i;
-        u32 lo = k - 1, hi = top;
-        while (hi - lo > 1) {
-            const u32 mid = lo + (hi - lo) / 2;
-            if (B(mid, k) <= idx) lo = mid;
-            else hi = mid;
-        }
-        vd[i] = lo;
-        idx -= B(lo, k);
-        top = lo;
+// critical edges of one dimension as (m, 2) int64 arrays
+py::array_t<int64_t> edge_array(const std::vector<int64_t>& flat) {
+    const size_t m = flat.size() / 2;
+    py::array_t<int64_t> a({m, size_t(2)});
+    std::copy(flat.begin(), flat.end(), a.mutable_data());
+    return a;
+}
+
+py::tuple edges_to_python(const CriticalEdges& ce) {
+    py::list births, deaths;
+    for (size_t d = 0; d < ce.birth.size(); ++d) {
+        births.append(edge_array(ce.birth[d]));
+        deaths.append(edge_array(ce.death[d]));
     }
-    (void)x;
-    (void)dim;
-    float best = -1.0f;
-    std::pair<int64_t, int64_t> e{-1, -1};
-    for (u32 a = 0; a < nv; ++a)
-        for (u32 b = a + 1; b < nv; ++b) {
-            const float dd = dist(vd[a], vd[b]);
-            if (dd > best) {
-                best = dd;
-                e = {vd[a], vd[b]};
-            }
-        }
-    return e;
+    return py::make_tuple(births, deaths);
 }
 
 // persistence on a point cloud, returning per dimension the bars and, per
@@ -163,41 +147,20 @@ py::tuple persistence_pairs(F32Array x, u32 maxdim, float threshold, const std::
     const Result r = run_points(x, o);
     const u32 n = u32(x.shape(0)), dim = u32(x.shape(1));
     const Metric mt = Metric::parse(metric);
-    // exact pair distances for the tie-break between equally long edges
+    // the filtration's own distances, for the tie-break between equally long edges
     std::vector<float> full(size_t(n) * n);
     float radius = 0;
     check(tt_make_dense(x.data(), n, dim, mt.kind, mt.p, TT_LAYOUT_FULL, full.data(), &radius));
-    auto dist = [&](u32 a, u32 b) { return full[size_t(a) * n + b]; };
-    cob::Binomials<cob::u128> B(n, maxdim + 2);
-    py::list bars = to_python(r.barcode), births, deaths;
-    for (size_t d = 0; d < r.barcode.birth.size(); ++d) {
-        const size_t m = r.barcode.birth[d].size();
-        py::array_t<int64_t> cb({m, size_t(2)}), cd({m, size_t(2)});
-        auto vb = cb.mutable_unchecked<2>();
-        auto vdd = cd.mutable_unchecked<2>();
-        for (size_t i = 0; i < m; ++i) {
-            std::pair<int64_t, int64_t> eb{-1, -1}, ed{-1, -1};
-            const cob::u128 bs = r.barcode.birth_simplex[d][i], ds = r.barcode.death_simplex[d][i];
-            if (d == 0) {
-                if (std::isfinite(r.barcode.death[d][i])) ed = critical_edge(B, n, ds, 2, x.data(), dim, dist);
-            } else {
-                eb = critical_edge(B, n, bs, u32(d) + 1, x.data(), dim, dist);
-                if (std::isfinite(r.barcode.death[d][i])) ed = critical_edge(B, n, ds, u32(d) + 2, x.data(), dim, dist);
-            }
-            vb(i, 0) = eb.first;
-            vb(i, 1) = eb.second;
-            vdd(i, 0) = ed.first;
-            vdd(i, 1) = ed.second;
-        }
-        births.append(cb);
-        deaths.append(cd);
-    }
-    return py::make_tuple(bars, births, deaths, stats_dict(r.stats));
+    const CriticalEdges ce = critical_edges(r.barcode, n, [&](u32 a, u32 b) { return full[size_t(a) * n + b]; });
+    const py::tuple e = edges_to_python(ce);
+    return py::make_tuple(to_python(r.barcode), e[0], e[1], stats_dict(r.stats));
 }
 
 // BatchRunner: points as one packed float32 array (clouds concatenated) or a
 // list of (n_i, d_i) arrays.
-py::list batch_run(BatchRunner& br, py::object clouds, bool use_graph, u32 threads) {
+// With pairs, each entry also carries the cloud's critical edges:
+// (dgms, birth_edges, death_edges, info) as persistence_pairs.
+py::list batch_run(BatchRunner& br, py::object clouds, bool use_graph, u32 threads, bool pairs) {
     std::vector<float> packed;
     const float* p = nullptr;
     if (py::isinstance<py::list>(clouds) || py::isinstance<py::tuple>(clouds)) {
@@ -214,12 +177,20 @@ py::list batch_run(BatchRunner& br, py::object clouds, bool use_graph, u32 threa
         p = packed.data();
     }
     std::vector<Result> rs;
+    std::vector<CriticalEdges> ce;
     {
         py::gil_scoped_release release;
-        rs = br.run(p, use_graph, threads);
+        rs = br.run(p, use_graph, threads, pairs ? &ce : nullptr);
     }
     py::list out;
-    for (const auto& r : rs) out.append(py::make_tuple(to_python(r.barcode), stats_dict(r.stats)));
+    for (size_t c = 0; c < rs.size(); ++c) {
+        if (pairs) {
+            const py::tuple e = edges_to_python(ce[c]);
+            out.append(py::make_tuple(to_python(rs[c].barcode), e[0], e[1], stats_dict(rs[c].stats)));
+        } else {
+            out.append(py::make_tuple(to_python(rs[c].barcode), stats_dict(rs[c].stats)));
+        }
+    }
     return out;
 }
 
@@ -238,7 +209,8 @@ PYBIND11_MODULE(_turbotops, m) {
     py::class_<BatchRunner>(m, "BatchRunner")
         .def(py::init<const std::vector<u32>&, const std::vector<u32>&, const std::string&, u32>(), py::arg("n"),
              py::arg("dim"), py::arg("metric") = "l2", py::arg("maxdim") = 1)
-        .def("run", &batch_run, py::arg("clouds"), py::arg("use_graph") = true, py::arg("threads") = 0)
+        .def("run", &batch_run, py::arg("clouds"), py::arg("use_graph") = true, py::arg("threads") = 0,
+             py::arg("pairs") = false)
         .def_property_readonly("device_ms", &BatchRunner::device_ms)
         .def_property_readonly("rows_per_wave", &BatchRunner::rows_per_wave)
         .def_property_readonly("edges_total", &BatchRunner::edges_total)

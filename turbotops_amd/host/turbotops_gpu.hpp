@@ -14,6 +14,7 @@
 
 #include <cmath>
 #include <cstdint>
+#include <functional>
 #include <stdexcept>
 #include <string>
 #include <utility>
@@ -155,6 +156,16 @@ Result persistence(const f32* points, u32 n, u32 dim, const Options& o);
 Result persistence(const CsrGraph& g, u32 maxdim, f32 threshold = INFINITY, u32 threads = 0,
                    Mode mode = Mode::Auto);
 
+// Critical edges of a diagram: per dimension and bar, the longest edge
+// (a, b) of its birth simplex and of its death simplex, read with dist(a, b)
+// (ties: the first pair in descending vertex order); -1 where there is none
+// (H0 births, essential deaths). Every finite bar value is the length of
+// its critical edge, which is what the backward of a diagram needs.
+struct CriticalEdges {
+    std::vector<std::vector<int64_t>> birth, death;   // per dimension, 2 per bar
+};
+CriticalEdges critical_edges(const Barcode& bc, u32 n, const std::function<f32(u32, u32)>& dist);
+
 // Many clouds through one batched front end (include/ttgpu.h tt_batch_*):
 // dense full Rips at each cloud's enclosing radius, maxdim 1. Create once per
 // set of shapes; run() reduces the clouds on host threads (largest first),
@@ -167,7 +178,9 @@ class BatchRunner {
     ~BatchRunner();
     BatchRunner(const BatchRunner&) = delete;
     BatchRunner& operator=(const BatchRunner&) = delete;
-    std::vector<Result> run(const f32* packed_points, bool use_graph = true, u32 threads = 0);
+    // With `edges` != nullptr, also each cloud's critical edges.
+    std::vector<Result> run(const f32* packed_points, bool use_graph = true, u32 threads = 0,
+                            std::vector<CriticalEdges>* edges = nullptr);
     float device_ms() const { return device_ms_; }
     int rows_per_wave() const { return rpw_; }
     u64 edges_total() const { return edges_total_; }
