@@ -11,11 +11,16 @@ front-end throughput in GB/s of algorithmic bytes (4 n^2 matrix bytes
 written + 4 n d point bytes read per step).
 
 Prints ONE JSON line. Besides the contract keys it carries:
-  roofline      the dominant kernel's achieved algorithmic GB/s (HIP events on
-                the stream the kernel is launched on) against the 8 TB/s HBM peak
-  cpu_baseline  the reference's own make_dense + enclosing_radius (compiled
-                from its sources into oracle/_ref, SIMD build, all allotted
-                host cores) on the same cloud
+  roofline        the dominant kernel's achieved algorithmic GB/s (HIP events on
+                  the stream the kernel is launched on) against the 8 TB/s HBM peak
+  cpu_baseline    the reference's own make_dense + enclosing_radius (compiled
+                  from its sources into oracle/_ref, SIMD build, all allotted
+                  host cores) on the same cloud
+  batched         the 512-cloud batched front end (device time, GB/s) and
+                  diagrams/sec of the batched form against the plain per-cloud
+                  path on a bounded sample of the same clouds
+  differentiable  forward + backward of a top-8 H1 persistence loss on a
+                  noisy circle (n = 2048) and the backward kernel alone
 """
 import argparse
 import json
@@ -145,6 +150,137 @@ def run_dense(args):
     return out
 
 
+def run_edges(args):
+    """Dimension 0 of the coboundary reducer on the dense complex of the
+    single cloud (tt_dense_edges_dev over the matrix in HBM): count, compact,
+    radix sort, block minima and the apparent scan; edges/sec."""
+    import torch
+    from turbotops_amd import dense_edges_dev, dist_matrix_dev
+    x, metric = cloud(args.workload.replace("-edges", ""))
+    n, d = x.shape
+    xt = torch.from_numpy(x).cuda()
+    radius = torch.empty(1, dtype=torch.float32, device="cuda")
+    rowmax = torch.empty(n, dtype=torch.int32, device="cuda")
+    mat = dist_matrix_dev(xt, metric, "full", rowmax=rowmax, radius=radius).view(n, n)
+    torch.cuda.synchronize()
+    thr = float(radius.item())
+    for _ in range(args.warmup):
+        m = dense_edges_dev(mat, thr, count_only=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        m = dense_edges_dev(mat, thr, count_only=True)
+    torch.cuda.synchronize()
+    per = (time.perf_counter() - t0) / args.steps
+    return {"metric": "dense dimension-0 edges + apparent scan, edges/sec", "value": round(m / per, 1),
+            "unit": "edges/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+            "ms_per_step": round(per * 1e3, 3), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+            "dtype": "f32", "data": "synthetic",
+            "config": {"workload": f"{args.workload}: dimension-0 edges of the n={n} d={d} {metric} matrix at its "
+                                   f"enclosing radius {thr:.7g}", "edges": m}}
+
+
+def c4_clouds():
+    """The 512-cloud batched set: n_i ~ U{200..4000}, d_i in {2, 3, 8}, noisy
+    unions of 1-4 circles/spheres (sigma 0.02), seed 6."""
+    from turbotops_amd.synthetic import uneven_clouds
+    return uneven_clouds(count=512, nmin=200, nmax=4000, dims=(2, 3, 8), seed=6)
+
+
+def run_batched(args, metric="l2"):
+    """The batched form on the 512-cloud set.
+
+    front end: all 512 clouds through the recorded launch sequence (distances,
+    enclosing radii, edge sort, apparent scan, copies to host memory), timed
+    with the library's HIP events on its own stream; algorithmic bytes are
+    the distance stage's sum(4 n_i^2 + 4 n_i d_i).
+    diagrams/sec: whole runs (front end + host reductions) of the batched
+    form against the plain per-cloud path (persistence() on each cloud in
+    turn) on a bounded sample of the same set: its first `sample` clouds with
+    n_i <= `cap` (the host reduction of the largest circles takes minutes).
+    """
+    from turbotops_amd import BatchPersistence, persistence
+    clouds = c4_clouds()
+    algo = float(sum(4.0 * c.shape[0] ** 2 + 4.0 * c.shape[0] * c.shape[1] for c in clouds))
+    t0 = time.perf_counter()
+    bp = BatchPersistence([c.shape for c in clouds], metric=metric)
+    setup_s = time.perf_counter() - t0
+    bp.front_end(clouds)                 # autotune + graph capture
+    runs = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        bp.front_end(clouds)
+        runs.append((time.perf_counter() - t0, bp.device_ms))
+    wall, dev_ms = min(runs, key=lambda r: r[1])
+    kept, non_apparent = bp.edges
+    out = {"workload": f"512 clouds, n_i ~ U{{200..4000}}, d_i in {{2,3,8}}, circles/spheres sigma 0.02, seed 6, "
+                       f"{metric}, maxdim 1",
+           "front_end_device_ms": round(dev_ms, 3), "front_end_wall_ms": round(wall * 1e3, 3),
+           "distance_stage_bytes": algo, "front_end_gbs": round(algo / (dev_ms * 1e-3) / 1e9, 2),
+           "edges_kept": int(kept), "edges_not_apparent": int(non_apparent),
+           "rows_per_wave": bp.rows_per_wave, "setup_s": round(setup_s, 3)}
+    del bp
+    cap, count = args.c4_cap, args.c4_sample
+    if count <= 0:
+        return out
+    sample = [c for c in clouds if c.shape[0] <= cap][:count]
+    bs = BatchPersistence([c.shape for c in sample], metric=metric)
+    bs(sample)                           # autotune + graph capture
+    t0 = time.perf_counter()
+    res = bs(sample)
+    t_batch = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    plain = [persistence(c, maxdim=1, metric=metric)[0] for c in sample]
+    t_plain = time.perf_counter() - t0
+    same = all(all(np.array_equal(np.asarray(a), np.asarray(b)) for a, b in zip(r[0], p))
+               for r, p in zip(res, plain))
+    out["diagrams_per_sec"] = {
+        "batched": round(len(sample) / t_batch, 3), "plain": round(len(sample) / t_plain, 3),
+        "ratio": round(t_plain / t_batch, 2), "identical_diagrams": bool(same),
+        "sample": f"first {len(sample)} clouds with n_i <= {cap} of the 512 (sum n_i = "
+                  f"{sum(c.shape[0] for c in sample)}); whole runs: GPU front end + host reductions "
+                  f"(batched: all host threads; plain: persistence() per cloud)"}
+    return out
+
+
+def run_differentiable(args):
+    """Forward + backward of -sum(death - birth) over the top-8 H1 bars on a
+    noisy circle (radius 1, sigma 0.05, R^3, seed 7), L2; and the backward
+    kernel alone on a large edge set."""
+    import torch
+    from turbotops_amd import diagram, diagram_backward, top_k_persistence_loss
+    from turbotops_amd.synthetic import noisy_circle
+    n = args.c5_n
+    x = torch.tensor(noisy_circle(n=n, seed=7), device="cuda", requires_grad=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    dg = diagram(x, maxdim=1)
+    loss = top_k_persistence_loss(dg[1], 8)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    loss.backward()
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    # the kernel alone: 2e6 critical edges over 2e5 points in R^3
+    g = torch.Generator(device="cuda").manual_seed(0)
+    pts = torch.randn(200000, 3, device="cuda", generator=g)
+    e = torch.randint(0, 200000, (2000000, 2), device="cuda", dtype=torch.int32, generator=g)
+    gr = torch.randn(2000000, device="cuda", generator=g)
+    for _ in range(3):
+        diagram_backward(pts, e, gr)
+    torch.cuda.synchronize()
+    t3 = time.perf_counter()
+    for _ in range(10):
+        diagram_backward(pts, e, gr)
+    torch.cuda.synchronize()
+    kern = (time.perf_counter() - t3) / 10
+    return {"workload": f"noisy circle n={n} in R^3 (radius 1, sigma 0.05, seed 7), L2, loss -sum(death - birth) "
+                        f"over the top-8 H1 bars",
+            "forward_s": round(t1 - t0, 3), "backward_ms": round((t2 - t1) * 1e3, 3),
+            "loss": round(float(loss.detach()), 6), "h1_bars": int(dg[1].shape[0]),
+            "backward_call_ms_2e6_edges": round(kern * 1e3, 3)}
+
+
 def copy_probe(args):
     """A plain device copy of known size, for scaling the rocprofv3 FETCH_SIZE
     / WRITE_SIZE counters (1 GiB read + 1 GiB written per launch)."""
@@ -162,16 +298,31 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="c2a", help="c2a | c2b-l1 | c2b-linf | c2b-lp")
+    ap.add_argument("--workload", default="c2a",
+                    help="c2a | c2b-l1 | c2b-linf | c2b-lp (distance matrix); c2a-edges (dimension-0 edges)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--copy-probe", action="store_true", help="run only the copy kernel (PMC scaling)")
+    ap.add_argument("--no-batched", action="store_true", help="skip the 512-cloud batched leg")
+    ap.add_argument("--no-diff", action="store_true", help="skip the differentiable leg")
+    ap.add_argument("--c4-cap", type=int, default=1200, help="largest n_i in the diagrams/sec sample")
+    ap.add_argument("--c4-sample", type=int, default=48, help="clouds in the diagrams/sec sample")
+    ap.add_argument("--c5-n", type=int, default=2048, help="points of the differentiable circle")
     args = ap.parse_args()
     if args.gpus != 1:
         raise SystemExit("bench.py measures one GPU: this build runs the front end on a single MI355X "
                          "(--gpus 1); multi-GPU sharding is design-only")
     if args.copy_probe:
         return copy_probe(args)
+    if args.workload.endswith("-edges"):
+        print(json.dumps(run_edges(args)), flush=True)
+        return
     out = run_dense(args)
+    if not args.no_batched:
+        print("bench: batched leg", file=sys.stderr, flush=True)
+        out["batched"] = run_batched(args)
+    if not args.no_diff:
+        print("bench: differentiable leg", file=sys.stderr, flush=True)
+        out["differentiable"] = run_differentiable(args)
     print(json.dumps(out), flush=True)
 
 

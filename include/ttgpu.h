@@ -154,6 +154,40 @@ tt_status tt_dense_filtration(const float* pts, uint32_t n, uint32_t dim, int ki
                               uint64_t** idx, float** d, uint8_t** apparent,
                               float* threshold_out);
 
+/* Reducer::dim0's kept edges, sort and apparent test on a sparse complex:
+ * the flag complex of a weighted graph in CSR form (rows sorted by neighbour
+ * id, as CsrGraph::from_rows builds them, core.hpp:113-134), as
+ * persistence(const CsrGraph&, ...) (turbotops.hpp:179-209) and the kNN /
+ * radius-graph branches of persistence() (turbotops.hpp:255-303) feed the
+ * reducer (rips.hpp:685-849, SparseComplex::cofacets :428-471). Every edge
+ * v > u with w <= threshold as (idx = v(v-1)/2 + u, w) in `precedes` order
+ * (diameter ascending, index descending), and apparent[e] = 1 when the edge
+ * is the zero-pivot facet of its zero-pivot cofacet. The library mallocs
+ * *idx, *d and (if apparent != NULL) *apparent; release them with tt_free. */
+tt_status tt_sparse_edges(uint32_t n, const uint64_t* row_ptr, const uint32_t* col, const float* w,
+                          float threshold, uint64_t* m, uint64_t** idx, float** d, uint8_t** apparent);
+
+/* ---- edges of the boundary (RedZeD) engine -------------------------------- */
+
+/* persistence()'s boundary branch front end (turbotops.hpp:243-246:
+ * make_dense, enclosing_radius as threshold when threshold = +inf) and
+ * RedZeD's build_edges over the dense complex (redzed.hpp:544-613,
+ * DenseGeom::bucket_row :677-705): every edge i < j with d(i, j) <= threshold
+ * in (diameter ascending, combinatorial index ascending) order, laid out as
+ * EdgeList holds it (redzed.hpp:436-442): verts[2r] = i, verts[2r+1] = j,
+ * diam[r]. mat (host, may be NULL) receives the matrix in `layout`. The
+ * library mallocs *verts and *diam; release them with tt_free. */
+tt_status tt_build_edges(const float* pts, uint32_t n, uint32_t dim, int kind, float p,
+                         float threshold, int layout, float* mat, uint64_t* m, int32_t** verts,
+                         float** diam, float* threshold_out);
+
+/* build_edges over a graph (redzed.hpp:544-613 with SparseGeom, :831-979;
+ * redzed::compute_graph :1452-1461): edges u < v with w <= threshold of a CSR
+ * graph (rows sorted by neighbour id), in the same order and layout. */
+tt_status tt_build_edges_csr(uint32_t n, const uint64_t* row_ptr, const uint32_t* col,
+                             const float* w, float threshold, uint64_t* m, int32_t** verts,
+                             float** diam);
+
 /* ---- batches of clouds ---------------------------------------------------- */
 
 /* Many clouds of uneven size and dimension through the dense front end of

@@ -194,6 +194,10 @@ class Reference:
                                       C.POINTER(C.c_double), _u64p]
         L.ref_persistence_csr.argtypes = [C.c_uint32, _u64p, _u32p, _f32p, C.c_uint32, C.c_float,
                                           C.c_int] + bars_out
+        L.ref_build_edges.argtypes = [_f32p, C.c_uint32, C.c_uint32, C.c_char_p, C.c_float, _u64p,
+                                      C.POINTER(C.POINTER(C.c_int32)), C.POINTER(_f32p), _f32p]
+        L.ref_build_edges_csr.argtypes = [C.c_uint32, _u64p, _u32p, _f32p, C.c_float, _u64p,
+                                          C.POINTER(C.POINTER(C.c_int32)), C.POINTER(_f32p)]
         L.ref_time_dense.argtypes = [_f32p, C.c_uint32, C.c_uint32, C.c_char_p, C.c_uint32,
                                      C.POINTER(C.c_double), C.POINTER(C.c_double), _f32p]
         L.ref_time_graph.argtypes = [_f32p, C.c_uint32, C.c_uint32, C.c_char_p, C.c_int, C.c_float,
@@ -283,6 +287,30 @@ class Reference:
         k = m.value
         return (self._take(idx, k, np.uint64), self._take(dd, k, np.float32), self._take(ap, k, np.uint8),
                 thr.value)
+
+    def build_edges(self, x, metric="l2", threshold=float("inf")):
+        """redzed::detail::build_edges on the dense complex: (verts (m, 2),
+        diam, threshold)."""
+        x = _f32(x)
+        n, d = x.shape
+        m = C.c_uint64()
+        pv, pd = C.POINTER(C.c_int32)(), _f32p()
+        thr = C.c_float()
+        self._check(self.lib.ref_build_edges(_ptr(x, _f32p), n, d, metric.encode(), threshold, C.byref(m),
+                                             C.byref(pv), C.byref(pd), C.byref(thr)))
+        k = m.value
+        return self._take(pv, 2 * k, np.int32).reshape(k, 2), self._take(pd, k, np.float32), thr.value
+
+    def build_edges_csr(self, n, row_ptr, col, w, threshold=float("inf")):
+        rp = np.ascontiguousarray(row_ptr, np.uint64)
+        col = np.ascontiguousarray(col, np.uint32)
+        w = np.ascontiguousarray(w, np.float32)
+        m = C.c_uint64()
+        pv, pd = C.POINTER(C.c_int32)(), _f32p()
+        self._check(self.lib.ref_build_edges_csr(n, _ptr(rp, _u64p), _ptr(col, _u32p), _ptr(w, _f32p), threshold,
+                                                 C.byref(m), C.byref(pv), C.byref(pd)))
+        k = m.value
+        return self._take(pv, 2 * k, np.int32).reshape(k, 2), self._take(pd, k, np.float32)
 
     def persistence(self, x, maxdim=1, threshold=float("inf"), metric="l2", graph="full", k=15,
                     mode="auto", budget=0, threads=0):

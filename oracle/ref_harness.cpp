@@ -187,6 +187,50 @@ int ref_dense_edges(const float* pts, uint32_t n, uint32_t dim, const char* metr
     });
 }
 
+// redzed::detail::build_edges (redzed.hpp:544-613) over the dense complex of
+// the points at `threshold` (+inf: the enclosing radius, as
+// turbotops.hpp:245 sets it): EdgeList's verts (2 per edge) and diam.
+int ref_build_edges(const float* pts, uint32_t n, uint32_t dim, const char* metric, float threshold,
+                    uint64_t* m, int32_t** verts, float** diam, float* thr_out) {
+    return guard([&] {
+        const PointCloud c{pts, n, dim, Metric::parse(metric)};
+        rips::DenseComplex cx = rips::make_dense(n, c, ~u64(0));
+        cx.threshold = std::isinf(threshold) ? cx.enclosing_radius() : threshold;
+        *thr_out = cx.threshold;
+        redzed::detail::DenseGeom geo(cx);
+        redzed::detail::EdgeList el;
+        const redzed::detail::BinomialTable bt(int(n), 3);
+        redzed::detail::build_edges(geo, cx.threshold, bt, el);
+        *m = el.count;
+        el.verts.resize(2 * el.count);
+        el.diam.resize(el.count);
+        *verts = dup(el.verts);
+        *diam = dup(el.diam);
+    });
+}
+
+// build_edges over a CSR graph (SparseGeom, redzed.hpp:831-979).
+int ref_build_edges_csr(uint32_t n, const uint64_t* row_ptr, const uint32_t* col, const float* w,
+                        float threshold, uint64_t* m, int32_t** verts, float** diam) {
+    return guard([&] {
+        CsrGraph g;
+        g.n = n;
+        g.row_ptr.assign(row_ptr, row_ptr + n + 1);
+        g.col_idx.assign(col, col + row_ptr[n]);
+        g.weights.assign(w, w + row_ptr[n]);
+        g.m = row_ptr[n] / 2;
+        redzed::detail::SparseGeom<uint32_t> geo(g);
+        redzed::detail::EdgeList el;
+        const redzed::detail::BinomialTable bt(int(n), 3);
+        redzed::detail::build_edges(geo, threshold, bt, el);
+        *m = el.count;
+        el.verts.resize(2 * el.count);
+        el.diam.resize(el.count);
+        *verts = dup(el.verts);
+        *diam = dup(el.diam);
+    });
+}
+
 // tops::persistence on points (turbotops.hpp:211-303). graph: 0 full, 1 knn;
 // mode: 0 auto, 1 boundary, 2 coboundary.
 int ref_persistence(const float* pts, uint32_t n, uint32_t dim, const char* metric,

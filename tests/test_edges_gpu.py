@@ -95,3 +95,36 @@ def test_device_entry_two_phase(gpu, restatement):
     with pytest.raises(ValueError, match="capacity"):
         lib.call("tt_dense_edges_dev", C.c_void_p(mat.data_ptr()), 300, C.c_float(1.0), 1, 1, C.byref(m),
                  C.c_void_p(idx.data_ptr()), None, None, None)
+
+
+@pytest.mark.parametrize("metric,n,thr", [("linf", 4200, 1.0), ("l2", 4161, 1.5), ("l1", 130, np.inf),
+                                          ("l2", 700, np.inf)])
+def test_block_filtered_apparent_all_edges(gpu, restatement, metric, n, thr):
+    """The apparent scan reads only the 64-column blocks whose minima allow a
+    cofacet; every flag must equal the plain walk of the oracle. Tie-heavy
+    integer grids, n with a partial last block and n > 64 * 64 (more than one
+    pass over the block minima)."""
+    rng = np.random.default_rng(n)
+    x = rng.integers(0, 9, size=(n, 3)).astype(np.float32)
+    print(f"case block-filtered apparent metric={metric} n={n} thr={thr}", flush=True)
+    gi, gd, gf, thr = _edges(x, metric, thr)
+    mat = restatement.make_dense(x, metric, True)
+    idx, dd = restatement.dense_edges(mat, thr)
+    assert np.array_equal(gi, idx) and np.array_equal(bits(gd), bits(dd))
+    want = restatement.apparent_edges(mat, idx, dd)
+    assert np.array_equal(gf, want)
+    assert 0 < gf.sum() < gf.size
+
+
+def test_dense_edges_dev_wrapper(gpu, restatement):
+    import torch
+    from turbotops_amd import dense_edges_dev, dist_matrix_dev
+    x = np.random.default_rng(8).normal(size=(700, 3)).astype(np.float32)
+    mat = dist_matrix_dev(torch.from_numpy(x).cuda(), "l2", "full").view(700, 700)
+    m, idx, d, ap = dense_edges_dev(mat, 1.5)
+    assert dense_edges_dev(mat, 1.5, count_only=True) == m
+    full = restatement.make_dense(x, "l2", True)
+    ri, rd = restatement.dense_edges(full, 1.5)
+    assert np.array_equal(idx.cpu().numpy().astype(np.uint64), ri)
+    assert np.array_equal(bits(d.cpu().numpy()), bits(rd))
+    assert np.array_equal(ap.cpu().numpy(), restatement.apparent_edges(full, ri, rd))

@@ -10,7 +10,10 @@ from . import capi  # noqa: F401
 from .api import BatchPersistence, persistence, persistence_batch, persistence_pairs  # noqa: F401
 from .frontend import (  # noqa: F401
     CsrGraph,
+    build_edges,
+    build_edges_csr,
     dense_edges,
+    dense_edges_dev,
     dense_layout,
     dist_matrix_dev,
     knn_graph,
@@ -21,6 +24,7 @@ from .frontend import (  # noqa: F401
     mst_h0,
     parse_metric,
     radius_graph,
+    sparse_edges,
 )
 from .grad import DiagramBatch, diagram, diagram_backward, top_k_persistence_loss  # noqa: F401
 

@@ -85,9 +85,22 @@ class BatchPersistence:
             raise ValueError("clouds do not match the batch shapes")
         return self._r.run(clouds, bool(use_graph), int(threads), bool(pairs))
 
+    def front_end(self, clouds, use_graph=True):
+        """The batched device front end alone (no host reduction); the
+        results land in host memory before it returns. See device_ms."""
+        clouds = [np.ascontiguousarray(c, dtype=np.float32) for c in clouds]
+        if [c.shape for c in clouds] != self.shapes:
+            raise ValueError("clouds do not match the batch shapes")
+        self._r.front(clouds, bool(use_graph))
+
     @property
     def device_ms(self):
         return self._r.device_ms
+
+    @property
+    def edges(self):
+        """(kept edges, kept edges not in an apparent pair) of the last run."""
+        return self._r.edges_total, self._r.edges_kept
 
     @property
     def rows_per_wave(self):

@@ -20,11 +20,16 @@
 //   batch_radius   one workgroup per cloud: min of the row maxima
 //   batch_count    one wave per global row: kept edges j < i
 //   scan           global row offsets; cloud segments follow
-//   batch_compact  keys (cloud, diameter bits, M_c - 1 - idx): one sort orders
-//                  every cloud by `precedes` and keeps clouds contiguous
-//   counted radix sort over the device-side count
-//   batch_apparent one wave64 per edge (the apparent test of edges.hip)
-//   compaction of the non-apparent edges
+//   batch_compact  keys (cloud, diameter bits, M_c - 1 - idx) in row-major
+//                  order
+//   batch_block_min  per row of every cloud, the minima of its 64-column blocks
+//   batch_apparent 64 edges per wave (the two-phase apparent test of
+//                  apparent.hpp), on the unsorted edges: the test does not
+//                  depend on the order
+//   batch_select   compaction of the non-apparent keys
+//   counted radix sort of those keys over the device-side count: one sort
+//                  orders every cloud by `precedes` and keeps clouds contiguous
+//   batch_unpack   (idx, d) per non-apparent edge
 // Host copies then run per cloud, each followed by an event, so the host
 // reducer starts on a cloud as soon as its data has landed.
 #include <algorithm>
@@ -35,6 +40,7 @@
 #include <mutex>
 #include <vector>
 
+#include "apparent.hpp"
 #include "frontend.hpp"
 #include "tiles.hpp"
 
@@ -51,6 +57,7 @@ struct BatchShape {
     const u64* row_base;    // nclouds + 1
     const u64* tile_base;   // nclouds + 1
     const u64* pair_base;   // nclouds + 1 (key capacity offsets)
+    const u64* bmin_off;    // nclouds + 1 (row block minima offsets)
     u64 total_tiles, total_rows;
 };
 
@@ -169,7 +176,23 @@ __device__ __forceinline__ void decode_edge_b(u64 idx, u32& i, u32& j) {
 
 // Apparent test per sorted edge (one wave64 each) and the keep flag of the
 // non-apparent ones.
+// Block minima of every row of every cloud's matrix, one wave per row.
+__global__ __launch_bounds__(256) void batch_block_min(BatchShape b, const float* __restrict__ mats,
+                                                       float* __restrict__ bmins) {
+    const u32 lane = threadIdx.x & 63;
+    const u64 gw = (blockIdx.x * 256ull + threadIdx.x) >> 6;
+    const u64 nw = (u64(gridDim.x) * 256) >> 6;
+    for (u64 r = gw; r < b.total_rows; r += nw) {
+        const u32 c = find_segment(b.row_base, b.nclouds, r);
+        const u32 n = b.n[c];
+        const u32 i = u32(r - b.row_base[c]);
+        const u32 nb = (n + BLK - 1) / BLK;
+        block_min_row(mats + b.mat_off[c] + u64(i) * n, n, nb, bmins + b.bmin_off[c] + u64(i) * nb, lane);
+    }
+}
+
 __global__ __launch_bounds__(256) void batch_apparent(BatchShape b, const float* __restrict__ mats,
+                                                      const float* __restrict__ bmins,
                                                       const u64* __restrict__ keys,
                                                       const u32* __restrict__ total, int ib,
                                                       u32* __restrict__ keep) {
@@ -178,68 +201,57 @@ __global__ __launch_bounds__(256) void batch_apparent(BatchShape b, const float*
     const u64 nw = (u64(gridDim.x) * 256) >> 6;
     const u64 m = *total;
     const u64 imask = (u64(1) << ib) - 1;
-    for (u64 e = gw; e < m; e += nw) {
-        const u64 key = keys[e];
-        const u32 c = u32(key >> (31 + ib));
-        const float d = __uint_as_float(u32(key >> ib) & 0x7fffffffu);
-        const u32 n = b.n[c];
-        const u64 M = u64(n) * (n - 1) / 2;
-        u32 i, j;
-        decode_edge_b(M - 1 - (key & imask), i, j);
-        const float* mat = mats + b.mat_off[c];
-        const float* ri = mat + u64(i) * n;
-        const float* rj = mat + u64(j) * n;
-        int wf = -1;
-        float diw = 0.0f, djw = 0.0f;
-        for (int hi = int(n); hi > 0; hi -= 64) {
-            const int w = hi - 1 - int(lane);
-            float a = INFINITY, bb = INFINITY;
-            if (w >= 0) {
-                a = ri[w];
-                bb = rj[w];
-            }
-            const bool ok = w >= 0 && u32(w) != i && u32(w) != j && fmax_ref(a, bb) <= d;
-            const u64 bal = __ballot(ok);
-            if (bal) {
-                const int src = __builtin_ctzll(bal);
-                wf = hi - 1 - src;
-                diw = __shfl(a, src);
-                djw = __shfl(bb, src);
-                break;
-            }
+    for (u64 e0 = gw * 64; e0 < m; e0 += nw * 64) {
+        const u64 e = e0 + lane;
+        const bool valid = e < m;
+        u32 i = 1, j = 0, n = 2, nb = 1;
+        float d = 0.0f;
+        const float* mat = mats;
+        const float* bm = bmins;
+        if (valid) {
+            const u64 key = keys[e];
+            const u32 c = u32(key >> (31 + ib));
+            d = __uint_as_float(u32(key >> ib) & 0x7fffffffu);
+            n = b.n[c];
+            nb = (n + BLK - 1) / BLK;
+            const u64 M = u64(n) * (n - 1) / 2;
+            decode_edge_b(M - 1 - (key & imask), i, j);
+            mat = mats + b.mat_off[c];
+            bm = bmins + b.bmin_off[c];
         }
-        if (lane == 0) {
-            bool ap = false;
-            if (wf >= 0) {
-                const u32 w = u32(wf);
-                if (w > i) ap = true;
-                else if (w > j) ap = djw != d;
-                else ap = (djw != d) && (diw != d);
-            }
-            keep[e] = ap ? 0u : 1u;
-        }
+        const bool ap = edges_apparent_lanes<TOPK>(mat, bm, n, nb, valid, i, j, d, lane);
+        if (valid) keep[e] = ap ? 0u : 1u;
     }
 }
 
-__global__ void batch_emit(const u64* __restrict__ keys, const u32* __restrict__ keep,
-                           const u32* __restrict__ pos, const u32* __restrict__ total, int ib,
-                           const u32* __restrict__ n_of, u64* __restrict__ out_idx,
-                           float* __restrict__ out_d) {
+// The keys of the non-apparent edges, in compaction order.
+__global__ void batch_select(const u64* __restrict__ keys, const u32* __restrict__ keep,
+                             const u32* __restrict__ pos, const u32* __restrict__ total,
+                             u64* __restrict__ out) {
     const u64 m = *total;
+    for (u64 e = blockIdx.x * 256ull + threadIdx.x; e < m; e += u64(gridDim.x) * 256)
+        if (keep[e]) out[pos[e]] = keys[e];
+}
+
+// Sorted keys of the non-apparent edges -> (idx, d).
+__global__ void batch_unpack(const u64* __restrict__ keys, const u32* __restrict__ kept, int ib,
+                             const u32* __restrict__ n_of, u64* __restrict__ out_idx,
+                             float* __restrict__ out_d) {
+    const u64 m = *kept;
     const u64 imask = (u64(1) << ib) - 1;
     for (u64 e = blockIdx.x * 256ull + threadIdx.x; e < m; e += u64(gridDim.x) * 256) {
-        if (!keep[e]) continue;
         const u64 key = keys[e];
         const u32 c = u32(key >> (31 + ib));
         const u64 n = n_of[c];
         const u64 M = n * (n - 1) / 2;
-        out_idx[pos[e]] = M - 1 - (key & imask);
-        out_d[pos[e]] = __uint_as_float(u32(key >> ib) & 0x7fffffffu);
+        out_idx[e] = M - 1 - (key & imask);
+        out_d[e] = __uint_as_float(u32(key >> ib) & 0x7fffffffu);
     }
 }
 
-// per cloud: start of its kept (non-apparent) edges = number of kept edges
-// before its first sorted edge
+// per cloud: start of its non-apparent edges in the sorted list = number of
+// non-apparent edges before its first compacted edge (compaction and sort are
+// both cloud-major)
 __global__ void batch_kept_segments(const u32* __restrict__ seg, const u32* __restrict__ pos,
                                     const u32* __restrict__ keep, const u32* __restrict__ total,
                                     u32 nclouds, u32* __restrict__ kseg) {
@@ -287,17 +299,17 @@ struct BatchState {
     int rpw = 16, ib = 1, cb = 0;
     bool tuned = false;
     std::vector<u32> n, dim;
-    std::vector<u64> pt_off, mat_off, row_base, tile_base, pair_base;
+    std::vector<u64> pt_off, mat_off, row_base, tile_base, pair_base, bmin_off;
     u64 total_pts = 0, total_mat = 0, total_rows = 0, total_tiles = 0, capacity = 0, h_cap = 0;
     u32 grid_dist = 0;
     // device
     Workspace ws;
-    float *d_pts = nullptr, *d_mats = nullptr, *d_thr = nullptr, *d_out_d = nullptr;
+    float *d_pts = nullptr, *d_mats = nullptr, *d_thr = nullptr, *d_out_d = nullptr, *d_bmin = nullptr;
     u32 *d_rowmax = nullptr, *d_cnt = nullptr, *d_total = nullptr, *d_seg = nullptr, *d_keep = nullptr,
         *d_pos = nullptr, *d_kept = nullptr, *d_kseg = nullptr, *d_hist = nullptr, *d_stmp = nullptr,
         *d_n = nullptr, *d_dim = nullptr;
     u64 *d_keys = nullptr, *d_alt = nullptr, *d_out_idx = nullptr, *d_pt_off = nullptr, *d_mat_off = nullptr,
-        *d_row_base = nullptr, *d_tile_base = nullptr, *d_pair_base = nullptr;
+        *d_row_base = nullptr, *d_tile_base = nullptr, *d_pair_base = nullptr, *d_bmin_off = nullptr;
     BatchShape shape{};
     // host (pinned)
     float* h_pts = nullptr;
@@ -339,13 +351,18 @@ struct BatchState {
         batch_segments<<<1, 256, 0, s>>>(shape, d_cnt, d_seg);
         batch_compact<<<gw, 256, 0, s>>>(shape, d_mats, d_thr, d_cnt, ib, d_keys);
         TTG_LAUNCH_CHECK();
-        radix_sort_u64_counted(d_keys, capacity, d_total, 0, cb + 31 + ib, d_alt, d_hist, d_stmp, s);
+        // the apparent test does not depend on the order, so it runs on the
+        // compacted (row-major) edges and only the rest are sorted
         const u32 ge = u32(std::min<u64>(std::max<u64>(capacity / 4, 1), 65536));
         fill_device_u32(d_keep, capacity, 0u, s);   // slots past the count stay 0
-        batch_apparent<<<ge, 256, 0, s>>>(shape, d_mats, d_keys, d_total, ib, d_keep);
+        batch_block_min<<<gw, 256, 0, s>>>(shape, d_mats, d_bmin);
+        batch_apparent<<<ge, 256, 0, s>>>(shape, d_mats, d_bmin, d_keys, d_total, ib, d_keep);
         exclusive_scan_u32_buf(d_keep, d_pos, capacity, d_kept, d_stmp, s);
-        batch_emit<<<ge, 256, 0, s>>>(d_keys, d_keep, d_pos, d_total, ib, d_n, d_out_idx, d_out_d);
+        batch_select<<<ge, 256, 0, s>>>(d_keys, d_keep, d_pos, d_total, d_alt);
         batch_kept_segments<<<1, 256, 0, s>>>(d_seg, d_pos, d_keep, d_total, nclouds, d_kseg);
+        TTG_LAUNCH_CHECK();
+        radix_sort_u64_counted(d_alt, capacity, d_kept, 0, cb + 31 + ib, d_keys, d_hist, d_stmp, s);
+        batch_unpack<<<ge, 256, 0, s>>>(d_alt, d_kept, ib, d_n, d_out_idx, d_out_d);
         TTG_LAUNCH_CHECK();
     }
 };
@@ -380,6 +397,7 @@ void batch_setup(BatchState& b, u32 nclouds, const uint32_t* n, const uint32_t* 
     b.row_base.assign(nclouds + 1, 0);
     b.tile_base.assign(nclouds + 1, 0);
     b.pair_base.assign(nclouds + 1, 0);
+    b.bmin_off.assign(nclouds + 1, 0);
     u64 maxM = 1;
     for (u32 c = 0; c < nclouds; ++c) {
         if (b.n[c] && b.dim[c] == 0) fail(TT_EINVAL, "tt_batch_create: dim must be >= 1");
@@ -392,6 +410,7 @@ void batch_setup(BatchState& b, u32 nclouds, const uint32_t* n, const uint32_t* 
         b.row_base[c + 1] = b.row_base[c] + b.n[c];
         b.tile_base[c + 1] = b.tile_base[c] + nb * (nb + 1) / 2;
         b.pair_base[c + 1] = b.pair_base[c] + M;
+        b.bmin_off[c + 1] = b.bmin_off[c] + u64(b.n[c]) * ((u64(b.n[c]) + BLK - 1) / BLK);
     }
     b.total_pts = b.pt_off[nclouds];
     b.total_mat = b.mat_off[nclouds];
@@ -431,6 +450,8 @@ void batch_setup(BatchState& b, u32 nclouds, const uint32_t* n, const uint32_t* 
     b.d_row_base = ws.get<u64>(nclouds + 1);
     b.d_tile_base = ws.get<u64>(nclouds + 1);
     b.d_pair_base = ws.get<u64>(nclouds + 1);
+    b.d_bmin_off = ws.get<u64>(nclouds + 1);
+    b.d_bmin = ws.get<float>(std::max<u64>(b.bmin_off[nclouds], 1));
     TTG_HIP(hipStreamCreateWithFlags(&b.stream, hipStreamNonBlocking));
     hipStream_t s = b.stream;
     TTG_HIP(hipMemcpyAsync(b.d_n, b.n.data(), nclouds * 4, hipMemcpyHostToDevice, s));
@@ -440,9 +461,10 @@ void batch_setup(BatchState& b, u32 nclouds, const uint32_t* n, const uint32_t* 
     TTG_HIP(hipMemcpyAsync(b.d_row_base, b.row_base.data(), (nclouds + 1) * 8, hipMemcpyHostToDevice, s));
     TTG_HIP(hipMemcpyAsync(b.d_tile_base, b.tile_base.data(), (nclouds + 1) * 8, hipMemcpyHostToDevice, s));
     TTG_HIP(hipMemcpyAsync(b.d_pair_base, b.pair_base.data(), (nclouds + 1) * 8, hipMemcpyHostToDevice, s));
+    TTG_HIP(hipMemcpyAsync(b.d_bmin_off, b.bmin_off.data(), (nclouds + 1) * 8, hipMemcpyHostToDevice, s));
     TTG_HIP(hipStreamSynchronize(s));
     b.shape = BatchShape{nclouds, b.d_n, b.d_dim, b.d_pt_off, b.d_mat_off, b.d_row_base, b.d_tile_base,
-                         b.d_pair_base, b.total_tiles, b.total_rows};
+                         b.d_pair_base, b.d_bmin_off, b.total_tiles, b.total_rows};
     TTG_HIP(hipHostMalloc(reinterpret_cast<void**>(&b.h_pts), std::max<u64>(b.total_pts, 1) * 4, hipHostMallocDefault));
     TTG_HIP(hipHostMalloc(reinterpret_cast<void**>(&b.h_mats), std::max<u64>(b.total_mat, 1) * 4, hipHostMallocDefault));
     b.ev.resize(nclouds);

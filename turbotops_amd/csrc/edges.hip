@@ -17,13 +17,20 @@
 //                  edges into 64-bit keys  ((d - dmin) << ib) | (M - 1 - idx),
 //                  M = n(n-1)/2: ascending keys are the reference's
 //                  `precedes` order (diameter ascending, index descending)
-//   radix sort     only the bits that vary
-//   apparent_scan  one wave64 per edge: w runs from n - 1 down in chunks of
-//                  64; a ballot of max(d(i,w), d(j,w)) <= d finds the first
-//                  cofacet, and the zero-pivot-facet rule on the triangle
-//                  decides whether the edge is its apparent partner.
+//   block_min      per row, the minimum of every block of 64 columns
+//   apparent_scan  on the compacted (row-major) edges, 64 edges per wave:
+//                  the lanes first walk the top TOPK candidates w together
+//                  (coalesced through the matrix's symmetry); an edge still
+//                  open is then searched by the whole wave, the block minima of
+//                  rows i and j picking the blocks that can hold a w with
+//                  max(d(i,w), d(j,w)) <= d, highest first. The zero-pivot-facet
+//                  rule on the triangle decides whether the edge is its
+//                  apparent partner (apparent.hpp). The flag is appended below
+//                  the key.
+//   radix sort     only the bits that vary (plus the flag bit)
 #include <algorithm>
 
+#include "apparent.hpp"
 #include "frontend.hpp"
 
 namespace ttg {
@@ -78,7 +85,7 @@ __global__ __launch_bounds__(EB) void edge_count(const float* __restrict__ mat, 
 
 __global__ __launch_bounds__(EB) void edge_compact(const float* __restrict__ mat, u32 n, float thr,
                                                    const u32* __restrict__ off, u32 dmin, int ib,
-                                                   u64 M, u64* __restrict__ keys) {
+                                                   u64 M, bool comb, u64* __restrict__ keys) {
     const u32 lane = threadIdx.x & 63;
     const u32 gw = (blockIdx.x * EB + threadIdx.x) >> 6;
     const u32 nw = (gridDim.x * EB) >> 6;
@@ -94,7 +101,7 @@ __global__ __launch_bounds__(EB) void edge_compact(const float* __restrict__ mat
             if (keep) {
                 const u32 r = __builtin_amdgcn_mbcnt_hi(u32(b >> 32), __builtin_amdgcn_mbcnt_lo(u32(b), 0u));
                 const u64 idx = rowbase + j;
-                keys[o + r] = (u64(__float_as_uint(d) - dmin) << ib) | (M - 1 - idx);
+                keys[o + r] = (u64(__float_as_uint(d) - dmin) << ib) | (comb ? idx : M - 1 - idx);
             }
             o += __popcll(b);
         }
@@ -102,64 +109,69 @@ __global__ __launch_bounds__(EB) void edge_compact(const float* __restrict__ mat
 }
 
 __global__ __launch_bounds__(EB) void edge_unpack(const u64* __restrict__ keys, u64 m, u32 dmin,
-                                                  int ib, u64 M, u64* __restrict__ idx,
+                                                  int ib, u64 M, bool comb, u64* __restrict__ idx,
                                                   float* __restrict__ d) {
     const u64 mask = ib >= 64 ? ~0ull : ((u64(1) << ib) - 1);
     for (u64 t = blockIdx.x * u64(EB) + threadIdx.x; t < m; t += u64(gridDim.x) * EB) {
         const u64 k = keys[t];
-        if (idx) idx[t] = M - 1 - (k & mask);
+        if (idx) idx[t] = comb ? (k & mask) : M - 1 - (k & mask);
         if (d) d[t] = __uint_as_float(u32(k >> ib) + dmin);
     }
 }
 
-// Zero-apparent test of sorted edges, one wave64 per edge.
+// EdgeList vertex pairs (smaller id first) of sorted edge indices.
+__global__ __launch_bounds__(EB) void edge_verts(const u64* __restrict__ idx, u64 m,
+                                                 int32_t* __restrict__ verts) {
+    for (u64 t = blockIdx.x * u64(EB) + threadIdx.x; t < m; t += u64(gridDim.x) * EB) {
+        u32 i, j;
+        decode_edge(idx[t], i, j);
+        verts[2 * t] = int32_t(j);
+        verts[2 * t + 1] = int32_t(i);
+    }
+}
+
+// Block minima of every row of the full matrix, one wave per row.
+__global__ __launch_bounds__(EB) void block_min(const float* __restrict__ mat, u32 n, u32 nb,
+                                                float* __restrict__ bmin) {
+    const u32 lane = threadIdx.x & 63;
+    const u32 gw = (blockIdx.x * EB + threadIdx.x) >> 6;
+    const u32 nw = (gridDim.x * EB) >> 6;
+    for (u32 i = gw; i < n; i += nw) block_min_row(mat + u64(i) * n, n, nb, bmin + u64(i) * nb, lane);
+}
+
+// Zero-apparent test of the compacted (row-major) edges, 64 edges per wave,
+// one per lane (apparent.hpp); the flag becomes the key's lowest bit. The
+// test does not depend on the order, and a flag bit below a unique key does
+// not change the order of the sort that follows.
 __global__ __launch_bounds__(EB) void apparent_scan(const float* __restrict__ mat, u32 n,
-                                                    const u64* __restrict__ idx,
-                                                    const float* __restrict__ dd, u64 m,
-                                                    u8* __restrict__ flags) {
+                                                    const float* __restrict__ bmin, u32 nb,
+                                                    u64* __restrict__ keys, u64 m, int ib, u64 M) {
     const u32 lane = threadIdx.x & 63;
     const u64 gw = (blockIdx.x * u64(EB) + threadIdx.x) >> 6;
     const u64 nw = (u64(gridDim.x) * EB) >> 6;
-    for (u64 e = gw; e < m; e += nw) {
-        u32 i, j;
-        decode_edge(idx[e], i, j);
-        const float d = dd[e];
-        const float* __restrict__ ri = mat + u64(i) * n;
-        const float* __restrict__ rj = mat + u64(j) * n;
-        // first cofacet in descending w: max(d(i, w), d(j, w)) <= d, w not i, j
-        // (cofacets_full with cap = diameter, rips.hpp:326-343, :593-598)
-        int wf = -1;
-        float diw = 0.0f, djw = 0.0f;
-        for (int hi = int(n); hi > 0; hi -= 64) {
-            const int w = hi - 1 - int(lane);
-            float a = INFINITY, b = INFINITY;
-            if (w >= 0) {
-                a = ri[w];
-                b = rj[w];
-            }
-            const bool c = w >= 0 && u32(w) != i && u32(w) != j && fmax_ref(a, b) <= d;
-            const u64 bal = __ballot(c);
-            if (bal) {
-                const int src = __builtin_ctzll(bal);
-                wf = hi - 1 - src;
-                diw = __shfl(a, src);
-                djw = __shfl(b, src);
-                break;
-            }
+    const u64 mask = (u64(1) << ib) - 1;
+    for (u64 e0 = gw * 64; e0 < m; e0 += nw * 64) {
+        const u64 e = e0 + lane;
+        const bool valid = e < m;
+        u64 k = 0;
+        u32 i = 1, j = 0;
+        float d = 0.0f;
+        if (valid) {
+            k = keys[e];
+            decode_edge(M - 1 - (k & mask), i, j);
+            d = mat[u64(i) * n + j];
         }
-        if (lane == 0) {
-            u8 ap = 0;
-            if (wf >= 0) {
-                // zero_pivot_facet_v on the triangle, facets in ascending index
-                // order (rips.hpp:609-636): the first facet holding an edge of
-                // length d must be (i, j) itself.
-                const u32 w = u32(wf);
-                if (w > i) ap = 1;                                  // facet (i, j) comes first
-                else if (w > j) ap = djw != d;                      // (w, j) precedes (i, j)
-                else ap = (djw != d) && (diw != d);                 // (j, w), (i, w) precede
-            }
-            flags[e] = ap;
-        }
+        const bool ap = edges_apparent_lanes<TOPK>(mat, bmin, n, nb, valid, i, j, d, lane);
+        if (valid) keys[e] = (k << 1) | (ap ? 1u : 0u);
+    }
+}
+
+// Sorted keys carrying the apparent flag in bit 0 -> flags.
+__global__ __launch_bounds__(EB) void edge_flags(u64* __restrict__ keys, u64 m, u8* __restrict__ flags) {
+    for (u64 t = blockIdx.x * u64(EB) + threadIdx.x; t < m; t += u64(gridDim.x) * EB) {
+        const u64 k = keys[t];
+        flags[t] = u8(k & 1);
+        keys[t] = k >> 1;
     }
 }
 
@@ -172,7 +184,7 @@ u32 grid_for(u64 waves) {
 // Kept edges of a full device matrix, in filtration order, and optionally
 // their apparent flags. Buffers live in ws.
 DenseEdges dense_edges(const float* d_mat, u32 n, float thr, bool want_apparent, Workspace& ws,
-                       hipStream_t s) {
+                       hipStream_t s, bool comb) {
     DenseEdges out;
     if (n < 2) return out;
     if (u64(n) * (n - 1) / 2 >= (u64(1) << 32))
@@ -195,22 +207,32 @@ DenseEdges dense_edges(const float* d_mat, u32 n, float thr, bool want_apparent,
     const u32 dmin = hm[0], dmax = hm[1];
     const int ib = std::max(1, bit_length(M - 1));
     const int db = bit_length(u64(dmax - dmin));
-    if (ib + db > 64) fail(TT_ELENGTH, "dense_edges: key does not fit 64 bits");
+    if (ib + db + 1 > 64) fail(TT_ELENGTH, "dense_edges: key does not fit 64 bits");
+    if (want_apparent && comb) fail(TT_EINVAL, "dense_edges: the apparent test runs in `precedes` order only");
     u64* keys = ws.get<u64>(out.m);
     u64* alt = ws.get<u64>(out.m);
-    edge_compact<<<grid_for(n), EB, 0, s>>>(d_mat, n, thr, cnt, dmin, ib, M, keys);
+    edge_compact<<<grid_for(n), EB, 0, s>>>(d_mat, n, thr, cnt, dmin, ib, M, comb, keys);
     TTG_LAUNCH_CHECK();
-    radix_sort_u64(keys, nullptr, out.m, 0, ib + db, alt, nullptr, ws, s);
-    out.idx = alt;   // reuse the ping-pong buffer for the unpacked indices
-    out.d = ws.get<float>(out.m);
-    edge_unpack<<<u32(std::min<u64>((out.m + EB - 1) / EB, 65536)), EB, 0, s>>>(keys, out.m, dmin, ib, M,
-                                                                               out.idx, out.d);
-    TTG_LAUNCH_CHECK();
+    const int fb = want_apparent ? 1 : 0;   // the flag bit below the key (ib + db + 1 <= 64)
     if (want_apparent) {
-        out.apparent = ws.get<u8>(out.m);
-        apparent_scan<<<grid_for(out.m), EB, 0, s>>>(d_mat, n, out.idx, out.d, out.m, out.apparent);
+        const u32 nb = (n + BLK - 1) / BLK;
+        float* bmin = ws.get<float>(u64(n) * nb);
+        block_min<<<grid_for(n), EB, 0, s>>>(d_mat, n, nb, bmin);
+        TTG_LAUNCH_CHECK();
+        apparent_scan<<<grid_for((out.m + 63) / 64), EB, 0, s>>>(d_mat, n, bmin, nb, keys, out.m, ib, M);
         TTG_LAUNCH_CHECK();
     }
+    radix_sort_u64(keys, nullptr, out.m, 0, ib + db + fb, alt, nullptr, ws, s);
+    const u32 ug = u32(std::min<u64>((out.m + EB - 1) / EB, 65536));
+    if (want_apparent) {
+        out.apparent = ws.get<u8>(out.m);
+        edge_flags<<<ug, EB, 0, s>>>(keys, out.m, out.apparent);
+        TTG_LAUNCH_CHECK();
+    }
+    out.idx = alt;   // reuse the ping-pong buffer for the unpacked indices
+    out.d = ws.get<float>(out.m);
+    edge_unpack<<<ug, EB, 0, s>>>(keys, out.m, dmin, ib, M, comb, out.idx, out.d);
+    TTG_LAUNCH_CHECK();
     return out;
 }
 
@@ -292,6 +314,16 @@ void tt_free(void* p) { std::free(p); }
 
 }  // extern "C"
 
+namespace ttg {
+
+void edge_vertices(const u64* d_idx, u64 m, int32_t* d_verts, hipStream_t s) {
+    if (!m) return;
+    edge_verts<<<u32(std::min<u64>((m + EB - 1) / EB, 65536)), EB, 0, s>>>(d_idx, m, d_verts);
+    TTG_LAUNCH_CHECK();
+}
+
+}  // namespace ttg
+
 // The whole dense front end of compute_dense / persistence's dense branch
 // (rips.hpp:391-416, :370-380, :1073, :685-849) in one device pass: the
 // matrix (copied to `mat` in the requested host layout), the enclosing radius
@@ -350,6 +382,67 @@ extern "C" tt_status tt_dense_filtration(const float* pts, uint32_t n, uint32_t 
             TTG_HIP(hipMemcpyAsync(*idx, e.idx, e.m * sizeof(u64), hipMemcpyDeviceToHost, s));
             TTG_HIP(hipMemcpyAsync(*d, e.d, e.m * sizeof(float), hipMemcpyDeviceToHost, s));
             if (apparent) TTG_HIP(hipMemcpyAsync(*apparent, e.apparent, e.m, hipMemcpyDeviceToHost, s));
+        }
+        TTG_HIP(hipStreamSynchronize(s));
+    });
+}
+
+// The dense front end of persistence()'s boundary (RedZeD) branch
+// (turbotops.hpp:243-246: make_dense, enclosing_radius as threshold) and
+// RedZeD's build_edges over it (redzed.hpp:544-613 with DenseGeom::bucket_row
+// :677-705): edges i < j with d <= threshold in (diameter ascending,
+// combinatorial index ascending) order, as EdgeList holds them (:436-442).
+extern "C" tt_status tt_build_edges(const float* pts, uint32_t n, uint32_t dim, int kind, float p,
+                                    float threshold, int layout, float* mat, uint64_t* m,
+                                    int32_t** verts, float** diam, float* threshold_out) {
+    using namespace ttg;
+    return guarded([&] {
+        check_metric(kind, p);
+        check_points(pts, u64(n) * dim, "tt_build_edges");
+        if (!m || !verts || !diam) fail(TT_EINVAL, "tt_build_edges: null output");
+        if (std::isnan(threshold) || threshold < 0.0f)
+            fail(TT_EINVAL, "persistence: threshold must be non-negative or +inf");
+        if (layout != TT_LAYOUT_FULL && layout != TT_LAYOUT_LOWER)
+            fail(TT_EINVAL, "tt_build_edges: unknown layout");
+        *m = 0;
+        *verts = nullptr;
+        *diam = nullptr;
+        hipStream_t s = nullptr;
+        Workspace ws;
+        float* dx = ws.get<float>(u64(n) * dim);
+        float* dm = ws.get<float>(u64(n) * n);
+        u32* rowmax = ws.get<u32>(n);
+        float* dr = ws.get<float>(1);
+        if (u64(n) * dim)
+            TTG_HIP(hipMemcpyAsync(dx, pts, u64(n) * dim * sizeof(float), hipMemcpyHostToDevice, s));
+        dist_matrix(dx, n, dim, kind, p, TT_LAYOUT_FULL, dm, rowmax, dr, s);
+        float thr = threshold;
+        if (std::isinf(threshold)) {
+            TTG_HIP(hipMemcpyAsync(&thr, dr, sizeof(float), hipMemcpyDeviceToHost, s));
+            TTG_HIP(hipStreamSynchronize(s));
+        }
+        if (threshold_out) *threshold_out = thr;
+        const DenseEdges e = dense_edges(dm, n, thr, false, ws, s, true);
+        int32_t* vd = ws.get<int32_t>(2 * e.m);
+        edge_vertices(e.idx, e.m, vd, s);
+        if (mat) {
+            if (layout == TT_LAYOUT_FULL) {
+                if (u64(n) * n)
+                    TTG_HIP(hipMemcpyAsync(mat, dm, u64(n) * n * sizeof(float), hipMemcpyDeviceToHost, s));
+            } else {
+                const u64 cells = u64(n) * (n ? n - 1 : 0) / 2;
+                float* low = ws.get<float>(cells);
+                dist_matrix(dx, n, dim, kind, p, TT_LAYOUT_LOWER, low, nullptr, nullptr, s);
+                if (cells) TTG_HIP(hipMemcpyAsync(mat, low, cells * sizeof(float), hipMemcpyDeviceToHost, s));
+            }
+        }
+        *m = e.m;
+        *verts = static_cast<int32_t*>(std::malloc(std::max<u64>(1, 2 * e.m) * sizeof(int32_t)));
+        *diam = static_cast<float*>(std::malloc(std::max<u64>(1, e.m) * sizeof(float)));
+        if (!*verts || !*diam) fail(TT_EDEVICE, "out of host memory");
+        if (e.m) {
+            TTG_HIP(hipMemcpyAsync(*verts, vd, 2 * e.m * sizeof(int32_t), hipMemcpyDeviceToHost, s));
+            TTG_HIP(hipMemcpyAsync(*diam, e.d, e.m * sizeof(float), hipMemcpyDeviceToHost, s));
         }
         TTG_HIP(hipStreamSynchronize(s));
     });

@@ -21,7 +21,21 @@ struct DenseEdges {
     float* d = nullptr;      // device
     u8* apparent = nullptr;  // device (null unless requested)
 };
+// comb: ties in diameter broken by ascending index (RedZeD's order)
+// instead of descending (the coboundary reducer's `precedes`).
 DenseEdges dense_edges(const float* d_mat, u32 n, float thr, bool want_apparent, Workspace& ws,
-                       hipStream_t s);
+                       hipStream_t s, bool comb = false);
+// EdgeList vertex pairs (smaller id first) of edge indices.
+void edge_vertices(const u64* d_idx, u64 m, int32_t* d_verts, hipStream_t s);
+
+// sparse.hip
+struct SparseEdges {
+    u64 m = 0;
+    u64* idx = nullptr;      // device, filtration order
+    float* d = nullptr;      // device
+    u8* apparent = nullptr;  // device (null unless requested)
+};
+SparseEdges sparse_edges(const u64* d_row_ptr, const u32* d_col, const float* d_w, u32 n, float thr,
+                         bool want_apparent, Workspace& ws, hipStream_t s, bool comb = false);
 
 }  // namespace ttg

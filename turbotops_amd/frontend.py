@@ -125,6 +125,112 @@ def dense_edges(x, metric="l2", threshold=math.inf, apparent=True):
     return idx, dd, ap, thr.value
 
 
+def dense_edges_dev(mat, threshold, apparent=True, count_only=False, stream=None):
+    """Device form of dense_edges over a full (n, n) float32 matrix already in
+    HBM (a torch tensor): (m, idx, d, apparent) as torch tensors on the same
+    device, or just m with count_only (the whole pipeline still runs)."""
+    import torch
+    n = int(mat.shape[0])
+    if mat.dtype != torch.float32 or not mat.is_contiguous() or mat.numel() != n * n:
+        raise ValueError("mat must be a contiguous float32 (n, n) tensor")
+    lib = capi.lib()
+    m = C.c_uint64()
+    st = _stream(stream)
+    lib.call("tt_dense_edges_dev", C.c_void_p(mat.data_ptr()), n, C.c_float(threshold), int(bool(apparent)), 0,
+             C.byref(m), None, None, None, st)
+    k = m.value
+    if count_only:
+        return k
+    idx = torch.empty(k, dtype=torch.int64, device=mat.device)
+    dd = torch.empty(k, dtype=torch.float32, device=mat.device)
+    ap = torch.empty(k, dtype=torch.uint8, device=mat.device) if apparent else None
+    lib.call("tt_dense_edges_dev", C.c_void_p(mat.data_ptr()), n, C.c_float(threshold), int(bool(apparent)), k,
+             C.byref(m), C.c_void_p(idx.data_ptr()), C.c_void_p(dd.data_ptr()),
+             C.c_void_p(ap.data_ptr()) if apparent else None, st)
+    return k, idx, dd, ap
+
+
+def sparse_edges(graph, threshold=math.inf, apparent=True):
+    """Reducer::dim0's kept edges, sort and apparent test on the flag complex
+    of a CSR graph (rips.hpp:685-849, SparseComplex::cofacets :428-471).
+    graph: a CsrGraph or (n, row_ptr, col, w). Returns (idx u64, d f32,
+    apparent u8 or None), edges ordered by (diameter asc, index desc)."""
+    if isinstance(graph, CsrGraph):
+        n, rp, col, w = graph.n, graph.row_ptr, graph.col_idx, graph.weights
+    else:
+        n, rp, col, w = graph
+    rp = np.ascontiguousarray(rp, np.uint64)
+    col = np.ascontiguousarray(col, np.uint32)
+    w = np.ascontiguousarray(w, np.float32)
+    if rp.shape != (int(n) + 1,):
+        raise ValueError("row_ptr must have n + 1 entries")
+    if col.shape[0] < int(rp[-1]) or w.shape[0] < int(rp[-1]):
+        raise ValueError("col and w must hold row_ptr[n] entries")
+    lib = capi.lib()
+    m = C.c_uint64()
+    pi, pd, pa = C.POINTER(C.c_uint64)(), C.POINTER(C.c_float)(), C.POINTER(C.c_uint8)()
+    lib.call("tt_sparse_edges", int(n), _addr(rp), _addr(col), _addr(w), C.c_float(threshold), C.byref(m),
+             C.byref(pi), C.byref(pd), C.byref(pa) if apparent else None)
+    k = m.value
+    idx = _take(lib, pi, k, C.c_uint64, np.uint64)
+    dd = _take(lib, pd, k, C.c_float, np.float32)
+    ap = _take(lib, pa, k, C.c_uint8, np.uint8) if apparent else None
+    return idx, dd, ap
+
+
+def build_edges(x, metric="l2", threshold=math.inf, layout="full", with_matrix=False):
+    """The boundary (RedZeD) engine's front end: make_dense + enclosing
+    radius (turbotops.hpp:243-246) and build_edges (redzed.hpp:544-613).
+    Returns (verts (m, 2) int32 with verts[:, 0] < verts[:, 1], diam f32,
+    threshold[, mat]) in (diameter asc, combinatorial index asc) order."""
+    x = _f32(x)
+    if x.ndim != 2:
+        raise ValueError("x must be a 2-d array (n, d)")
+    n, dim = x.shape
+    kind, p = _metric(metric)
+    lib = capi.lib()
+    full = layout == "full"
+    mat = None
+    if with_matrix:
+        mat = np.empty(n * n if full else n * (n - 1) // 2, np.float32)
+    m = C.c_uint64()
+    pv, pd = C.POINTER(C.c_int32)(), C.POINTER(C.c_float)()
+    thr = C.c_float()
+    lib.call("tt_build_edges", _addr(x), n, dim, kind, p, C.c_float(threshold),
+             capi.LAYOUT_FULL if full else capi.LAYOUT_LOWER, _addr(mat) if mat is not None else None,
+             C.byref(m), C.byref(pv), C.byref(pd), C.byref(thr))
+    k = m.value
+    verts = _take(lib, pv, 2 * k, C.c_int32, np.int32).reshape(k, 2)
+    diam = _take(lib, pd, k, C.c_float, np.float32)
+    if with_matrix:
+        return verts, diam, thr.value, (mat.reshape(n, n) if full else mat)
+    return verts, diam, thr.value
+
+
+def build_edges_csr(graph, threshold=math.inf):
+    """build_edges over a CSR graph (redzed.hpp:544-613 with SparseGeom):
+    (verts (m, 2) int32, diam f32) in (diameter asc, combinatorial index asc)
+    order."""
+    if isinstance(graph, CsrGraph):
+        n, rp, col, w = graph.n, graph.row_ptr, graph.col_idx, graph.weights
+    else:
+        n, rp, col, w = graph
+    rp = np.ascontiguousarray(rp, np.uint64)
+    col = np.ascontiguousarray(col, np.uint32)
+    w = np.ascontiguousarray(w, np.float32)
+    if rp.shape != (int(n) + 1,):
+        raise ValueError("row_ptr must have n + 1 entries")
+    if col.shape[0] < int(rp[-1]) or w.shape[0] < int(rp[-1]):
+        raise ValueError("col and w must hold row_ptr[n] entries")
+    lib = capi.lib()
+    m = C.c_uint64()
+    pv, pd = C.POINTER(C.c_int32)(), C.POINTER(C.c_float)()
+    lib.call("tt_build_edges_csr", int(n), _addr(rp), _addr(col), _addr(w), C.c_float(threshold), C.byref(m),
+             C.byref(pv), C.byref(pd))
+    k = m.value
+    return _take(lib, pv, 2 * k, C.c_int32, np.int32).reshape(k, 2), _take(lib, pd, k, C.c_float, np.float32)
+
+
 class CsrGraph:
     """CsrGraph (include/turbotops/core.hpp:92-149): n vertices, m undirected
     edges stored in both rows, rows sorted by neighbour id, no self loops."""

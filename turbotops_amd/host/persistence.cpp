@@ -34,23 +34,6 @@ std::vector<T> take(T* p, u64 count) {
     return v;
 }
 
-template <class F>
-void parallel_for(u64 n, F&& fn) {
-    const u32 nt = u32(std::min<u64>(num_threads(), n));
-    if (nt <= 1) {
-        for (u64 i = 0; i < n; ++i) fn(i);
-        return;
-    }
-    std::atomic<u64> next{0};
-    std::vector<std::thread> pool;
-    for (u32 t = 0; t < nt; ++t)
-        pool.emplace_back([&] {
-            for (u64 i; (i = next.fetch_add(1024)) < n;)
-                for (u64 e = i; e < std::min(n, i + 1024); ++e) fn(e);
-        });
-    for (auto& th : pool) th.join();
-}
-
 // validate_options (turbotops.hpp:170-175)
 void validate_options(f32 threshold, u32 k) {
     if (std::isnan(threshold) || threshold < 0.0f)
@@ -117,37 +100,25 @@ void reduce_all(const Cx& cx, u32 maxdim, const u64* idx, const f32* d, const u8
     st.emergent = tel.emergent;
 }
 
-// dim0 inputs of a sparse complex: kept edges (u < v, w <= threshold) in
-// filtration order and their apparent flags.
-template <class I>
-void sparse_edges(const cob::SparseView& cx, u32 maxdim, std::vector<u64>& idx, std::vector<f32>& d,
-                  std::vector<u8>& apparent) {
-    struct E {
-        u64 idx;
-        f32 d;
-    };
-    std::vector<E> edges;
-    for (u32 v = 0; v < cx.n; ++v)
-        for (u64 t = cx.row_ptr[v]; t < cx.row_ptr[v + 1]; ++t) {
-            const u32 u = cx.col[t];
-            if (u < v && cx.w[t] <= cx.threshold) edges.push_back(E{u64(v) * (v - 1) / 2 + u, cx.w[t]});
-        }
-    std::sort(edges.begin(), edges.end(),
-              [](const E& a, const E& b) { return a.d < b.d || (a.d == b.d && a.idx > b.idx); });
-    idx.resize(edges.size());
-    d.resize(edges.size());
-    for (size_t e = 0; e < edges.size(); ++e) {
-        idx[e] = edges[e].idx;
-        d[e] = edges[e].d;
+// dim0 inputs of a sparse complex, from the GPU (tt_sparse_edges): kept
+// edges (u < v, w <= threshold) in filtration order and their apparent flags.
+struct SparseEdgeList {
+    u64 m = 0;
+    u64* idx = nullptr;
+    f32* d = nullptr;
+    u8* apparent = nullptr;
+    SparseEdgeList(const CsrGraph& g, f32 threshold, bool want_apparent) {
+        check(tt_sparse_edges(g.n, g.row_ptr.data(), g.col_idx.data(), g.weights.data(), threshold, &m, &idx,
+                              &d, want_apparent ? &apparent : nullptr));
     }
-    apparent.assign(maxdim >= 1 ? edges.size() : 0, 0);
-    if (maxdim >= 1) {
-        cob::Diagram scratch;
-        cob::Telemetry tel;
-        const cob::Cohomology<I, cob::SparseView> r(cx, std::max<u32>(maxdim, 1), scratch, tel);
-        parallel_for(edges.size(), [&](u64 e) { apparent[e] = r.edge_apparent(idx[e], d[e]); });
+    ~SparseEdgeList() {
+        tt_free(idx);
+        tt_free(d);
+        tt_free(apparent);
     }
-}
+    SparseEdgeList(const SparseEdgeList&) = delete;
+    SparseEdgeList& operator=(const SparseEdgeList&) = delete;
+};
 
 }  // namespace
 
@@ -304,12 +275,8 @@ Result persistence(const CsrGraph& g, u32 maxdim, f32 threshold, u32 threads, Mo
     cx.row_ptr = g.row_ptr.data();
     cx.col = g.col_idx.data();
     cx.w = g.weights.data();
-    std::vector<u64> idx;
-    std::vector<f32> d;
-    std::vector<u8> ap;
-    if (cob::index_fits<u64>(g.n, std::max<u32>(maxdim, 1))) sparse_edges<u64>(cx, maxdim, idx, d, ap);
-    else sparse_edges<cob::u128>(cx, maxdim, idx, d, ap);
-    reduce_all(cx, maxdim, idx.data(), d.data(), ap.empty() ? nullptr : ap.data(), idx.size(), r.barcode, r.stats);
+    const SparseEdgeList e(g, threshold, maxdim >= 1);
+    reduce_all(cx, maxdim, e.idx, e.d, e.apparent, e.m, r.barcode, r.stats);
     r.barcode.ensure(maxdim);
     r.stats.method = "sparse";
     r.stats.threshold = threshold;
@@ -471,6 +438,15 @@ CriticalEdges critical_edges(const Barcode& bc, u32 n, const std::function<f32(u
         }
     }
     return ce;
+}
+
+void BatchRunner::front(const f32* pts, bool use_graph) {
+    const u64 total = total_points();
+    for (u64 i = 0; i < total; ++i)
+        if (!std::isfinite(pts[i])) throw std::invalid_argument("persistence: non-finite coordinate");
+    check(tt_batch_run(h_, pts, use_graph ? 1 : 0));
+    for (u32 c = 0; c < u32(n_.size()); ++c) check(tt_batch_wait(h_, c));
+    check(tt_batch_info(h_, &device_ms_, &rpw_, &edges_total_, &edges_kept_));
 }
 
 std::vector<Result> BatchRunner::run(const f32* pts, bool use_graph, u32 threads,

@@ -158,24 +158,32 @@ py::tuple persistence_pairs(F32Array x, u32 maxdim, float threshold, const std::
 
 // BatchRunner: points as one packed float32 array (clouds concatenated) or a
 // list of (n_i, d_i) arrays.
-// With pairs, each entry also carries the cloud's critical edges:
-// (dgms, birth_edges, death_edges, info) as persistence_pairs.
-py::list batch_run(BatchRunner& br, py::object clouds, bool use_graph, u32 threads, bool pairs) {
+std::vector<float> pack(BatchRunner& br, py::object clouds) {
     std::vector<float> packed;
-    const float* p = nullptr;
     if (py::isinstance<py::list>(clouds) || py::isinstance<py::tuple>(clouds)) {
         for (auto h : clouds) {
             F32Array a = py::cast<F32Array>(h);
             packed.insert(packed.end(), a.data(), a.data() + a.size());
         }
-        if (packed.size() != br.total_points()) throw py::value_error("clouds do not match the batch shapes");
-        p = packed.data();
     } else {
         F32Array a = py::cast<F32Array>(clouds);
-        if (u64(a.size()) != br.total_points()) throw py::value_error("clouds do not match the batch shapes");
         packed.assign(a.data(), a.data() + a.size());
-        p = packed.data();
     }
+    if (packed.size() != br.total_points()) throw py::value_error("clouds do not match the batch shapes");
+    return packed;
+}
+
+void batch_front(BatchRunner& br, py::object clouds, bool use_graph) {
+    const std::vector<float> packed = pack(br, clouds);
+    py::gil_scoped_release release;
+    br.front(packed.data(), use_graph);
+}
+
+// With pairs, each entry also carries the cloud's critical edges:
+// (dgms, birth_edges, death_edges, info) as persistence_pairs.
+py::list batch_run(BatchRunner& br, py::object clouds, bool use_graph, u32 threads, bool pairs) {
+    const std::vector<float> packed = pack(br, clouds);
+    const float* p = packed.data();
     std::vector<Result> rs;
     std::vector<CriticalEdges> ce;
     {
@@ -209,6 +217,7 @@ PYBIND11_MODULE(_turbotops, m) {
     py::class_<BatchRunner>(m, "BatchRunner")
         .def(py::init<const std::vector<u32>&, const std::vector<u32>&, const std::string&, u32>(), py::arg("n"),
              py::arg("dim"), py::arg("metric") = "l2", py::arg("maxdim") = 1)
+        .def("front", &batch_front, py::arg("clouds"), py::arg("use_graph") = true)
         .def("run", &batch_run, py::arg("clouds"), py::arg("use_graph") = true, py::arg("threads") = 0,
              py::arg("pairs") = false)
         .def_property_readonly("device_ms", &BatchRunner::device_ms)

@@ -181,6 +181,9 @@ class BatchRunner {
     // With `edges` != nullptr, also each cloud's critical edges.
     std::vector<Result> run(const f32* packed_points, bool use_graph = true, u32 threads = 0,
                             std::vector<CriticalEdges>* edges = nullptr);
+    // The device front end alone (distances, radii, edge sort, apparent
+    // scan) and the copies of its results to host memory, no reduction.
+    void front(const f32* packed_points, bool use_graph = true);
     float device_ms() const { return device_ms_; }
     int rows_per_wave() const { return rpw_; }
     u64 edges_total() const { return edges_total_; }
