@@ -46,6 +46,7 @@ py::dict stats_dict(const Stats& s) {
     d["seconds_total"] = s.seconds_total;
     d["peak_rss_mb"] = s.peak_rss_mb;
     d["seconds_front"] = s.seconds_front;
+    d["seconds_reduce"] = s.seconds_reduce;
     return d;
 }
 

@@ -142,7 +142,7 @@ struct Stats {
     f32 threshold = INFINITY;
     u64 edges = 0;
     bool wide_index = false;
-    double seconds_graph = 0, seconds_front = 0, seconds_total = 0;
+    double seconds_graph = 0, seconds_front = 0, seconds_reduce = 0, seconds_total = 0;
     double peak_rss_mb = 0;
     std::vector<u64> columns, apparent, emergent;
 };
