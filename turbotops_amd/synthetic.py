@@ -66,3 +66,18 @@ def noisy_circle(n=2048, sigma=0.05, seed=7, dim=3):
     x[:, 0] = np.cos(t)
     x[:, 1] = np.sin(t)
     return (x + rng.normal(scale=sigma, size=x.shape)).astype(np.float32)
+
+
+def noisy_circles(count=64, nmin=512, nmax=4096, sigma=0.05, seed=9, dim=3):
+    """`count` noisy circles as noisy_circle, n_i ~ U{nmin..nmax}, drawn from
+    one default_rng(seed) stream."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(count):
+        n = int(rng.integers(nmin, nmax + 1))
+        t = rng.uniform(0, 2 * np.pi, size=n)
+        x = np.zeros((n, dim))
+        x[:, 0] = np.cos(t)
+        x[:, 1] = np.sin(t)
+        out.append((x + rng.normal(scale=sigma, size=x.shape)).astype(np.float32))
+    return out
