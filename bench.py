@@ -180,6 +180,66 @@ def run_edges(args):
                                    f"enclosing radius {thr:.7g}", "edges": m}}
 
 
+VALU_PEAK_TFLOPS = 157.3   # MI355X dense FP32 vector peak
+
+
+def run_graph(args):
+    """The point-space front-end stages on their own clouds: exact kNN graph
+    (k = 15) and radius graph (tau = 0.15) on n = 200000 points U[0,1]^8
+    (seed 5), and H0 by Boruvka on n = 1e6 points U[0,1]^3 (seed 4).
+    Host-to-host calls (points in, CSR / deaths out); the PCIe share is under
+    2 ms of each. Roofline: FP32 VALU, 3d + 1 flops per pair evaluation."""
+    from turbotops_amd import knn_graph_exact, mst_deaths, radius_graph
+    w = args.workload
+    if w == "c2c-mst":
+        x = np.random.default_rng(4).uniform(size=(1000000, 3)).astype(np.float32)
+    else:
+        x = np.random.default_rng(5).uniform(size=(200000, 8)).astype(np.float32)
+    n, d = x.shape
+    fn = {"c3a-knn": lambda: knn_graph_exact(x, "l2", 15), "c3b-radius": lambda: radius_graph(x, "l2", 0.15),
+          "c2c-mst": lambda: mst_deaths(x, "l2")}[w]
+    for _ in range(args.warmup):
+        res = fn()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        res = fn()
+    per = (time.perf_counter() - t0) / args.steps
+    if w == "c3a-knn":
+        pairs = float(n) * (n - 1)                 # every query scans every candidate
+        size = {"edges": int(res.row_ptr[-1]) // 2}
+    elif w == "c3b-radius":
+        pairs = float(n) * (n - 1) / 2             # lower-triangle tiles
+        size = {"edges": int(res.row_ptr[-1]) // 2}
+    else:
+        pairs = None
+        size = {"deaths": int(res.size)}
+    out = {"metric": f"{w} graph stage, seconds per call", "value": round(per, 5), "unit": "s",
+           "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(per * 1e3, 3),
+           "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+           "config": dict({"workload": f"{w}: n={n} d={d} L2"}, **size)}
+    if pairs is not None:
+        tflops = pairs * (3 * d + 1) / per / 1e12
+        out["roofline"] = {"bound": "valu", "achieved": round(tflops, 2), "peak": VALU_PEAK_TFLOPS,
+                           "unit": "TFLOP/s", "frac": round(tflops / VALU_PEAK_TFLOPS, 4), "traffic": None}
+    if not args.no_cpu:
+        from oracle import Reference, reference_available
+        if reference_available(simd=True):
+            threads = min(CPU_CORES_ALLOTTED, os.cpu_count() or 1)
+            ref = Reference(simd=True)
+            if w == "c2c-mst":
+                m = 100000                         # Prim is O(n^2): time a sample, scale by n^2
+                secs, _ = ref.time_graph(x[:m], "l2", "mst", threads=threads)
+                out["cpu_baseline"] = {"value": round(secs * (n / m) ** 2, 2), "unit": "s", "cores": threads,
+                                       "kind": "reference",
+                                       "sample": f"mst::h0 on the first {m} points: {secs:.2f} s, scaled by (n/m)^2"}
+            else:
+                which = "knn" if w == "c3a-knn" else "radius"
+                secs, _ = ref.time_graph(x, "l2", which, tau=0.15, k=15, threads=threads)
+                out["cpu_baseline"] = {"value": round(secs, 3), "unit": "s", "cores": threads, "kind": "reference",
+                                       "sample": f"the full workload once ({which}, k-d tree, {threads} threads)"}
+    return out
+
+
 def c4_clouds():
     """The 512-cloud batched set: n_i ~ U{200..4000}, d_i in {2, 3, 8}, noisy
     unions of 1-4 circles/spheres (sigma 0.02), seed 6."""
@@ -212,11 +272,14 @@ def run_batched(args, metric="l2"):
         bp.front_end(clouds)
         runs.append((time.perf_counter() - t0, bp.device_ms))
     wall, dev_ms = min(runs, key=lambda r: r[1])
+    dist_ms = bp.time_distance_stage()
     kept, non_apparent = bp.edges
     out = {"workload": f"512 clouds, n_i ~ U{{200..4000}}, d_i in {{2,3,8}}, circles/spheres sigma 0.02, seed 6, "
                        f"{metric}, maxdim 1",
            "front_end_device_ms": round(dev_ms, 3), "front_end_wall_ms": round(wall * 1e3, 3),
-           "distance_stage_bytes": algo, "front_end_gbs": round(algo / (dev_ms * 1e-3) / 1e9, 2),
+           "distance_stage_bytes": algo, "distance_stage_ms": round(dist_ms, 3),
+           "distance_stage_gbs": round(algo / (dist_ms * 1e-3) / 1e9, 1),
+           "distance_stage_frac": round(algo / (dist_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
            "edges_kept": int(kept), "edges_not_apparent": int(non_apparent),
            "rows_per_wave": bp.rows_per_wave, "setup_s": round(setup_s, 3)}
     del bp
@@ -251,6 +314,11 @@ def run_differentiable(args):
     from turbotops_amd import diagram, diagram_backward, top_k_persistence_loss
     from turbotops_amd.synthetic import noisy_circle
     n = args.c5_n
+    # one tiny forward + backward off the clock: loads the kernels that torch
+    # and the library use the first time
+    xw = torch.tensor(noisy_circle(n=64, seed=1), device="cuda", requires_grad=True)
+    top_k_persistence_loss(diagram(xw, maxdim=1)[1], 8).backward()
+    torch.cuda.synchronize()
     x = torch.tensor(noisy_circle(n=n, seed=7), device="cuda", requires_grad=True)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -274,11 +342,33 @@ def run_differentiable(args):
         diagram_backward(pts, e, gr)
     torch.cuda.synchronize()
     kern = (time.perf_counter() - t3) / 10
-    return {"workload": f"noisy circle n={n} in R^3 (radius 1, sigma 0.05, seed 7), L2, loss -sum(death - birth) "
-                        f"over the top-8 H1 bars",
-            "forward_s": round(t1 - t0, 3), "backward_ms": round((t2 - t1) * 1e3, 3),
-            "loss": round(float(loss.detach()), 6), "h1_bars": int(dg[1].shape[0]),
-            "backward_call_ms_2e6_edges": round(kern * 1e3, 3)}
+    out = {"workload": f"noisy circle n={n} in R^3 (radius 1, sigma 0.05, seed 7), L2, loss -sum(death - birth) "
+                       f"over the top-8 H1 bars",
+           "forward_s": round(t1 - t0, 3), "backward_ms": round((t2 - t1) * 1e3, 3),
+           "loss": round(float(loss.detach()), 6), "h1_bars": int(dg[1].shape[0]),
+           "backward_call_ms_2e6_edges": round(kern * 1e3, 3)}
+    if args.c5_batch > 0:
+        from turbotops_amd import DiagramBatch
+        from turbotops_amd.synthetic import noisy_circles
+        clouds = noisy_circles(count=args.c5_batch, nmin=512, nmax=args.c5_batch_nmax, seed=9)
+        db = DiagramBatch([c.shape for c in clouds])
+
+        def batch_step():
+            xs = [torch.tensor(c, device="cuda", requires_grad=True) for c in clouds]
+            loss = sum(top_k_persistence_loss(o[1], 8) for o in db(xs))
+            loss.backward()
+            torch.cuda.synchronize()
+            return xs
+
+        batch_step()                     # autotune + graph capture
+        t0 = time.perf_counter()
+        batch_step()
+        tb = time.perf_counter() - t0
+        out["batch"] = {"clouds": len(clouds), "step_s": round(tb, 3),
+                        "diagrams_per_sec": round(len(clouds) / tb, 3),
+                        "sample": f"{len(clouds)} noisy circles with n_i ~ U{{512..{args.c5_batch_nmax}}} (seed 9) "
+                                  f"of the 64-cloud U{{512..4096}} batch; forward + backward per step"}
+    return out
 
 
 def copy_probe(args):
@@ -299,14 +389,17 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="c2a",
-                    help="c2a | c2b-l1 | c2b-linf | c2b-lp (distance matrix); c2a-edges (dimension-0 edges)")
+                    help="c2a | c2b-l1 | c2b-linf | c2b-lp (distance matrix); c2a-edges (dimension-0 edges); "
+                         "c3a-knn | c3b-radius | c2c-mst (graph stages)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--copy-probe", action="store_true", help="run only the copy kernel (PMC scaling)")
     ap.add_argument("--no-batched", action="store_true", help="skip the 512-cloud batched leg")
     ap.add_argument("--no-diff", action="store_true", help="skip the differentiable leg")
-    ap.add_argument("--c4-cap", type=int, default=1200, help="largest n_i in the diagrams/sec sample")
-    ap.add_argument("--c4-sample", type=int, default=48, help="clouds in the diagrams/sec sample")
+    ap.add_argument("--c4-cap", type=int, default=1500, help="largest n_i in the diagrams/sec sample")
+    ap.add_argument("--c4-sample", type=int, default=128, help="clouds in the diagrams/sec sample")
     ap.add_argument("--c5-n", type=int, default=2048, help="points of the differentiable circle")
+    ap.add_argument("--c5-batch", type=int, default=16, help="clouds in the differentiable batch sample")
+    ap.add_argument("--c5-batch-nmax", type=int, default=1024, help="largest n_i in that sample")
     args = ap.parse_args()
     if args.gpus != 1:
         raise SystemExit("bench.py measures one GPU: this build runs the front end on a single MI355X "
@@ -315,6 +408,9 @@ def main():
         return copy_probe(args)
     if args.workload.endswith("-edges"):
         print(json.dumps(run_edges(args)), flush=True)
+        return
+    if args.workload in ("c3a-knn", "c3b-radius", "c2c-mst"):
+        print(json.dumps(run_graph(args)), flush=True)
         return
     out = run_dense(args)
     if not args.no_batched:

@@ -215,6 +215,11 @@ tt_status tt_batch_cloud(tt_batch* b, uint32_t cloud, const float** mat, float* 
  * the kept / non-apparent edge counts. */
 tt_status tt_batch_info(tt_batch* b, float* device_ms, int* rows_per_wave, uint64_t* edges_total,
                         uint64_t* edges_kept);
+/* The distance stage of the batch alone, re-run on the points of the last
+ * run with the tuned tile shape and timed with HIP events on the batch's
+ * stream (best of two launches after a warm-up), in milliseconds. The
+ * matrices it rewrites are bit-identical. */
+tt_status tt_batch_time_dist(tt_batch* b, float* ms);
 void tt_batch_destroy(tt_batch* b);
 
 /* ---- differentiation ------------------------------------------------------- */

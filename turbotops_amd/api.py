@@ -97,6 +97,10 @@ class BatchPersistence:
     def device_ms(self):
         return self._r.device_ms
 
+    def time_distance_stage(self):
+        """The distance stage alone, re-run on the last points (ms)."""
+        return self._r.time_distance_stage()
+
     @property
     def edges(self):
         """(kept edges, kept edges not in an apparent pair) of the last run."""

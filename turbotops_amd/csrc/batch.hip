@@ -633,6 +633,28 @@ tt_status tt_batch_info(tt_batch* b, float* device_ms, int* rows_per_wave, uint6
     });
 }
 
+tt_status tt_batch_time_dist(tt_batch* b, float* ms) {
+    return guarded([&] {
+        if (!b || !ms) fail(TT_EINVAL, "tt_batch_time_dist: null argument");
+        BatchState& st = b->st;
+        if (!st.tuned) fail(TT_EINVAL, "tt_batch_time_dist: run the batch first");
+        float best = 1e30f;
+        for (int rep = 0; rep < 3; ++rep) {   // the first launch warms up
+            fill_device_u32(st.d_rowmax, std::max<u64>(st.total_rows, 1), 0u, st.stream);
+            TTG_HIP(hipEventRecord(st.t0, st.stream));
+            if (st.total_tiles)
+                launch_batch_dist_v(st.variant, st.rpw, st.grid_dist, st.stream, st.shape, st.ma, st.d_pts,
+                                    st.d_mats, st.d_rowmax);
+            TTG_HIP(hipEventRecord(st.t1, st.stream));
+            TTG_HIP(hipEventSynchronize(st.t1));
+            float t = 0;
+            TTG_HIP(hipEventElapsedTime(&t, st.t0, st.t1));
+            if (rep) best = std::min(best, t);
+        }
+        *ms = best;
+    });
+}
+
 void tt_batch_destroy(tt_batch* b) { delete b; }
 
 }  // extern "C"

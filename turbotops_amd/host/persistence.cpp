@@ -442,6 +442,12 @@ CriticalEdges critical_edges(const Barcode& bc, u32 n, const std::function<f32(u
     return ce;
 }
 
+float BatchRunner::time_distance_stage() {
+    float ms = 0;
+    check(tt_batch_time_dist(h_, &ms));
+    return ms;
+}
+
 void BatchRunner::front(const f32* pts, bool use_graph) {
     const u64 total = total_points();
     for (u64 i = 0; i < total; ++i)

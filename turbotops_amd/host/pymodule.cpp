@@ -221,6 +221,7 @@ PYBIND11_MODULE(_turbotops, m) {
         .def("front", &batch_front, py::arg("clouds"), py::arg("use_graph") = true)
         .def("run", &batch_run, py::arg("clouds"), py::arg("use_graph") = true, py::arg("threads") = 0,
              py::arg("pairs") = false)
+        .def("time_distance_stage", &BatchRunner::time_distance_stage)
         .def_property_readonly("device_ms", &BatchRunner::device_ms)
         .def_property_readonly("rows_per_wave", &BatchRunner::rows_per_wave)
         .def_property_readonly("edges_total", &BatchRunner::edges_total)

@@ -185,6 +185,8 @@ class BatchRunner {
     // scan) and the copies of its results to host memory, no reduction.
     void front(const f32* packed_points, bool use_graph = true);
     float device_ms() const { return device_ms_; }
+    // the distance stage alone (tt_batch_time_dist), in ms
+    float time_distance_stage();
     int rows_per_wave() const { return rpw_; }
     u64 edges_total() const { return edges_total_; }
     u64 edges_kept() const { return edges_kept_; }
