@@ -97,8 +97,8 @@ __device__ __forceinline__ const float* shfl_ptr(const float* p, int src) {
 }
 
 // Apparent flags of up to 64 edges at once, one per lane (whole wave). Each
-// lane passes its edge's cloud: mat (its full n x n matrix), bmin (its block
-// minima, nb per row), n, and `valid` = the lane holds an edge.
+// lane passes its edge's cloud: mat (its full n x n matrix, row stride ld),
+// bmin (its block minima, nb per row), n, and `valid` = the lane holds an edge.
 // Phase 1: the lanes walk their top TOPK candidates w = n-1, n-2, ... in
 // step; d(i, w) is read in row i (one address for the lanes of one row) and
 // d(j, w) as mat[w * n + j], which the full layout holds bit for bit equal to
@@ -107,8 +107,8 @@ __device__ __forceinline__ const float* shfl_ptr(const float* p, int src) {
 // wave with the block filter (edge_is_apparent).
 template <u32 TOPK>
 __device__ __forceinline__ bool edges_apparent_lanes(const float* __restrict__ mat, const float* __restrict__ bmin,
-                                                     u32 n, u32 nb, bool valid, u32 i, u32 j, float d,
-                                                     u32 lane) {
+                                                     u32 n, u32 ld, u32 nb, bool valid, u32 i, u32 j,
+                                                     float d, u32 lane) {
     bool done = !valid, ap = false;
     for (u32 k = 0; k < TOPK; ++k) {
         if (!done) {
@@ -117,8 +117,8 @@ __device__ __forceinline__ bool edges_apparent_lanes(const float* __restrict__ m
             } else {
                 const u32 w = n - 1 - k;
                 if (w != i && w != j) {
-                    const float a = mat[u64(i) * n + w];
-                    const float c = mat[u64(w) * n + j];
+                    const float a = mat[u64(i) * ld + w];
+                    const float c = mat[u64(w) * ld + j];
                     if (fmax_ref(a, c) <= d) {
                         ap = facet_rule(w, i, j, d, a, c);
                         done = true;
@@ -133,10 +133,11 @@ __device__ __forceinline__ bool edges_apparent_lanes(const float* __restrict__ m
         const int src = __builtin_ctzll(open);
         open &= open - 1;
         const u32 ei = __shfl(i, src), ej = __shfl(j, src), en = __shfl(n, src), enb = __shfl(nb, src);
+        const u32 eld = __shfl(ld, src);
         const float ed = __shfl(d, src);
         const float* em = shfl_ptr(mat, src);
         const float* eb = shfl_ptr(bmin, src);
-        const bool r = edge_is_apparent(em + u64(ei) * en, em + u64(ej) * en, eb + u64(ei) * enb,
+        const bool r = edge_is_apparent(em + u64(ei) * eld, em + u64(ej) * eld, eb + u64(ei) * enb,
                                         eb + u64(ej) * enb, en, enb, ei, ej, ed, lane, en - TOPK);
         if (int(lane) == src) ap = r;
     }

@@ -14,9 +14,9 @@
 // data-dependent count from device memory, so the whole sequence can be
 // replayed from a graph:
 //   batch_dist     persistent: blocks stride over the global tile list; a
-//                  binary search over tile offsets names the cloud, and the
-//                  tile runs the single-cloud tile body (tiles.hpp) for that
-//                  cloud's dimension
+//                  per-tile cloud table names the cloud, and the tile runs the
+//                  single-cloud tile body (tiles.hpp) for that cloud's
+//                  dimension
 //   batch_radius   one workgroup per cloud: min of the row maxima
 //   batch_count    one wave per global row: kept edges j < i
 //   scan           global row offsets; cloud segments follow
@@ -38,6 +38,7 @@
 #include <cstring>
 #include <map>
 #include <mutex>
+#include <numeric>
 #include <vector>
 
 #include "apparent.hpp"
@@ -53,23 +54,22 @@ struct BatchShape {
     const u32* n;
     const u32* dim;
     const u64* pt_off;
-    const u64* mat_off;
+    const u64* mat_off;     // device matrices: rows padded to row_ld(n) floats
     const u64* row_base;    // nclouds + 1
     const u64* tile_base;   // nclouds + 1
     const u64* pair_base;   // nclouds + 1 (key capacity offsets)
     const u64* bmin_off;    // nclouds + 1 (row block minima offsets)
+    const u32* tile_cloud;  // total_tiles: the cloud of each tile
+    const u32* row_cloud;   // total_rows: the cloud of each row
     u64 total_tiles, total_rows;
+    u64 tile_mul;           // coprime with total_tiles: the tile visiting order
 };
 
-__device__ __forceinline__ u32 find_segment(const u64* base, u32 count, u64 t) {
-    u32 lo = 0, hi = count;   // base[lo] <= t < base[hi]
-    while (hi - lo > 1) {
-        const u32 mid = (lo + hi) >> 1;
-        if (base[mid] <= t) lo = mid;
-        else hi = mid;
-    }
-    return lo;
-}
+// Row stride of a device matrix: n rounded up to 32 floats, so that every
+// row starts on a 128-byte line and the tiles' 256-byte row segments are
+// whole lines (a segment that straddles lines halves the write rate). The
+// host copy is unpadded.
+__host__ __device__ __forceinline__ u32 row_ld(u32 n) { return (n + 31u) & ~31u; }
 
 template <int V, int RPW>
 __global__ __launch_bounds__(64 * (64 / RPW)) void batch_dist(BatchShape b, MetricArgs ma,
@@ -77,8 +77,12 @@ __global__ __launch_bounds__(64 * (64 / RPW)) void batch_dist(BatchShape b, Metr
                                                               float* __restrict__ mats,
                                                               u32* __restrict__ rowmax) {
     __shared__ TileSmem<RPW> sm;
-    for (u64 t = blockIdx.x; t < b.total_tiles; t += gridDim.x) {
-        const u32 c = __builtin_amdgcn_readfirstlane(find_segment(b.tile_base, b.nclouds, t));
+    for (u64 s = blockIdx.x; s < b.total_tiles; s += gridDim.x) {
+        // visit the tiles in a scattered order, so the tiles in flight at one
+        // time belong to many clouds and the row-maximum atomics of a small
+        // cloud do not pile onto a few cache lines at once
+        const u64 t = (s * b.tile_mul) % b.total_tiles;
+        const u32 c = __builtin_amdgcn_readfirstlane(b.tile_cloud[t]);
         const u32 n = b.n[c], dim = b.dim[c];
         const float* x = pts + b.pt_off[c];
         float* mat = mats + b.mat_off[c];
@@ -86,10 +90,10 @@ __global__ __launch_bounds__(64 * (64 / RPW)) void batch_dist(BatchShape b, Metr
         u32 bi, bj;
         lower_tile_of(t - b.tile_base[c], bi, bj);
         switch (dim) {
-            case 2: dist_tile<V, 2, TT_LAYOUT_FULL, RPW>(x, n, dim, ma, bi, bj, mat, rm, sm); break;
-            case 3: dist_tile<V, 3, TT_LAYOUT_FULL, RPW>(x, n, dim, ma, bi, bj, mat, rm, sm); break;
-            case 8: dist_tile<V, 8, TT_LAYOUT_FULL, RPW>(x, n, dim, ma, bi, bj, mat, rm, sm); break;
-            default: dist_tile<V, 0, TT_LAYOUT_FULL, RPW>(x, n, dim, ma, bi, bj, mat, rm, sm); break;
+            case 2: dist_tile<V, 2, TT_LAYOUT_FULL, RPW>(x, n, dim, ma, bi, bj, mat, rm, sm, row_ld(n)); break;
+            case 3: dist_tile<V, 3, TT_LAYOUT_FULL, RPW>(x, n, dim, ma, bi, bj, mat, rm, sm, row_ld(n)); break;
+            case 8: dist_tile<V, 8, TT_LAYOUT_FULL, RPW>(x, n, dim, ma, bi, bj, mat, rm, sm, row_ld(n)); break;
+            default: dist_tile<V, 0, TT_LAYOUT_FULL, RPW>(x, n, dim, ma, bi, bj, mat, rm, sm, row_ld(n)); break;
         }
         __syncthreads();
     }
@@ -118,10 +122,10 @@ __global__ __launch_bounds__(256) void batch_count(BatchShape b, const float* __
     const u64 gw = (blockIdx.x * 256ull + threadIdx.x) >> 6;
     const u64 nw = (u64(gridDim.x) * 256) >> 6;
     for (u64 r = gw; r < b.total_rows; r += nw) {
-        const u32 c = find_segment(b.row_base, b.nclouds, r);
+        const u32 c = b.row_cloud[r];
         const u32 n = b.n[c];
         const u32 i = u32(r - b.row_base[c]);
-        const float* row = mats + b.mat_off[c] + u64(i) * n;
+        const float* row = mats + b.mat_off[c] + u64(i) * row_ld(n);
         const float th = thr[c];
         u32 k = 0;
         for (u32 j = lane; j < i; j += 64) k += row[j] <= th;
@@ -143,10 +147,10 @@ __global__ __launch_bounds__(256) void batch_compact(BatchShape b, const float* 
     const u64 gw = (blockIdx.x * 256ull + threadIdx.x) >> 6;
     const u64 nw = (u64(gridDim.x) * 256) >> 6;
     for (u64 r = gw; r < b.total_rows; r += nw) {
-        const u32 c = find_segment(b.row_base, b.nclouds, r);
+        const u32 c = b.row_cloud[r];
         const u32 n = b.n[c];
         const u32 i = u32(r - b.row_base[c]);
-        const float* row = mats + b.mat_off[c] + u64(i) * n;
+        const float* row = mats + b.mat_off[c] + u64(i) * row_ld(n);
         const float th = thr[c];
         const u64 M = u64(n) * (n - 1) / 2;
         const u64 cloud_bits = u64(c) << (31 + ib);
@@ -183,11 +187,12 @@ __global__ __launch_bounds__(256) void batch_block_min(BatchShape b, const float
     const u64 gw = (blockIdx.x * 256ull + threadIdx.x) >> 6;
     const u64 nw = (u64(gridDim.x) * 256) >> 6;
     for (u64 r = gw; r < b.total_rows; r += nw) {
-        const u32 c = find_segment(b.row_base, b.nclouds, r);
+        const u32 c = b.row_cloud[r];
         const u32 n = b.n[c];
         const u32 i = u32(r - b.row_base[c]);
         const u32 nb = (n + BLK - 1) / BLK;
-        block_min_row(mats + b.mat_off[c] + u64(i) * n, n, nb, bmins + b.bmin_off[c] + u64(i) * nb, lane);
+        block_min_row(mats + b.mat_off[c] + u64(i) * row_ld(n), n, nb, bmins + b.bmin_off[c] + u64(i) * nb,
+                      lane);
     }
 }
 
@@ -219,7 +224,7 @@ __global__ __launch_bounds__(256) void batch_apparent(BatchShape b, const float*
             mat = mats + b.mat_off[c];
             bm = bmins + b.bmin_off[c];
         }
-        const bool ap = edges_apparent_lanes<TOPK>(mat, bm, n, nb, valid, i, j, d, lane);
+        const bool ap = edges_apparent_lanes<TOPK>(mat, bm, n, row_ld(n), nb, valid, i, j, d, lane);
         if (valid) keep[e] = ap ? 0u : 1u;
     }
 }
@@ -299,7 +304,7 @@ struct BatchState {
     int rpw = 16, ib = 1, cb = 0;
     bool tuned = false;
     std::vector<u32> n, dim;
-    std::vector<u64> pt_off, mat_off, row_base, tile_base, pair_base, bmin_off;
+    std::vector<u64> pt_off, mat_off, dmat_off, row_base, tile_base, pair_base, bmin_off;
     u64 total_pts = 0, total_mat = 0, total_rows = 0, total_tiles = 0, capacity = 0, h_cap = 0;
     u32 grid_dist = 0;
     // device
@@ -310,6 +315,7 @@ struct BatchState {
         *d_n = nullptr, *d_dim = nullptr;
     u64 *d_keys = nullptr, *d_alt = nullptr, *d_out_idx = nullptr, *d_pt_off = nullptr, *d_mat_off = nullptr,
         *d_row_base = nullptr, *d_tile_base = nullptr, *d_pair_base = nullptr, *d_bmin_off = nullptr;
+    u32 *d_tile_cloud = nullptr, *d_row_cloud = nullptr;
     BatchShape shape{};
     // host (pinned)
     float* h_pts = nullptr;
@@ -377,6 +383,15 @@ struct tt_batch {
 
 namespace {
 
+// An odd multiplier near 0.618 T that is coprime with T (a permutation of
+// the tile ids when used as t * A mod T).
+u64 tile_stride(u64 T) {
+    if (T <= 2) return 1;
+    u64 a = u64(double(T) * 0.6180339887) | 1;
+    while (std::gcd(a, T) != 1) a += 2;
+    return a;
+}
+
 void batch_setup(BatchState& b, u32 nclouds, const uint32_t* n, const uint32_t* dim, int kind, float p,
                  float threshold) {
     check_metric(kind, p);
@@ -394,6 +409,7 @@ void batch_setup(BatchState& b, u32 nclouds, const uint32_t* n, const uint32_t* 
     b.dim.assign(dim, dim + nclouds);
     b.pt_off.assign(nclouds + 1, 0);
     b.mat_off.assign(nclouds + 1, 0);
+    b.dmat_off.assign(nclouds + 1, 0);
     b.row_base.assign(nclouds + 1, 0);
     b.tile_base.assign(nclouds + 1, 0);
     b.pair_base.assign(nclouds + 1, 0);
@@ -407,6 +423,7 @@ void batch_setup(BatchState& b, u32 nclouds, const uint32_t* n, const uint32_t* 
         maxM = std::max<u64>(maxM, M);
         b.pt_off[c + 1] = b.pt_off[c] + u64(b.n[c]) * b.dim[c];
         b.mat_off[c + 1] = b.mat_off[c] + u64(b.n[c]) * b.n[c];
+        b.dmat_off[c + 1] = b.dmat_off[c] + u64(b.n[c]) * row_ld(b.n[c]);
         b.row_base[c + 1] = b.row_base[c] + b.n[c];
         b.tile_base[c + 1] = b.tile_base[c] + nb * (nb + 1) / 2;
         b.pair_base[c + 1] = b.pair_base[c] + M;
@@ -425,7 +442,7 @@ void batch_setup(BatchState& b, u32 nclouds, const uint32_t* n, const uint32_t* 
 
     Workspace& ws = b.ws;
     b.d_pts = ws.get<float>(std::max<u64>(b.total_pts, 1));
-    b.d_mats = ws.get<float>(std::max<u64>(b.total_mat, 1));
+    b.d_mats = ws.get<float>(std::max<u64>(b.dmat_off[nclouds], 1));
     b.d_rowmax = ws.get<u32>(std::max<u64>(b.total_rows, 1));
     b.d_thr = ws.get<float>(nclouds);
     b.d_cnt = ws.get<u32>(b.total_rows + 1);
@@ -451,20 +468,33 @@ void batch_setup(BatchState& b, u32 nclouds, const uint32_t* n, const uint32_t* 
     b.d_tile_base = ws.get<u64>(nclouds + 1);
     b.d_pair_base = ws.get<u64>(nclouds + 1);
     b.d_bmin_off = ws.get<u64>(nclouds + 1);
+    b.d_tile_cloud = ws.get<u32>(std::max<u64>(b.total_tiles, 1));
+    b.d_row_cloud = ws.get<u32>(std::max<u64>(b.total_rows, 1));
     b.d_bmin = ws.get<float>(std::max<u64>(b.bmin_off[nclouds], 1));
     TTG_HIP(hipStreamCreateWithFlags(&b.stream, hipStreamNonBlocking));
     hipStream_t s = b.stream;
     TTG_HIP(hipMemcpyAsync(b.d_n, b.n.data(), nclouds * 4, hipMemcpyHostToDevice, s));
     TTG_HIP(hipMemcpyAsync(b.d_dim, b.dim.data(), nclouds * 4, hipMemcpyHostToDevice, s));
     TTG_HIP(hipMemcpyAsync(b.d_pt_off, b.pt_off.data(), (nclouds + 1) * 8, hipMemcpyHostToDevice, s));
-    TTG_HIP(hipMemcpyAsync(b.d_mat_off, b.mat_off.data(), (nclouds + 1) * 8, hipMemcpyHostToDevice, s));
+    TTG_HIP(hipMemcpyAsync(b.d_mat_off, b.dmat_off.data(), (nclouds + 1) * 8, hipMemcpyHostToDevice, s));
     TTG_HIP(hipMemcpyAsync(b.d_row_base, b.row_base.data(), (nclouds + 1) * 8, hipMemcpyHostToDevice, s));
     TTG_HIP(hipMemcpyAsync(b.d_tile_base, b.tile_base.data(), (nclouds + 1) * 8, hipMemcpyHostToDevice, s));
     TTG_HIP(hipMemcpyAsync(b.d_pair_base, b.pair_base.data(), (nclouds + 1) * 8, hipMemcpyHostToDevice, s));
     TTG_HIP(hipMemcpyAsync(b.d_bmin_off, b.bmin_off.data(), (nclouds + 1) * 8, hipMemcpyHostToDevice, s));
+    {   // per tile and per row, the cloud it belongs to (one load instead of a search)
+        std::vector<u32> tc(std::max<u64>(b.total_tiles, 1)), rc(std::max<u64>(b.total_rows, 1));
+        for (u32 c = 0; c < nclouds; ++c) {
+            std::fill(tc.begin() + b.tile_base[c], tc.begin() + b.tile_base[c + 1], c);
+            std::fill(rc.begin() + b.row_base[c], rc.begin() + b.row_base[c + 1], c);
+        }
+        TTG_HIP(hipMemcpyAsync(b.d_tile_cloud, tc.data(), tc.size() * 4, hipMemcpyHostToDevice, s));
+        TTG_HIP(hipMemcpyAsync(b.d_row_cloud, rc.data(), rc.size() * 4, hipMemcpyHostToDevice, s));
+        TTG_HIP(hipStreamSynchronize(s));
+    }
     TTG_HIP(hipStreamSynchronize(s));
     b.shape = BatchShape{nclouds, b.d_n, b.d_dim, b.d_pt_off, b.d_mat_off, b.d_row_base, b.d_tile_base,
-                         b.d_pair_base, b.d_bmin_off, b.total_tiles, b.total_rows};
+                         b.d_pair_base, b.d_bmin_off, b.d_tile_cloud, b.d_row_cloud, b.total_tiles,
+                         b.total_rows, tile_stride(b.total_tiles)};
     TTG_HIP(hipHostMalloc(reinterpret_cast<void**>(&b.h_pts), std::max<u64>(b.total_pts, 1) * 4, hipHostMallocDefault));
     TTG_HIP(hipHostMalloc(reinterpret_cast<void**>(&b.h_mats), std::max<u64>(b.total_mat, 1) * 4, hipHostMallocDefault));
     b.ev.resize(nclouds);
@@ -556,10 +586,10 @@ void batch_run(BatchState& b, const float* pts, int use_graph) {
     }
     // per cloud: matrix + its kept edges, then an event the host waits on
     for (u32 c = 0; c < b.nclouds; ++c) {
-        const u64 cells = b.mat_off[c + 1] - b.mat_off[c];
-        if (cells)
-            TTG_HIP(hipMemcpyAsync(b.h_mats + b.mat_off[c], b.d_mats + b.mat_off[c], cells * 4,
-                                   hipMemcpyDeviceToHost, s));
+        const u64 nc = b.n[c];
+        if (nc)   // padded device rows -> the reference's n x n layout
+            TTG_HIP(hipMemcpy2DAsync(b.h_mats + b.mat_off[c], nc * 4, b.d_mats + b.dmat_off[c],
+                                     u64(row_ld(u32(nc))) * 4, nc * 4, nc, hipMemcpyDeviceToHost, s));
         const u64 e0 = b.kseg[c], e1 = b.kseg[c + 1];
         if (e1 > e0) {
             TTG_HIP(hipMemcpyAsync(b.h_idx + e0, b.d_out_idx + e0, (e1 - e0) * 8, hipMemcpyDeviceToHost, s));

@@ -52,7 +52,7 @@ __global__ __launch_bounds__(64 * (64 / RPW)) void dist_tiles(const float* __res
     __shared__ TileSmem<RPW> sm;
     u32 bi, bj;
     lower_tile_of(blockIdx.x, bi, bj);
-    dist_tile<V, DIM, LAYOUT, RPW>(x, n, dim_rt, ma, bi, bj, mat, rowmax, sm);
+    dist_tile<V, DIM, LAYOUT, RPW>(x, n, dim_rt, ma, bi, bj, mat, rowmax, sm, n);
 }
 
 __global__ void fill_u32(u32* __restrict__ p, u64 count, u32 v) {

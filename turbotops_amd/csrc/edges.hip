@@ -161,7 +161,7 @@ __global__ __launch_bounds__(EB) void apparent_scan(const float* __restrict__ ma
             decode_edge(M - 1 - (k & mask), i, j);
             d = mat[u64(i) * n + j];
         }
-        const bool ap = edges_apparent_lanes<TOPK>(mat, bmin, n, nb, valid, i, j, d, lane);
+        const bool ap = edges_apparent_lanes<TOPK>(mat, bmin, n, n, nb, valid, i, j, d, lane);
         if (valid) keys[e] = (k << 1) | (ap ? 1u : 0u);
     }
 }

@@ -191,15 +191,94 @@ __global__ __launch_bounds__(GB) void knn_scan(const float* __restrict__ x, u32 
     }
 }
 
+// Compile-time dimension: the candidate is the same for every lane, so its
+// coordinates come through scalar loads (the points are read-only here) and
+// feed the VALU as SGPR operands; no LDS staging, no barriers. The next
+// candidate is loaded while the current one is measured. The ordering key
+// (pdist<KEY>: for L2 the sum before the square root) rejects most
+// candidates without the exact square root: only keys at or below
+// key_reject_above(k-th distance) are turned into distances and compared by
+// the reference's (distance, id) rule.
+template <int V, int DIM, int KMAX>
+__global__ __launch_bounds__(GB) void knn_scan_s(const float* __restrict__ x, u32 n, MetricArgs ma, u32 k,
+                                                 u32* __restrict__ ids) {
+    const u32 i = blockIdx.x * GB + threadIdx.x;
+    const u32 qi = i < n ? i : n - 1;
+    float q[DIM];
+#pragma unroll
+    for (int t = 0; t < DIM; ++t) q[t] = x[u64(qi) * DIM + t];
+    float bd[KMAX];
+    u32 bid[KMAX];
+#pragma unroll
+    for (int t = 0; t < KMAX; ++t) {
+        bd[t] = u32(t) < k ? INFINITY : -INFINITY;   // slots past k never change
+        bid[t] = 0xffffffffu;
+    }
+    float kth = INFINITY;
+    u32 kthj = 0xffffffffu;
+    float reject = INFINITY;   // keys above this cannot enter
+    float c[DIM];
+#pragma unroll
+    for (int t = 0; t < DIM; ++t) c[t] = x[t];
+    for (u32 j = 0; j < n; ++j) {
+        float cur[DIM];
+#pragma unroll
+        for (int t = 0; t < DIM; ++t) cur[t] = c[t];
+        const u32 jn = j + 1 < n ? j + 1 : j;
+#pragma unroll
+        for (int t = 0; t < DIM; ++t) c[t] = x[u64(jn) * DIM + t];
+        const float key = pdist<V, DIM, true>(cur, q, DIM, ma);
+        if (key <= reject && j != qi) {
+            const float d = key_to_dist<V>(key);
+            if (d < kth || (d == kth && j < kthj)) {
+                float cd = d;
+                u32 cj = j;
+#pragma unroll
+                for (int t = 0; t < KMAX; ++t) {
+                    const bool sw = cd < bd[t] || (cd == bd[t] && cj < bid[t]);
+                    const float td = bd[t];
+                    const u32 tj = bid[t];
+                    bd[t] = sw ? cd : td;
+                    bid[t] = sw ? cj : tj;
+                    cd = sw ? td : cd;
+                    cj = sw ? tj : cj;
+                }
+                float nk = bd[0];
+                u32 nj = bid[0];
+#pragma unroll
+                for (int t = 1; t < KMAX; ++t)
+                    if (u32(t) == k - 1) {
+                        nk = bd[t];
+                        nj = bid[t];
+                    }
+                kth = nk;
+                kthj = nj;
+                reject = key_reject_above<V>(kth);
+            }
+        }
+    }
+    if (i < n) {
+#pragma unroll
+        for (int t = 0; t < KMAX; ++t)
+            if (u32(t) < k) ids[u64(i) * k + t] = bid[t];
+    }
+}
+
+template <int V, int DIM, int KMAX>
+void knn_one(u32 blocks, hipStream_t s, const float* x, u32 n, u32 dim, MetricArgs ma, u32 k, u32* ids) {
+    if constexpr (DIM > 0) knn_scan_s<V, DIM, KMAX><<<blocks, GB, 0, s>>>(x, n, ma, k, ids);
+    else knn_scan<V, DIM, KMAX><<<blocks, GB, 0, s>>>(x, n, dim, ma, k, ids);
+}
+
 template <int V, int DIM>
 struct KnnLaunch {
     static void run(u32 kmax, u32 blocks, hipStream_t s, const float* x, u32 n, u32 dim,
                     MetricArgs ma, u32 k, u32* ids) {
         switch (kmax) {
-            case 8: knn_scan<V, DIM, 8><<<blocks, GB, 0, s>>>(x, n, dim, ma, k, ids); break;
-            case 16: knn_scan<V, DIM, 16><<<blocks, GB, 0, s>>>(x, n, dim, ma, k, ids); break;
-            case 32: knn_scan<V, DIM, 32><<<blocks, GB, 0, s>>>(x, n, dim, ma, k, ids); break;
-            default: knn_scan<V, DIM, 64><<<blocks, GB, 0, s>>>(x, n, dim, ma, k, ids); break;
+            case 8: knn_one<V, DIM, 8>(blocks, s, x, n, dim, ma, k, ids); break;
+            case 16: knn_one<V, DIM, 16>(blocks, s, x, n, dim, ma, k, ids); break;
+            case 32: knn_one<V, DIM, 32>(blocks, s, x, n, dim, ma, k, ids); break;
+            default: knn_one<V, DIM, 64>(blocks, s, x, n, dim, ma, k, ids); break;
         }
     }
 };

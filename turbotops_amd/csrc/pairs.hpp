@@ -23,7 +23,9 @@ struct MetricArgs {
 
 // ---- per-pair kernels, compile-time dimension -----------------------------
 // a: row point (wave-uniform, scalar loads); b: column point in VGPRs.
-template <int V, int DIM>
+// KEY = true returns, for L2, the sum before the square root (the ordering
+// key of pdist<..., KEY>); the other metrics return the distance either way.
+template <int V, int DIM, bool KEY = false>
 __device__ __forceinline__ float pair_dist(const float* __restrict__ a, const float (&b)[DIM],
                                            const MetricArgs& ma) {
     if constexpr (V == MV_L2) {                       // metric.hpp:107-111
@@ -33,7 +35,8 @@ __device__ __forceinline__ float pair_dist(const float* __restrict__ a, const fl
             const float d = a[k] - b[k];
             s = __builtin_fmaf(d, d, s);
         }
-        return __builtin_sqrtf(s);
+        if constexpr (KEY) return s;
+        else return __builtin_sqrtf(s);
     } else if constexpr (V == MV_L1) {                // metric.hpp:157
         float s = 0.0f;
 #pragma unroll
@@ -136,7 +139,7 @@ __device__ __forceinline__ void rows_runtime(const float* __restrict__ x, u32 n,
     }
 }
 
-template <int V, int DIM, int R>
+template <int V, int DIM, int R, bool KEY = false>
 __device__ __forceinline__ void rows_static(const float* __restrict__ x, u32 n,
                                             const MetricArgs& ma, u32 ibase, u32 jc,
                                             float (&val)[R]) {
@@ -146,7 +149,7 @@ __device__ __forceinline__ void rows_static(const float* __restrict__ x, u32 n,
 #pragma unroll
     for (u32 r = 0; r < R; ++r) {
         const u32 i = min(ibase + r, n - 1);
-        val[r] = pair_dist<V, DIM>(x + u64(i) * DIM, b, ma);
+        val[r] = pair_dist<V, DIM, KEY>(x + u64(i) * DIM, b, ma);
     }
 }
 
@@ -193,6 +196,23 @@ __device__ __forceinline__ float pdist(const float* __restrict__ a, const float*
             }
             return m * pow_f32(s, ma.inv_p);
         }
+    }
+}
+
+// A key above which the distance is certainly above d: for L2 the keys are
+// sums before the square root, and any s > (d^2 (1 + 4e-7)) rounded up has
+// sqrtf(s) >= d + 1 ulp (d (1 + 2e-7) exceeds d by at least 1.68 ulp, more
+// than the half ulp of rounding). For the other metrics the key is the
+// distance itself. +inf stays +inf. Used to skip the exact square root for
+// candidates that cannot pass a "<= d" test.
+template <int V>
+__device__ __forceinline__ float key_reject_above(float d) {
+    if constexpr (V == MV_L2) {
+        if (!(d < INFINITY)) return INFINITY;
+        const float b = float(double(d) * double(d) * (1.0 + 4e-7));   // b >= 0
+        return b < INFINITY ? __uint_as_float(__float_as_uint(b) + 1u) : INFINITY;   // next float up
+    } else {
+        return d;
     }
 }
 

@@ -38,11 +38,13 @@ struct TileSmem {
     float part_col[NW][DTILE];
 };
 
+// ld: row stride of the full layout in floats (n, or n rounded up for
+// 128-byte aligned rows); unused for the lower layout.
 template <int V, int DIM, int LAYOUT, int RPW>
 __device__ __forceinline__ void dist_tile(const float* __restrict__ x, u32 n, u32 dim_rt,
                                           const MetricArgs& ma, u32 bi, u32 bj,
                                           float* __restrict__ mat, u32* __restrict__ rowmax,
-                                          TileSmem<RPW>& sm) {
+                                          TileSmem<RPW>& sm, u32 ld) {
     constexpr u32 NW = TileSmem<RPW>::NW;
     const u32 lane = threadIdx.x & 63;
     const u32 wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -63,7 +65,7 @@ __device__ __forceinline__ void dist_tile(const float* __restrict__ x, u32 n, u3
         sm.tile[wave * RPW + r][lane] = v;
         cmax = fmax_ref(cmax, v);
         if constexpr (LAYOUT == TT_LAYOUT_FULL) {
-            if (in) __builtin_nontemporal_store(v, mat + u64(i) * n + j);
+            if (in) __builtin_nontemporal_store(v, mat + u64(i) * ld + j);
         } else {
             if (in && j < i) __builtin_nontemporal_store(v, mat + u64(i) * (i - 1) / 2 + j);
         }
@@ -80,7 +82,7 @@ __device__ __forceinline__ void dist_tile(const float* __restrict__ x, u32 n, u3
         rmax = fmax_ref(rmax, v);
         if constexpr (LAYOUT == TT_LAYOUT_FULL) {
             const u32 jj = bj * DTILE + cl;
-            if (bi != bj && i < n && jj < n) __builtin_nontemporal_store(v, mat + u64(jj) * n + i);
+            if (bi != bj && i < n && jj < n) __builtin_nontemporal_store(v, mat + u64(jj) * ld + i);
         }
     }
     sm.part_row[wave][lane] = rmax;
