@@ -1,6 +1,7 @@
 // SPDX-License-Identifier: MIT
-// apparent.hpp — the zero-apparent test of an edge on a full distance
-// matrix, one wave64 per edge, with a block-minimum filter.
+// apparent.hpp — the zero-apparent test of edges on a full distance matrix,
+// 64 edges per wave: a lane-parallel walk over the top candidates, then a
+// block-minimum filtered search by 16-lane groups for the edges still open.
 //
 // Reference semantics (include/turbotops/rips.hpp:585-665 with cofacets_full
 // :326-343 and cap = the edge's diameter d): the edge (i, j), i > j, is
@@ -12,8 +13,8 @@
 // finds w, up to 2n floats per edge for short edges. Here every row also has
 // its minima over blocks of 64 columns, bmin[i][b] = min over w in block b of
 // d(i, w); a block holds a candidate only if bmin[i][b] <= d and
-// bmin[j][b] <= d, so one 64-lane pass over the block minima picks the
-// blocks worth reading, highest first, and only those are read. The filter is
+// bmin[j][b] <= d, so one pass over the block minima picks the blocks worth
+// reading, highest first, and only those are read. The filter is
 // conservative (a block's minimum includes w = i and w = j), so the first w
 // found is exactly the plain walk's.
 #pragma once
@@ -23,7 +24,7 @@
 namespace ttg {
 
 constexpr u32 BLK = 64;
-constexpr u32 TOPK = 64;   // candidates the lanes walk together before the wave search
+constexpr u32 TOPK = 64;   // candidates the lanes walk together before the grouped search
 
 __device__ __forceinline__ float wave_min(float v) {
 #pragma unroll
@@ -50,46 +51,6 @@ __device__ __forceinline__ bool facet_rule(u32 w, u32 i, u32 j, float d, float d
     return djw != d && diw != d;                // (j, w), (i, w) precede
 }
 
-// Apparent flag of edge (i, j) with diameter d, searching only w < w_hi
-// (whole wave; result valid in every lane). ri, rj: rows i and j; bi, bj:
-// their block minima.
-__device__ __forceinline__ bool edge_is_apparent(const float* __restrict__ ri, const float* __restrict__ rj,
-                                                 const float* __restrict__ bi, const float* __restrict__ bj,
-                                                 u32 n, u32 nb, u32 i, u32 j, float d, u32 lane,
-                                                 u32 w_hi) {
-    int wf = -1;
-    float diw = 0.0f, djw = 0.0f;
-    const int top_blk = w_hi ? int((w_hi - 1) / BLK) : -1;
-    (void)nb;
-    for (int top = top_blk; top >= 0 && wf < 0; top -= 64) {
-        const int b = top - int(lane);
-        bool cand = false;
-        if (b >= 0) cand = fmax_ref(bi[b], bj[b]) <= d;
-        u64 bal = __ballot(cand);
-        while (bal) {
-            const int blk = top - __builtin_ctzll(bal);   // highest remaining block first
-            bal &= bal - 1;
-            const int w = blk * int(BLK) + int(BLK - 1 - lane);
-            float a = INFINITY, c = INFINITY;
-            const bool in = u32(w) < w_hi;              // w_hi <= n
-            if (in) {
-                a = ri[w];
-                c = rj[w];
-            }
-            const bool ok = in && u32(w) != i && u32(w) != j && fmax_ref(a, c) <= d;
-            const u64 hit = __ballot(ok);
-            if (hit) {
-                const int src = __builtin_ctzll(hit);       // lowest lane = largest w
-                wf = blk * int(BLK) + int(BLK - 1) - src;
-                diw = __shfl(a, src);
-                djw = __shfl(c, src);
-                break;
-            }
-        }
-    }
-    return wf >= 0 && facet_rule(u32(wf), i, j, d, diw, djw);
-}
-
 __device__ __forceinline__ const float* shfl_ptr(const float* p, int src) {
     const u64 v = reinterpret_cast<u64>(p);
     const u32 lo = __shfl(u32(v), src), hi = __shfl(u32(v >> 32), src);
@@ -103,8 +64,11 @@ __device__ __forceinline__ const float* shfl_ptr(const float* p, int src) {
 // step; d(i, w) is read in row i (one address for the lanes of one row) and
 // d(j, w) as mat[w * n + j], which the full layout holds bit for bit equal to
 // d(w, j), so consecutive edges of one row read one coalesced segment.
-// Phase 2: each edge still open is searched below its top TOPK by the whole
-// wave with the block filter (edge_is_apparent).
+// Phase 2: each edge still open is searched below its top TOPK with the block
+// filter: the block minima of rows i and j pick the blocks that can hold a
+// w with max(d(i,w), d(j,w)) <= d, highest first, and a ballot over each such
+// block finds the first cofacet. The filter is conservative (a block's minimum
+// includes w = i and w = j), so the first w found is exactly the plain walk's.
 template <u32 TOPK>
 __device__ __forceinline__ bool edges_apparent_lanes(const float* __restrict__ mat, const float* __restrict__ bmin,
                                                      u32 n, u32 ld, u32 nb, bool valid, u32 i, u32 j,
@@ -128,18 +92,89 @@ __device__ __forceinline__ bool edges_apparent_lanes(const float* __restrict__ m
         }
         if (!__ballot(!done)) break;
     }
+    // Phase 2, four open edges at a time: lanes 16g .. 16g + 15 search the
+    // g-th one, each lane checking 4 of 64 blocks (or of 64 candidates) per
+    // step, so four searches overlap their dependent loads.
     u64 open = __ballot(!done && n > TOPK);
+    const u32 g = lane >> 4, gl = lane & 15;
     while (open) {
-        const int src = __builtin_ctzll(open);
-        open &= open - 1;
-        const u32 ei = __shfl(i, src), ej = __shfl(j, src), en = __shfl(n, src), enb = __shfl(nb, src);
-        const u32 eld = __shfl(ld, src);
-        const float ed = __shfl(d, src);
-        const float* em = shfl_ptr(mat, src);
-        const float* eb = shfl_ptr(bmin, src);
-        const bool r = edge_is_apparent(em + u64(ei) * eld, em + u64(ej) * eld, eb + u64(ei) * enb,
-                                        eb + u64(ej) * enb, en, enb, ei, ej, ed, lane, en - TOPK);
-        if (int(lane) == src) ap = r;
+        int src[4];
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            src[t] = open ? __builtin_ctzll(open) : -1;
+            open &= open ? open - 1 : 0;
+        }
+        const int mine = g == 0 ? src[0] : g == 1 ? src[1] : g == 2 ? src[2] : src[3];
+        const bool active = mine >= 0;
+        const int sl = active ? mine : int(lane);
+        const u32 ei = __shfl(i, sl), ej = __shfl(j, sl), en = __shfl(n, sl), enb = __shfl(nb, sl);
+        const u32 eld = __shfl(ld, sl);
+        const float ed = __shfl(d, sl);
+        const float* em = shfl_ptr(mat, sl);
+        const float* eb = shfl_ptr(bmin, sl);
+        const float* ri = em + u64(ei) * eld;
+        const float* rj = em + u64(ej) * eld;
+        const float* bi = eb + u64(ei) * enb;
+        const float* bj = eb + u64(ej) * enb;
+        const u32 w_hi = active ? en - TOPK : 0;           // phase 1 covered w >= w_hi
+        int top = active ? int((w_hi - 1) / BLK) : -1;
+        int wf = -1;
+        float diw = 0.0f, djw = 0.0f;
+        while (__ballot(active && wf < 0 && top >= 0)) {
+            const bool live = active && wf < 0 && top >= 0;
+            u64 cmask = 0;   // bit 16k + t: block top - t - 16k may hold a cofacet
+#pragma unroll
+            for (u32 k = 0; k < 4; ++k) {
+                const int b = top - int(gl) - 16 * int(k);
+                bool cand = false;
+                if (live && b >= 0) cand = fmax_ref(bi[b], bj[b]) <= ed;
+                cmask |= ((__ballot(cand) >> (16 * g)) & 0xffffull) << (16 * k);
+            }
+            while (__ballot(live && cmask != 0 && wf < 0)) {
+                const bool work = live && cmask != 0 && wf < 0;
+                const int blk = work ? top - __builtin_ctzll(cmask) : 0;   // highest block first
+                if (work) cmask &= cmask - 1;
+                u64 hit = 0;     // bit 16k + t: w = 64 blk + 63 - t - 16k passes
+                float av[4], cv[4];
+#pragma unroll
+                for (u32 k = 0; k < 4; ++k) {
+                    const int w = blk * int(BLK) + int(BLK - 1) - int(gl) - 16 * int(k);
+                    const bool in = work && u32(w) < w_hi;
+                    float a = INFINITY, c = INFINITY;
+                    if (in) {
+                        a = ri[w];
+                        c = rj[w];
+                    }
+                    av[k] = a;
+                    cv[k] = c;
+                    const bool ok = in && u32(w) != ei && u32(w) != ej && fmax_ref(a, c) <= ed;
+                    hit |= ((__ballot(ok) >> (16 * g)) & 0xffffull) << (16 * k);
+                }
+                const int q = hit ? __builtin_ctzll(hit) : 0;     // lowest bit = largest w
+                const int kq = q >> 4, from = int(16 * g) + (q & 15);
+                float a = 0.0f, c = 0.0f;
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                    const float ak = __shfl(av[k], from), ck = __shfl(cv[k], from);
+                    if (k == kq) {
+                        a = ak;
+                        c = ck;
+                    }
+                }
+                if (work && hit) {
+                    wf = blk * int(BLK) + int(BLK - 1) - (q & 15) - 16 * kq;
+                    diw = a;
+                    djw = c;
+                }
+            }
+            if (live && wf < 0) top -= 64;
+        }
+        const bool r = wf >= 0 && facet_rule(u32(wf), ei, ej, ed, diw, djw);
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            const bool rt = __shfl(int(r), 16 * t) != 0;
+            if (int(lane) == src[t]) ap = rt;
+        }
     }
     return ap;
 }
