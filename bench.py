@@ -396,7 +396,7 @@ def main():
     ap.add_argument("--no-batched", action="store_true", help="skip the 512-cloud batched leg")
     ap.add_argument("--no-diff", action="store_true", help="skip the differentiable leg")
     ap.add_argument("--c4-cap", type=int, default=1500, help="largest n_i in the diagrams/sec sample")
-    ap.add_argument("--c4-sample", type=int, default=128, help="clouds in the diagrams/sec sample")
+    ap.add_argument("--c4-sample", type=int, default=192, help="clouds in the diagrams/sec sample")
     ap.add_argument("--c5-n", type=int, default=2048, help="points of the differentiable circle")
     ap.add_argument("--c5-batch", type=int, default=16, help="clouds in the differentiable batch sample")
     ap.add_argument("--c5-batch-nmax", type=int, default=1024, help="largest n_i in that sample")

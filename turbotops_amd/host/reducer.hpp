@@ -599,8 +599,13 @@ class Cohomology {
         S head{};
     };
 
+    // row v's other vertices by (distance, id), and those distances alongside
+    // (read in order by the merge instead of looked up through the ids)
     const std::vector<u32>& sorted_row(u32 v) {
-        if (srow_.empty()) srow_.resize(n_);
+        if (srow_.empty()) {
+            srow_.resize(n_);
+            srowd_.resize(n_);
+        }
         std::vector<u32>& r = srow_[v];
         if (r.empty() && n_ > 1) {
             r.reserve(n_ - 1);
@@ -608,6 +613,9 @@ class Cohomology {
                 if (w != v) r.push_back(w);
             const f32* row = cx_.mat + u64(v) * n_;
             std::sort(r.begin(), r.end(), [row](u32 a, u32 b) { return row[a] < row[b] || (row[a] == row[b] && a < b); });
+            std::vector<f32>& rd = srowd_[v];
+            rd.resize(r.size());
+            for (size_t t = 0; t < r.size(); ++t) rd[t] = row[r[t]];
         }
         return r;
     }
@@ -622,11 +630,13 @@ class Cohomology {
     bool next_b(EdgeStream& s, u32& w_out, f32& k_out) {
         const std::vector<u32>& ri = sorted_row(s.i);
         const std::vector<u32>& rj = sorted_row(s.j);
+        const f32* sdi = srowd_[s.i].data();
+        const f32* sdj = srowd_[s.j].data();
         const f32* di = cx_.mat + u64(s.i) * n_;
         const f32* dj = cx_.mat + u64(s.j) * n_;
         for (;;) {
-            const f32 ai = s.pi < ri.size() ? di[ri[s.pi]] : INFINITY;
-            const f32 aj = s.pj < rj.size() ? dj[rj[s.pj]] : INFINITY;
+            const f32 ai = s.pi < ri.size() ? sdi[s.pi] : INFINITY;
+            const f32 aj = s.pj < rj.size() ? sdj[s.pj] : INFINITY;
             if (s.pi >= ri.size() && s.pj >= rj.size()) return false;
             if (s.pi < ri.size() && (s.pj >= rj.size() || ai <= aj)) {
                 const u32 w = ri[s.pi++];
@@ -650,9 +660,9 @@ class Cohomology {
 
     // first position in a sorted row with distance > d
     u32 first_above(u32 v, f32 d) {
-        const std::vector<u32>& r = sorted_row(v);
-        const f32* row = cx_.mat + u64(v) * n_;
-        return u32(std::upper_bound(r.begin(), r.end(), d, [row](f32 x, u32 w) { return x < row[w]; }) - r.begin());
+        sorted_row(v);
+        const std::vector<f32>& rd = srowd_[v];
+        return u32(std::upper_bound(rd.begin(), rd.end(), d) - rd.begin());
     }
 
     bool advance(EdgeStream& s) {
@@ -883,6 +893,7 @@ class Cohomology {
     }
 
     std::vector<std::vector<u32>> srow_;
+    std::vector<std::vector<f32>> srowd_;
     std::vector<EdgeStream> pool_;
     std::vector<u32> free_, group_;
     RadixHeap lheap_;
