@@ -65,6 +65,20 @@ def test_knn_and_knn_graph(gpu, restatement, metric, n, d, kind, k):
     assert g.valid()
 
 
+@pytest.mark.parametrize("metric,n,d,kind,k", [("l2", 8192, 3, "grid", 15), ("l1", 8300, 8, "unif", 64),
+                                                ("linf", 9000, 2, "grid", 9), ("l2", 12300, 4, "gauss", 1)])
+def test_knn_candidate_chunks(gpu, restatement, metric, n, d, kind, k):
+    """Clouds large enough that the candidates are split into chunks whose
+    partial lists are merged; grid clouds put equal distances in different
+    chunks, where the smaller index must win."""
+    from turbotops_amd import knn_lists_exact
+    x = _cloud(n, d, n + 3 * d, kind)
+    print(f"case knn chunks metric={metric} n={n} d={d} k={k}", flush=True)
+    ids = knn_lists_exact(x, metric, k)
+    want = restatement.knn(x, metric, k)
+    assert ids.shape == want.shape and np.array_equal(ids, want)
+
+
 @pytest.mark.parametrize("metric", ["lp:1.5", "lp:3"])
 def test_knn_lp(gpu, restatement, metric):
     """Lp distances may differ from glibc powf by an ulp; lists must agree
