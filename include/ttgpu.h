@@ -211,8 +211,21 @@ tt_status tt_batch_wait(tt_batch* b, uint32_t cloud);
  * of dimension 0 need no others. */
 tt_status tt_batch_cloud(tt_batch* b, uint32_t cloud, const float** mat, float* threshold,
                          uint64_t* m, const uint64_t** idx, const float** d);
+/* Which edges a run hands over. comb = 0 (the default): the kept edges that
+ * are not in an apparent pair, in the coboundary engine's order (d ascending,
+ * index descending; rips.hpp:116-118), read with tt_batch_cloud. comb = 1:
+ * every kept edge in the boundary engine's order (d ascending, combinatorial
+ * index ascending; redzed.hpp:544-613), read with tt_batch_cloud_edges.
+ * Takes effect at the next run (the recorded sequence is rebuilt). No
+ * reference counterpart: the reference has no batch API. */
+tt_status tt_batch_set_order(tt_batch* b, int comb);
+/* Cloud results in the comb = 1 order: the full matrix, its threshold, and m
+ * edges as 2 m vertex ids (lo < hi per edge) with their diameters. */
+tt_status tt_batch_cloud_edges(tt_batch* b, uint32_t cloud, const float** mat, float* threshold,
+                               uint64_t* m, const uint32_t** verts, const float** d);
 /* Device time of the last run, the autotuned rows per wave (8/16/32), and
- * the kept / non-apparent edge counts. */
+ * the kept / non-apparent edge counts (both the kept count in the comb = 1
+ * order). */
 tt_status tt_batch_info(tt_batch* b, float* device_ms, int* rows_per_wave, uint64_t* edges_total,
                         uint64_t* edges_kept);
 /* The distance stage of the batch alone, re-run on the points of the last

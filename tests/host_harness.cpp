@@ -1,6 +1,8 @@
 // SPDX-License-Identifier: MIT
 // host_harness.cpp — TEST INFRASTRUCTURE ONLY: a C ABI around the host
-// reducer (turbotops_amd/host/reducer.hpp) so the CPU test suite can check it
+// reducers (turbotops_amd/host/reducer.hpp, the cohomology engine, and
+// turbotops_amd/host/zerodiff.hpp, the boundary engine for H1) so the CPU
+// test suite can check them
 // against the reference with inputs the oracle computes (full matrix, sorted
 // edges, apparent flags) — no GPU involved. Not part of the product.
 #include <cstdlib>
@@ -8,6 +10,7 @@
 #include <string>
 
 #include "reducer.hpp"
+#include "zerodiff.hpp"
 
 using namespace tops::cob;
 
@@ -57,6 +60,23 @@ int run(const Cx& cx, uint32_t maxdim, const uint64_t* idx, const float* d, cons
 
 extern "C" {
 const char* hh_last_error() { return g_err.c_str(); }
+
+// H0 and H1 by the boundary engine: full n x n matrix and m edges as 2 m
+// vertex ids (lo < hi) with diameters, in (diameter, comb index) order.
+int hh_boundary(const float* mat, uint32_t n, const uint32_t* verts, const float* d, uint64_t m,
+                uint64_t* nbars, uint32_t** dims, float** b, float** e) {
+    try {
+        Diagram out;
+        out.ensure(1);
+        tops::zd::BoundaryH1 eng(n, mat, n);
+        if (n) eng.run(m, verts, d, out);
+        flatten(out, nbars, dims, b, e);
+        return 0;
+    } catch (const std::exception& ex) {
+        g_err = ex.what();
+        return 1;
+    }
+}
 void hh_free(void* p) { std::free(p); }
 
 int hh_dense(const float* mat, uint32_t n, float threshold, uint32_t maxdim, const uint64_t* idx, const float* d,

@@ -207,6 +207,19 @@ def test_selftest_clouds_vs_reference(gpu, reference, kind, n, dim, maxdim, metr
     assert bars_close(dr, ref_dgms(rr, maxdim), 1e-5) if lp else canon(dr) == ref_canon(rr)
 
 
+def test_engine_choice(gpu):
+    """maxdim 1 of a dense cloud runs the boundary engine (the reference's
+    auto choice, include/turbotops/turbotops.hpp:144-152), maxdim 2 and
+    mode-free sparse input the cohomology engine; H0 and H1 agree."""
+    from turbotops_amd import persistence
+    x = np.random.default_rng(4).normal(size=(500, 3)).astype(np.float32)
+    d1, i1 = persistence(x, maxdim=1)
+    d2, i2 = persistence(x, maxdim=2)
+    assert i1["method"] == "dense-redzed" and i1["engine"] == "boundary"
+    assert i2["method"] == "dense" and i2["engine"] == "coboundary"
+    assert canon(d1) == canon(d2[:2])
+
+
 def test_torus_c1_bar_for_bar(gpu, reference):
     """The plain CPU reference configuration: n = 1000 points on a torus in
     R^3 (R = 1, r = 0.3, angles U[0, 2pi)^2, noise N(0, 0.01^2), seed 1),

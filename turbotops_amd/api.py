@@ -69,13 +69,15 @@ class BatchPersistence:
     end (dense full Rips at each cloud's enclosing radius, H0 and H1), with
     host reductions in parallel. Create once per set of shapes; call with the
     clouds (a list of (n_i, d_i) arrays) for a list of (dgms, info), each equal
-    to persistence(cloud, maxdim=1, metric=metric)."""
+    to persistence(cloud, maxdim=1, metric=metric). mode "auto" and
+    "boundary" reduce with the boundary engine, "coboundary" with the
+    cohomology engine (same diagrams)."""
 
-    def __init__(self, shapes, metric="l2", maxdim=1):
+    def __init__(self, shapes, metric="l2", maxdim=1, mode="auto"):
         ext = _ext()
         self.shapes = [(int(n), int(d)) for n, d in shapes]
         self._r = ext.BatchRunner([n for n, _ in self.shapes], [d for _, d in self.shapes], str(metric),
-                                  int(maxdim))
+                                  int(maxdim), str(mode))
 
     def __call__(self, clouds, use_graph=True, threads=0, pairs=False):
         """With pairs=True each entry is (dgms, birth_edges, death_edges, info)

@@ -30,6 +30,9 @@
 //   counted radix sort of those keys over the device-side count: one sort
 //                  orders every cloud by `precedes` and keeps clouds contiguous
 //   batch_unpack   (idx, d) per non-apparent edge
+// In the other edge order (tt_batch_set_order), for the boundary engine, the
+// compaction keys hold the combinatorial index itself, the apparent stages
+// are skipped, and every kept edge is sorted and unpacked to its vertex pair.
 // Host copies then run per cloud, each followed by an event, so the host
 // reducer starts on a cloud as soon as its data has landed.
 #include <algorithm>
@@ -141,7 +144,7 @@ __global__ void batch_segments(BatchShape b, const u32* __restrict__ off, u32* _
 
 __global__ __launch_bounds__(256) void batch_compact(BatchShape b, const float* __restrict__ mats,
                                                      const float* __restrict__ thr,
-                                                     const u32* __restrict__ off, int ib,
+                                                     const u32* __restrict__ off, int ib, bool comb,
                                                      u64* __restrict__ keys) {
     const u32 lane = threadIdx.x & 63;
     const u64 gw = (blockIdx.x * 256ull + threadIdx.x) >> 6;
@@ -163,7 +166,8 @@ __global__ __launch_bounds__(256) void batch_compact(BatchShape b, const float* 
             const u64 bal = __ballot(keep);
             if (keep) {
                 const u32 rk = __builtin_amdgcn_mbcnt_hi(u32(bal >> 32), __builtin_amdgcn_mbcnt_lo(u32(bal), 0u));
-                keys[o + rk] = cloud_bits | (u64(__float_as_uint(d)) << ib) | (M - 1 - (rowbase + j));
+                const u64 idx = rowbase + j;
+                keys[o + rk] = cloud_bits | (u64(__float_as_uint(d)) << ib) | (comb ? idx : M - 1 - idx);
             }
             o += __popcll(bal);
         }
@@ -254,6 +258,25 @@ __global__ void batch_unpack(const u64* __restrict__ keys, const u32* __restrict
     }
 }
 
+// Sorted keys of every kept edge in (diameter, combinatorial index) order ->
+// (lo | hi << 32, d): the vertex pair reads as two u32 (lo, hi) in memory.
+__global__ void batch_unpack_comb(const u64* __restrict__ keys, const u32* __restrict__ total, int ib,
+                                  u64* __restrict__ out_pair, float* __restrict__ out_d) {
+    const u64 m = *total;
+    const u64 imask = (u64(1) << ib) - 1;
+    for (u64 e = blockIdx.x * 256ull + threadIdx.x; e < m; e += u64(gridDim.x) * 256) {
+        const u64 key = keys[e];
+        u32 hi, lo;
+        decode_edge_b(key & imask, hi, lo);
+        out_pair[e] = u64(lo) | (u64(hi) << 32);
+        out_d[e] = __uint_as_float(u32(key >> ib) & 0x7fffffffu);
+    }
+}
+
+__global__ void copy_u32(const u32* __restrict__ src, u32 count, u32* __restrict__ dst) {
+    for (u32 t = threadIdx.x; t < count; t += blockDim.x) dst[t] = src[t];
+}
+
 // per cloud: start of its non-apparent edges in the sorted list = number of
 // non-apparent edges before its first compacted edge (compaction and sort are
 // both cloud-major)
@@ -303,6 +326,7 @@ struct BatchState {
     MetricArgs ma{};
     int rpw = 16, ib = 1, cb = 0;
     bool tuned = false;
+    bool comb = false;   // hand over every kept edge in (diameter, comb index) order
     std::vector<u32> n, dim;
     std::vector<u64> pt_off, mat_off, dmat_off, row_base, tile_base, pair_base, bmin_off;
     u64 total_pts = 0, total_mat = 0, total_rows = 0, total_tiles = 0, capacity = 0, h_cap = 0;
@@ -355,11 +379,20 @@ struct BatchState {
         batch_count<<<gw, 256, 0, s>>>(shape, d_mats, d_thr, d_cnt);
         exclusive_scan_u32_buf(d_cnt, d_cnt, total_rows + 1, d_total, d_stmp, s);
         batch_segments<<<1, 256, 0, s>>>(shape, d_cnt, d_seg);
-        batch_compact<<<gw, 256, 0, s>>>(shape, d_mats, d_thr, d_cnt, ib, d_keys);
+        batch_compact<<<gw, 256, 0, s>>>(shape, d_mats, d_thr, d_cnt, ib, comb, d_keys);
         TTG_LAUNCH_CHECK();
+        const u32 ge = u32(std::min<u64>(std::max<u64>(capacity / 4, 1), 65536));
+        if (comb) {
+            // every kept edge, sorted by (cloud, diameter, comb index); the
+            // cloud segments are those of the compaction
+            radix_sort_u64_counted(d_keys, capacity, d_total, 0, cb + 31 + ib, d_alt, d_hist, d_stmp, s);
+            batch_unpack_comb<<<ge, 256, 0, s>>>(d_keys, d_total, ib, d_out_idx, d_out_d);
+            copy_u32<<<1, 256, 0, s>>>(d_seg, nclouds + 1, d_kseg);
+            TTG_LAUNCH_CHECK();
+            return;
+        }
         // the apparent test does not depend on the order, so it runs on the
         // compacted (row-major) edges and only the rest are sorted
-        const u32 ge = u32(std::min<u64>(std::max<u64>(capacity / 4, 1), 65536));
         fill_device_u32(d_keep, capacity, 0u, s);   // slots past the count stay 0
         batch_block_min<<<gw, 256, 0, s>>>(shape, d_mats, d_bmin);
         batch_apparent<<<ge, 256, 0, s>>>(shape, d_mats, d_bmin, d_keys, d_total, ib, d_keep);
@@ -570,7 +603,7 @@ void batch_run(BatchState& b, const float* pts, int use_graph) {
     u32 kept = 0;
     TTG_HIP(hipMemcpyAsync(b.kseg.data(), b.d_kseg, (b.nclouds + 1) * 4, hipMemcpyDeviceToHost, s));
     TTG_HIP(hipMemcpyAsync(b.thr.data(), b.d_thr, b.nclouds * 4, hipMemcpyDeviceToHost, s));
-    TTG_HIP(hipMemcpyAsync(&kept, b.d_kept, 4, hipMemcpyDeviceToHost, s));
+    TTG_HIP(hipMemcpyAsync(&kept, b.comb ? b.d_total : b.d_kept, 4, hipMemcpyDeviceToHost, s));
     TTG_HIP(hipStreamSynchronize(s));
     TTG_HIP(hipEventElapsedTime(&b.last_ms, b.t0, b.t1));
     b.kseg[b.nclouds] = kept;
@@ -639,11 +672,41 @@ tt_status tt_batch_cloud(tt_batch* b, uint32_t cloud, const float** mat, float* 
     return guarded([&] {
         if (!b || cloud >= b->st.nclouds) fail(TT_EINVAL, "tt_batch_cloud: bad cloud");
         BatchState& s = b->st;
+        if (s.comb) fail(TT_EINVAL, "tt_batch_cloud: the batch hands over every edge; use tt_batch_cloud_edges");
         if (mat) *mat = s.h_mats + s.mat_off[cloud];
         if (threshold) *threshold = s.thr[cloud];
         const u64 e0 = s.kseg[cloud], e1 = s.kseg[cloud + 1];
         if (m) *m = e1 - e0;
         if (idx) *idx = s.h_idx + e0;
+        if (d) *d = s.h_d + e0;
+    });
+}
+
+tt_status tt_batch_set_order(tt_batch* b, int comb) {
+    return guarded([&] {
+        if (!b) fail(TT_EINVAL, "tt_batch_set_order: null handle");
+        BatchState& s = b->st;
+        if (s.comb == (comb != 0)) return;
+        s.comb = comb != 0;
+        // the recorded sequence belongs to the other order
+        if (s.exec) TTG_HIP(hipGraphExecDestroy(s.exec));
+        if (s.graph) TTG_HIP(hipGraphDestroy(s.graph));
+        s.exec = nullptr;
+        s.graph = nullptr;
+    });
+}
+
+tt_status tt_batch_cloud_edges(tt_batch* b, uint32_t cloud, const float** mat, float* threshold, uint64_t* m,
+                               const uint32_t** verts, const float** d) {
+    return guarded([&] {
+        if (!b || cloud >= b->st.nclouds) fail(TT_EINVAL, "tt_batch_cloud_edges: bad cloud");
+        BatchState& s = b->st;
+        if (!s.comb) fail(TT_EINVAL, "tt_batch_cloud_edges: the batch hands over the coboundary order");
+        if (mat) *mat = s.h_mats + s.mat_off[cloud];
+        if (threshold) *threshold = s.thr[cloud];
+        const u64 e0 = s.kseg[cloud], e1 = s.kseg[cloud + 1];
+        if (m) *m = e1 - e0;
+        if (verts) *verts = reinterpret_cast<const uint32_t*>(s.h_idx + e0);
         if (d) *d = s.h_d + e0;
     });
 }

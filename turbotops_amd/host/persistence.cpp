@@ -13,6 +13,7 @@
 #include <thread>
 
 #include "turbotops_gpu.hpp"
+#include "zerodiff.hpp"
 
 namespace tops {
 
@@ -51,10 +52,18 @@ bool boundary_fits(u32 n, u32 maxdim) {
     }
 }
 
-// The reference's select_mode (turbotops.hpp:121-162) keeps its checks and
-// errors; both of its engines give the same diagram, and this build reduces
-// with the cohomology engine in every mode.
+// The boundary engine (zerodiff.hpp) serves what the reference's auto mode
+// sends to RedZeD on dense input (turbotops.hpp:144-152): H1 (maxdim = 1) of
+// a full Rips complex; --mode boundary with maxdim 1 as well. Everything else
+// runs on the cohomology engine, which gives the same diagram.
+bool boundary_engine(Mode mode, u32 n, u32 maxdim, bool dense_full) {
+    return dense_full && maxdim == 1 && mode != Mode::Coboundary && boundary_fits(n, maxdim);
+}
+
+// The reference's select_mode (turbotops.hpp:121-162): its checks and errors,
+// and the engine that serves each case here.
 std::string select_mode(Mode mode, u32 n, u32 maxdim, double mean_degree) {
+    const bool dense = std::isinf(mean_degree);
     if (mode == Mode::Coboundary)
         return "coboundary (--mode coboundary): cohomology reducer, maxdim <= 30, 64/128-bit indices, any input";
     if (mode == Mode::Boundary) {
@@ -66,11 +75,27 @@ std::string select_mode(Mode mode, u32 n, u32 maxdim, double mean_degree) {
                 "boundary mode (RedZeD) needs every simplex index in 64 bits (C(n, maxdim + 2) < 2^64, "
                 "implied n < 2^31); this problem does not fit — use the default coboundary mode instead");
         if (maxdim + 2 > 32) throw std::invalid_argument("redzed: maxdim must be <= 30");
+        if (boundary_engine(mode, n, maxdim, dense))
+            return "boundary (--mode boundary): reduction to zero differentials on the GPU's edge order";
         return "boundary (--mode boundary): served by the cohomology reducer (identical diagram)";
     }
     if (maxdim == 0) return "auto: H0 only, minimum spanning tree (no reduction engine)";
-    (void)mean_degree;
+    if (boundary_engine(mode, n, maxdim, dense))
+        return "auto: boundary (reduction to zero differentials) for H1 of dense full-Rips input";
     return "auto: cohomology reducer on GPU front-end outputs";
+}
+
+// H0 and H1 by the boundary engine: full matrix (row stride n) and every
+// edge up to the threshold in (diameter, combinatorial index) order.
+void reduce_boundary(u32 n, const f32* mat, u64 m, const u32* verts, const f32* d, Barcode& out, Stats& st) {
+    const auto t0 = std::chrono::steady_clock::now();
+    out.ensure(1);
+    if (n) {
+        zd::BoundaryH1 eng(n, mat, n);
+        eng.run(m, verts, d, out);
+    }
+    st.edges = m;
+    st.seconds_reduce = since(t0);
 }
 
 // Runs the reducer from dimension 1 after dim0 on the given edges.
@@ -313,6 +338,27 @@ Result persistence(const f32* points, u32 n, u32 dim, const Options& o) {
                                                                  : "rips: maxdim must be <= 30");
         int layout = 0;
         check(tt_dense_layout(n, budget, &layout));
+        if (layout == TT_LAYOUT_FULL && boundary_engine(o.mode, n, o.maxdim, true)) {
+            std::vector<f32> mat(u64(n) * n);
+            u64 m = 0;
+            int32_t* pv = nullptr;
+            f32* pd = nullptr;
+            f32 thr = INFINITY;
+            const auto tf = std::chrono::steady_clock::now();
+            check(tt_build_edges(points, n, dim, cloud.metric.kind, cloud.metric.p, INFINITY, TT_LAYOUT_FULL,
+                                 mat.data(), &m, &pv, &pd, &thr));
+            st.seconds_front = since(tf);
+            const std::vector<int32_t> verts = take(pv, 2 * m);
+            const std::vector<f32> d = take(pd, m);
+            reduce_boundary(n, mat.data(), m, reinterpret_cast<const u32*>(verts.data()), d.data(), r.barcode, st);
+            st.engine = "boundary";
+            st.method = "dense-redzed";
+            st.threshold = thr;
+            r.barcode.ensure(o.maxdim);
+            st.seconds_total = since(t0);
+            st.peak_rss_mb = peak_rss_mb();
+            return r;
+        }
         std::vector<f32> mat(rips::dense_bytes(n, layout == TT_LAYOUT_FULL) / sizeof(f32));
         u64 m = 0;
         u64* pidx = nullptr;
@@ -366,12 +412,16 @@ Result persistence(const f32* points, u32 n, u32 dim, const Options& o) {
 
 // ---- batches ---------------------------------------------------------------
 BatchRunner::BatchRunner(const std::vector<u32>& n, const std::vector<u32>& dim, const std::string& metric,
-                         u32 maxdim)
+                         u32 maxdim, Mode mode)
     : n_(n), dim_(dim), maxdim_(maxdim) {
     if (n.size() != dim.size()) throw std::invalid_argument("persistence_batch: n and dim differ in length");
     if (maxdim != 1) throw std::invalid_argument("persistence_batch: maxdim must be 1 (H0 and H1)");
     const Metric m = Metric::parse(metric);
+    u32 nmax = 0;
+    for (u32 v : n) nmax = std::max(nmax, v);
+    boundary_ = boundary_engine(mode, nmax, maxdim, true);
     check(tt_batch_create(u32(n.size()), n.data(), dim.data(), m.kind, m.p, INFINITY, &h_));
+    if (boundary_) check(tt_batch_set_order(h_, 1));
 }
 
 BatchRunner::~BatchRunner() {
@@ -481,19 +531,26 @@ std::vector<Result> BatchRunner::run(const f32* pts, bool use_graph, u32 threads
                 const f32* mat = nullptr;
                 f32 thr = 0;
                 u64 m = 0;
-                const u64* idx = nullptr;
                 const f32* d = nullptr;
-                check(tt_batch_cloud(h_, c, &mat, &thr, &m, &idx, &d));
-                cob::DenseView cx;
-                cx.n = n_[c];
-                cx.threshold = thr;
-                cx.full = true;
-                cx.mat = mat;
                 Result& r = out[c];
-                reduce_all(cx, maxdim_, idx, d, nullptr, m, r.barcode, r.stats);
+                if (boundary_) {
+                    const u32* verts = nullptr;
+                    check(tt_batch_cloud_edges(h_, c, &mat, &thr, &m, &verts, &d));
+                    reduce_boundary(n_[c], mat, m, verts, d, r.barcode, r.stats);
+                    r.stats.engine = "boundary";
+                } else {
+                    const u64* idx = nullptr;
+                    check(tt_batch_cloud(h_, c, &mat, &thr, &m, &idx, &d));
+                    cob::DenseView cx;
+                    cx.n = n_[c];
+                    cx.threshold = thr;
+                    cx.full = true;
+                    cx.mat = mat;
+                    reduce_all(cx, maxdim_, idx, d, nullptr, m, r.barcode, r.stats);
+                    r.stats.engine = "coboundary";
+                }
                 r.barcode.ensure(maxdim_);
                 r.stats.method = "dense-batch";
-                r.stats.engine = "coboundary";
                 r.stats.threshold = thr;
                 if (edges) {
                     const u64 nn = n_[c];

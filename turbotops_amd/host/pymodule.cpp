@@ -47,6 +47,7 @@ py::dict stats_dict(const Stats& s) {
     d["peak_rss_mb"] = s.peak_rss_mb;
     d["seconds_front"] = s.seconds_front;
     d["seconds_reduce"] = s.seconds_reduce;
+    d["engine"] = s.engine;
     return d;
 }
 
@@ -216,8 +217,12 @@ PYBIND11_MODULE(_turbotops, m) {
     m.def("persistence_pairs", &persistence_pairs, py::arg("x"), py::arg("maxdim"), py::arg("threshold"),
           py::arg("metric"), py::arg("graph"), py::arg("k"), py::arg("threads"), py::arg("memory_budget"));
     py::class_<BatchRunner>(m, "BatchRunner")
-        .def(py::init<const std::vector<u32>&, const std::vector<u32>&, const std::string&, u32>(), py::arg("n"),
-             py::arg("dim"), py::arg("metric") = "l2", py::arg("maxdim") = 1)
+        .def(py::init([](const std::vector<u32>& n, const std::vector<u32>& dim, const std::string& metric,
+                         u32 maxdim, const std::string& mode) {
+                 return std::make_unique<BatchRunner>(n, dim, metric, maxdim, parse_mode(mode));
+             }),
+             py::arg("n"), py::arg("dim"), py::arg("metric") = "l2", py::arg("maxdim") = 1,
+             py::arg("mode") = "auto")
         .def("front", &batch_front, py::arg("clouds"), py::arg("use_graph") = true)
         .def("run", &batch_run, py::arg("clouds"), py::arg("use_graph") = true, py::arg("threads") = 0,
              py::arg("pairs") = false)

@@ -173,8 +173,11 @@ CriticalEdges critical_edges(const Barcode& bc, u32 n, const std::function<f32(u
 // cloud alone.
 class BatchRunner {
   public:
+    // mode: Auto and Boundary reduce with the boundary engine (every kept
+    // edge in (diameter, comb index) order), Coboundary with the cohomology
+    // engine (the non-apparent edges in its order).
     BatchRunner(const std::vector<u32>& n, const std::vector<u32>& dim, const std::string& metric,
-                u32 maxdim = 1);
+                u32 maxdim = 1, Mode mode = Mode::Auto);
     ~BatchRunner();
     BatchRunner(const BatchRunner&) = delete;
     BatchRunner& operator=(const BatchRunner&) = delete;
@@ -196,6 +199,7 @@ class BatchRunner {
     tt_batch* h_ = nullptr;
     std::vector<u32> n_, dim_;
     u32 maxdim_ = 1;
+    bool boundary_ = true;
     float device_ms_ = 0;
     int rpw_ = 0;
     u64 edges_total_ = 0, edges_kept_ = 0;

@@ -1,0 +1,444 @@
+// SPDX-License-Identifier: MIT
+// zerodiff.hpp — H0 and H1 of a full Rips filtration in the boundary
+// direction: reduction to zero differentials with active enumeration.
+//
+// The method is C. Kapulkin and N. Kershaw's RedZeD ("reduction to zero
+// differentials"), the engine the reference picks by default for H1 of dense
+// full-Rips input (include/turbotops/turbotops.hpp:121-162, select_mode; the
+// engine is include/turbotops/redzed.hpp:1001-1411). The reference picks it
+// only for maxdim = 1, and this file covers exactly that case, written from
+// the method: its inputs are what the GPU front end already produces (the
+// full distance matrix and every edge up to the threshold in filtration
+// order), so the host does no distance or sort work.
+//
+// Filtration. Edges in (diameter, combinatorial index) order; edge rank r is
+// its 1-based position. A triangle enters right after its youngest edge, the
+// last of its three edges in that order. Any order that refines diameter
+// gives the same diagram.
+//
+// Classes. H1 is held in a basis of generators, each named by the rank of the
+// edge that created it. A tree edge (it joins two components; union-find)
+// has class 0. A non-tree edge x starts as its own generator. For each
+// edge z with a nonzero class (an "active" edge) S[z] is the sorted set of
+// generators in that class, and G[g] is the sorted set of active edges whose
+// class contains generator g.
+//
+// Triangles. t = {a, b, w} entering with x = (a, b) has boundary class
+// beta = S[x] + S[(a, w)] + S[(b, w)] (mod 2). If beta != 0 its youngest
+// generator j dies at diam(t): every class that contains j gets beta added
+// (S[z] ^= beta for z in G[j]), which removes j from the basis, and the bar
+// (diam(j), diam(t)) is recorded when it is not empty.
+//
+// Active enumeration. Only triangles with an active face besides x can carry
+// a class other than x's own; they are found from the active edges at x's
+// endpoints. Every other triangle entering with x has boundary S[x], so the
+// first of them kills x's class and the rest are then zero: one witness is
+// enough, and it is found by a word-parallel AND of the endpoints' rows of
+// the radius graph (all edges of diameter <= diam(x)).
+//
+// Ready test: is x the youngest face of {a, b, w} (a < b, D = diam(x), and
+// d(w, a), d(w, b) <= D)? A face of diameter D that comes after x in
+// combinatorial order would be. Face {w, b} comes after {a, b} iff w > a, and
+// face {a, w} iff w > b. So: w < a: ready; a < w < b: ready iff d(w, b) < D;
+// w > b: ready iff d(w, a) < D and d(w, b) < D.
+//
+// Output: bars with their birth and death simplices (edge and triangle
+// combinatorial indices), as the cohomology engine records them, so the
+// critical edges of every bar are read the same way.
+#pragma once
+
+#include <algorithm>
+#include <bit>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <utility>
+#include <vector>
+
+#include "reducer.hpp"
+
+namespace tops::zd {
+
+using cob::f32;
+using cob::u128;
+using cob::u32;
+using cob::u64;
+
+// u32 key (> 0) -> u32 value (> 0). Open addressing with linear probing over
+// one array of (key << 32 | value) words, multiplicative hashing, load at most
+// 1/2, and deletion by backward shift (no tombstones).
+class RankMap {
+public:
+    RankMap() { rehash(16); }
+    u32 get(u32 k) const {
+        for (size_t i = home(k);; i = (i + 1) & mask_) {
+            const u64 s = slots_[i];
+            if (u32(s >> 32) == k) return u32(s);
+            if (!s) return 0;
+        }
+    }
+    void put(u32 k, u32 v) {
+        if (2 * (used_ + 1) > slots_.size()) rehash(2 * slots_.size());
+        size_t i = home(k);
+        while (slots_[i] && u32(slots_[i] >> 32) != k) i = (i + 1) & mask_;
+        if (!slots_[i]) ++used_;
+        slots_[i] = (u64(k) << 32) | v;
+    }
+    void erase(u32 k) {
+        size_t i = home(k);
+        while (true) {
+            if (!slots_[i]) return;
+            if (u32(slots_[i] >> 32) == k) break;
+            i = (i + 1) & mask_;
+        }
+        // shift later members of the probe run back over the hole
+        size_t hole = i;
+        for (size_t j = (i + 1) & mask_; slots_[j]; j = (j + 1) & mask_) {
+            const size_t h = home(u32(slots_[j] >> 32));
+            // j's home lies cyclically in (hole, j]: it stays
+            const bool stays = hole <= j ? (hole < h && h <= j) : (hole < h || h <= j);
+            if (!stays) {
+                slots_[hole] = slots_[j];
+                hole = j;
+            }
+        }
+        slots_[hole] = 0;
+        --used_;
+    }
+    template <class F>
+    void for_each(F&& f) const {
+        for (u64 s : slots_)
+            if (s) f(u32(s >> 32), u32(s));
+    }
+
+private:
+    std::vector<u64> slots_;
+    size_t mask_ = 0, used_ = 0;
+    int shift_ = 0;
+    size_t home(u32 k) const { return size_t((u64(k) * 0x9E3779B97F4A7C15ull) >> shift_); }
+    void rehash(size_t cap) {
+        std::vector<u64> old;
+        old.swap(slots_);
+        slots_.assign(cap, 0);
+        mask_ = cap - 1;
+        shift_ = 64 - std::countr_zero(cap);
+        used_ = 0;
+        for (u64 s : old)
+            if (s) put(u32(s >> 32), u32(s));
+    }
+};
+
+// out = a ^ b for sorted sets
+inline void xor_sorted(const std::vector<u32>& a, const std::vector<u32>& b, std::vector<u32>& out) {
+    out.clear();
+    size_t i = 0, j = 0;
+    while (i < a.size() && j < b.size()) {
+        if (a[i] < b[j]) out.push_back(a[i++]);
+        else if (b[j] < a[i]) out.push_back(b[j++]);
+        else {
+            ++i;
+            ++j;
+        }
+    }
+    out.insert(out.end(), a.begin() + i, a.end());
+    out.insert(out.end(), b.begin() + j, b.end());
+}
+
+// x ^= v (sorted); a single toggle when v holds one element
+inline void xor_into(std::vector<u32>& x, const std::vector<u32>& v, std::vector<u32>& tmp) {
+    if (v.size() == 1) {
+        const auto it = std::lower_bound(x.begin(), x.end(), v[0]);
+        if (it != x.end() && *it == v[0]) x.erase(it);
+        else x.insert(it, v[0]);
+        return;
+    }
+    xor_sorted(x, v, tmp);
+    x.swap(tmp);
+}
+
+inline u64 comb2(u32 lo, u32 hi) { return u64(hi) * (hi - 1) / 2 + lo; }
+
+inline u128 comb3(u32 a, u32 b, u32 c) {
+    u32 v[3] = {a, b, c};
+    std::sort(v, v + 3);
+    return u128(v[2]) * (v[2] - 1) * (v[2] - 2) / 6 + u128(v[1]) * (v[1] - 1) / 2 + v[0];
+}
+
+class BoundaryH1 {
+public:
+    // mat: full n x n distance matrix, row stride ld.
+    BoundaryH1(u32 n, const f32* mat, u64 ld) : n_(n), W_((u64(n) + 63) / 64), mat_(mat), ld_(ld) {}
+
+    // verts: 2 m vertex ids (lo < hi per edge), diam: m diameters, in
+    // (diameter, combinatorial index) order, every edge up to the threshold.
+    void run(u64 m, const u32* verts, const f32* diam, cob::Diagram& out) {
+        if (m >= 0xffffffffull) throw std::length_error("boundary engine: more than 2^32 - 2 edges");
+        out.ensure(1);
+        m_ = m;
+        verts_ = verts;
+        diam_ = diam;
+        adj_.assign(n_ * W_, 0);
+        seen_.assign(W_, 0);
+        seed_a_.assign(n_, 0);
+        seed_b_.assign(n_, 0);
+        inc_.assign(n_, {});
+        parent_.resize(n_);
+        for (u32 v = 0; v < n_; ++v) parent_[v] = v;
+        size_.assign(n_, 1);
+        cand_.clear();
+        G_ = RankMap();
+        cls_.clear();
+        head_.clear();
+        owner_.clear();
+        cls_free_.clear();
+        gen_.clear();
+        gen_free_.clear();
+        rptr_ = resolved_ = 0;
+        for (u64 e = 0; e < m; ++e) {
+            const u32 a = verts[2 * e], b = verts[2 * e + 1];
+            const f32 d = diam[e];
+            u32 ra = find(a), rb = find(b);
+            if (ra != rb) {
+                if (size_[ra] < size_[rb]) std::swap(ra, rb);
+                parent_[rb] = ra;
+                size_[ra] += size_[rb];
+                if (d > 0.0f) out.add(0, 0.0f, d, u128(a), u128(comb2(a, b)));
+            } else {
+                cycle_edge(u32(e + 1), a, b, d, out);
+            }
+        }
+        for (u32 v = 0; v < n_; ++v)
+            if (find(v) == v) out.add(0, 0.0f, INFINITY, u128(v), u128(~u64(0)));
+        std::vector<u32> alive;
+        G_.for_each([&](u32 g, u32) { alive.push_back(g); });
+        std::sort(alive.begin(), alive.end());
+        for (u32 g : alive)
+            out.add(1, diam_[g - 1], INFINITY, u128(comb2(verts_[2 * (g - 1)], verts_[2 * (g - 1) + 1])),
+                    u128(~u64(0)));
+    }
+
+    u64 triangles_resolved() const { return resolved_; }
+
+private:
+    static constexpr u32 MULTI = ~0u;   // head_ of a class with two or more generators
+
+    u32 n_;
+    u64 W_;
+    const f32* mat_;
+    u64 ld_;
+    u64 m_ = 0, rptr_ = 0, resolved_ = 0;
+    const u32* verts_ = nullptr;
+    const f32* diam_ = nullptr;
+    std::vector<u64> adj_;                               // radius graph rows, W_ words each
+    std::vector<u64> seen_;                              // candidate vertices of the current edge
+    std::vector<u32> seed_a_, seed_b_;                   // 1 + class slot of edge (a, w) / (b, w)
+    std::vector<u32> cand_;
+    std::vector<std::vector<std::pair<u32, u32>>> inc_;  // per vertex: (other end, class slot)
+    std::vector<u32> parent_, size_;
+    // classes of active edges, by slot: generator set, its single element (0
+    // when empty, MULTI when larger), and the edge's rank
+    std::vector<std::vector<u32>> cls_;
+    std::vector<u32> head_, owner_, cls_free_;
+    // G sets (class slots holding a generator), by slot; G_: generator -> 1 + slot
+    std::vector<std::vector<u32>> gen_;
+    std::vector<u32> gen_free_;
+    RankMap G_;
+    std::vector<u32> beta_, touched_, tmp_;
+    u32 cur_ = 0;                                        // the edge being processed (not linked yet)
+
+    f32 dist(u32 u, u32 v) const { return mat_[u64(u) * ld_ + v]; }
+    u32 find(u32 x) {
+        while (parent_[x] != x) {
+            parent_[x] = parent_[parent_[x]];
+            x = parent_[x];
+        }
+        return x;
+    }
+    bool adjacent(u32 u, u32 w) const { return (adj_[u * W_ + (w >> 6)] >> (w & 63)) & 1; }
+    void advance(f32 D) {
+        while (rptr_ < m_ && diam_[rptr_] <= D) {
+            const u32 u = verts_[2 * rptr_], v = verts_[2 * rptr_ + 1];
+            adj_[u * W_ + (v >> 6)] |= u64(1) << (v & 63);
+            adj_[v * W_ + (u >> 6)] |= u64(1) << (u & 63);
+            ++rptr_;
+        }
+    }
+    bool ready(u32 a, u32 b, f32 D, u32 w) const {
+        if (w < a) return true;
+        if (w < b) return dist(w, b) < D;
+        return dist(w, a) < D && dist(w, b) < D;
+    }
+    // a ready triangle {a, b, w} with no active face besides (a, b), or -1
+    int64_t leftover(u32 a, u32 b, f32 D) const {
+        const u64* ra = &adj_[a * W_];
+        const u64* rb = &adj_[b * W_];
+        for (u64 t = 0; t < W_; ++t) {
+            u64 c = ra[t] & rb[t] & ~seen_[t];
+            while (c) {
+                const u32 w = u32(t * 64 + std::countr_zero(c));
+                c &= c - 1;
+                if (ready(a, b, D, w)) return w;
+            }
+        }
+        return -1;
+    }
+
+    static u32 take(std::vector<std::vector<u32>>& pool, std::vector<u32>& freel) {
+        if (freel.empty()) {
+            pool.emplace_back();
+            return u32(pool.size() - 1);
+        }
+        const u32 s = freel.back();
+        freel.pop_back();
+        return s;
+    }
+    void set_head(u32 c) {
+        const auto& v = cls_[c];
+        head_[c] = v.empty() ? 0 : v.size() == 1 ? v[0] : MULTI;
+    }
+    // class {x} and G[x] = {its slot} for a new non-tree edge x; returns the slot
+    u32 create(u32 x) {
+        const u32 c = take(cls_, cls_free_);
+        if (head_.size() < cls_.size()) {
+            head_.resize(cls_.size());
+            owner_.resize(cls_.size());
+        }
+        cls_[c].assign(1, x);
+        head_[c] = x;
+        owner_[c] = x;
+        const u32 g = take(gen_, gen_free_);
+        gen_[g].assign(1, c);
+        G_.put(x, g + 1);
+        return c;
+    }
+    void unlink_from(u32 v, u32 c) {
+        auto& l = inc_[v];
+        for (size_t k = 0; k < l.size(); ++k)
+            if (l[k].second == c) {
+                l[k] = l.back();
+                l.pop_back();
+                return;
+            }
+    }
+    void drop_class(u32 c) {
+        const u32 z = owner_[c];
+        cls_[c].clear();
+        head_[c] = 0;
+        cls_free_.push_back(c);
+        if (z != cur_) {
+            unlink_from(verts_[2 * (z - 1)], c);
+            unlink_from(verts_[2 * (z - 1) + 1], c);
+        }
+    }
+    // beta_ (nonzero boundary class of a triangle of diameter D): its
+    // youngest generator dies
+    void resolve(f32 D, u128 tri, cob::Diagram& out) {
+        ++resolved_;
+        const u32 j = beta_.back();
+        const u32 gj = G_.get(j) - 1;
+        touched_ = gen_[gj];
+        for (u32 c : touched_) {
+            xor_into(cls_[c], beta_, tmp_);
+            set_head(c);
+            if (cls_[c].empty()) drop_class(c);
+        }
+        for (size_t t = 0; t + 1 < beta_.size(); ++t) {
+            const u32 y = beta_[t];
+            const u32 gy = G_.get(y) - 1;
+            xor_into(gen_[gy], touched_, tmp_);
+            if (gen_[gy].empty()) {
+                gen_free_.push_back(gy);
+                G_.erase(y);
+            }
+        }
+        gen_[gj].clear();
+        gen_free_.push_back(gj);
+        G_.erase(j);
+        const f32 birth = diam_[j - 1];
+        if (birth != D) out.add(1, birth, D, u128(comb2(verts_[2 * (j - 1)], verts_[2 * (j - 1) + 1])), tri);
+    }
+    // beta_ = sum of the classes in slots p, q, r (1 + slot, 0 = none);
+    // false when it is zero. Classes of one generator, the common case, are
+    // summed from head_ alone.
+    bool boundary_class(u32 p, u32 q, u32 r) {
+        const u32 hp = p ? head_[p - 1] : 0, hq = q ? head_[q - 1] : 0, hr = r ? head_[r - 1] : 0;
+        if (hp != MULTI && hq != MULTI && hr != MULTI) {
+            u32 v[3] = {hp, hq, hr};
+            if (v[0] > v[1]) std::swap(v[0], v[1]);
+            if (v[1] > v[2]) std::swap(v[1], v[2]);
+            if (v[0] > v[1]) std::swap(v[0], v[1]);
+            beta_.clear();
+            // zeros are absent classes; equal neighbours cancel in pairs
+            for (int k = 0; k < 3; ++k) {
+                if (!v[k]) continue;
+                if (k + 1 < 3 && v[k + 1] == v[k]) {
+                    ++k;
+                    continue;
+                }
+                beta_.push_back(v[k]);
+            }
+            return !beta_.empty();
+        }
+        static const std::vector<u32> none;
+        xor_sorted(p ? cls_[p - 1] : none, q ? cls_[q - 1] : none, tmp_);
+        xor_sorted(tmp_, r ? cls_[r - 1] : none, beta_);
+        return !beta_.empty();
+    }
+
+    void cycle_edge(u32 r, u32 a, u32 b, f32 D, cob::Diagram& out) {
+        advance(D);
+        for (u32 w : cand_) {
+            seen_[w >> 6] = 0;
+            seed_a_[w] = seed_b_[w] = 0;
+        }
+        cand_.clear();
+        for (const auto& [w, c] : inc_[b])
+            if (w != a && adjacent(a, w)) {
+                if (!((seen_[w >> 6] >> (w & 63)) & 1)) {
+                    seen_[w >> 6] |= u64(1) << (w & 63);
+                    cand_.push_back(w);
+                }
+                seed_b_[w] = c + 1;
+            }
+        for (const auto& [w, c] : inc_[a])
+            if (w != b && adjacent(b, w)) {
+                if (!((seen_[w >> 6] >> (w & 63)) & 1)) {
+                    seen_[w >> 6] |= u64(1) << (w & 63);
+                    cand_.push_back(w);
+                }
+                seed_a_[w] = c + 1;
+            }
+        cur_ = r;
+        if (cand_.empty()) {
+            // every triangle entering with x has boundary {x}: the first
+            // ready one kills x's class as it is born
+            if (leftover(a, b, D) < 0) {
+                const u32 c = create(r);
+                inc_[a].push_back({b, c});
+                inc_[b].push_back({a, c});
+            }
+            cur_ = 0;
+            return;
+        }
+        const u32 xc = create(r);
+        for (u32 w : cand_) {
+            if (!boundary_class(head_[xc] ? xc + 1 : 0, seed_a_[w], seed_b_[w])) continue;
+            if (!ready(a, b, D, w)) continue;
+            resolve(D, comb3(a, b, w), out);
+        }
+        if (head_[xc]) {
+            const int64_t w = leftover(a, b, D);
+            if (w >= 0) {
+                beta_ = cls_[xc];
+                resolve(D, comb3(a, b, u32(w)), out);
+            }
+        }
+        if (head_[xc]) {
+            inc_[a].push_back({b, xc});
+            inc_[b].push_back({a, xc});
+        }
+        cur_ = 0;
+    }
+};
+
+}  // namespace tops::zd
