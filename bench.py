@@ -16,11 +16,12 @@ Prints ONE JSON line. Besides the contract keys it carries:
   cpu_baseline    the reference's own make_dense + enclosing_radius (compiled
                   from its sources into oracle/_ref, SIMD build, all allotted
                   host cores) on the same cloud
-  batched         the 512-cloud batched front end (device time, GB/s) and
-                  diagrams/sec of the batched form against the plain per-cloud
-                  path on a bounded sample of the same clouds
+  batched         the 512-cloud batched front end (device time, GB/s), whole
+                  runs of all 512 clouds (diagrams/sec), and the batched form
+                  against the plain per-cloud path on the clouds with n <= 2000
   differentiable  forward + backward of a top-8 H1 persistence loss on a
-                  noisy circle (n = 2048) and the backward kernel alone
+                  noisy circle (n = 2048), the backward kernel alone, and the
+                  64-circle batch
 """
 import argparse
 import json
@@ -247,17 +248,28 @@ def c4_clouds():
     return uneven_clouds(count=512, nmin=200, nmax=4000, dims=(2, 3, 8), seed=6)
 
 
+def _front_timing(bp, clouds):
+    bp.front_end(clouds)                 # autotune + graph capture
+    runs = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        bp.front_end(clouds)
+        runs.append((time.perf_counter() - t0, bp.device_ms))
+    return min(runs, key=lambda r: r[1])
+
+
 def run_batched(args, metric="l2"):
     """The batched form on the 512-cloud set.
 
     front end: all 512 clouds through the recorded launch sequence (distances,
-    enclosing radii, edge sort, apparent scan, copies to host memory), timed
-    with the library's HIP events on its own stream; algorithmic bytes are
-    the distance stage's sum(4 n_i^2 + 4 n_i d_i).
+    enclosing radii, the edge sort, copies to host memory), timed with the
+    library's HIP events on its own stream, in the boundary engine's edge
+    order (the default: every kept edge by (diameter, comb index)) and in the
+    cohomology engine's (apparent test, then the non-apparent edges sorted);
+    algorithmic bytes of the distance stage are sum(4 n_i^2 + 4 n_i d_i).
     diagrams/sec: whole runs (front end + host reductions) of the batched
-    form against the plain per-cloud path (persistence() on each cloud in
-    turn) on a bounded sample of the same set: its first `sample` clouds with
-    n_i <= `cap` (the host reduction of the largest circles takes minutes).
+    form on all 512 clouds, and against the plain per-cloud path
+    (persistence() on each cloud in turn) on the clouds with n_i <= `cap`.
     """
     from turbotops_amd import BatchPersistence, persistence
     clouds = c4_clouds()
@@ -265,24 +277,29 @@ def run_batched(args, metric="l2"):
     t0 = time.perf_counter()
     bp = BatchPersistence([c.shape for c in clouds], metric=metric)
     setup_s = time.perf_counter() - t0
-    bp.front_end(clouds)                 # autotune + graph capture
-    runs = []
-    for _ in range(3):
-        t0 = time.perf_counter()
-        bp.front_end(clouds)
-        runs.append((time.perf_counter() - t0, bp.device_ms))
-    wall, dev_ms = min(runs, key=lambda r: r[1])
+    wall, dev_ms = _front_timing(bp, clouds)
     dist_ms = bp.time_distance_stage()
-    kept, non_apparent = bp.edges
+    kept, _ = bp.edges
     out = {"workload": f"512 clouds, n_i ~ U{{200..4000}}, d_i in {{2,3,8}}, circles/spheres sigma 0.02, seed 6, "
                        f"{metric}, maxdim 1",
            "front_end_device_ms": round(dev_ms, 3), "front_end_wall_ms": round(wall * 1e3, 3),
+           "front_end_edge_order": "boundary engine: every kept edge by (diameter, comb index)",
            "distance_stage_bytes": algo, "distance_stage_ms": round(dist_ms, 3),
            "distance_stage_gbs": round(algo / (dist_ms * 1e-3) / 1e9, 1),
            "distance_stage_frac": round(algo / (dist_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
-           "edges_kept": int(kept), "edges_not_apparent": int(non_apparent),
-           "rows_per_wave": bp.rows_per_wave, "setup_s": round(setup_s, 3)}
+           "edges_kept": int(kept), "rows_per_wave": bp.rows_per_wave, "setup_s": round(setup_s, 3)}
+    if args.c4_full:
+        bp(clouds)                       # first whole run: host buffers
+        t0 = time.perf_counter()
+        bp(clouds)
+        tf = time.perf_counter() - t0
+        out["diagrams_per_sec_all_512"] = {"batched": round(len(clouds) / tf, 3), "whole_run_s": round(tf, 3)}
     del bp
+    bc = BatchPersistence([c.shape for c in clouds], metric=metric, mode="coboundary")
+    wall_c, dev_c = _front_timing(bc, clouds)
+    out["front_end_coboundary_order"] = {"device_ms": round(dev_c, 3), "wall_ms": round(wall_c * 1e3, 3),
+                                         "edges_not_apparent": int(bc.edges[1])}
+    del bc
     cap, count = args.c4_cap, args.c4_sample
     if count <= 0:
         return out
@@ -300,7 +317,7 @@ def run_batched(args, metric="l2"):
     out["diagrams_per_sec"] = {
         "batched": round(len(sample) / t_batch, 3), "plain": round(len(sample) / t_plain, 3),
         "ratio": round(t_plain / t_batch, 2), "identical_diagrams": bool(same),
-        "sample": f"first {len(sample)} clouds with n_i <= {cap} of the 512 (sum n_i = "
+        "sample": f"the {len(sample)} clouds with n_i <= {cap} of the 512 (sum n_i = "
                   f"{sum(c.shape[0] for c in sample)}); whole runs: GPU front end + host reductions "
                   f"(batched: all host threads; plain: persistence() per cloud)"}
     return out
@@ -350,7 +367,8 @@ def run_differentiable(args):
     if args.c5_batch > 0:
         from turbotops_amd import DiagramBatch
         from turbotops_amd.synthetic import noisy_circles
-        clouds = noisy_circles(count=args.c5_batch, nmin=512, nmax=args.c5_batch_nmax, seed=9)
+        clouds = noisy_circles(count=64, nmin=512, nmax=4096, seed=9)
+        clouds = [c for c in clouds if c.shape[0] <= args.c5_batch_nmax][:args.c5_batch]
         db = DiagramBatch([c.shape for c in clouds])
 
         def batch_step():
@@ -366,8 +384,9 @@ def run_differentiable(args):
         tb = time.perf_counter() - t0
         out["batch"] = {"clouds": len(clouds), "step_s": round(tb, 3),
                         "diagrams_per_sec": round(len(clouds) / tb, 3),
-                        "sample": f"{len(clouds)} noisy circles with n_i ~ U{{512..{args.c5_batch_nmax}}} (seed 9) "
-                                  f"of the 64-cloud U{{512..4096}} batch; forward + backward per step"}
+                        "sample": f"{len(clouds)} of the 64 noisy circles n_i ~ U{{512..4096}} (seed 9), those with "
+                                  f"n_i <= {args.c5_batch_nmax}; forward + backward of the summed top-8 H1 losses "
+                                  f"per step"}
     return out
 
 
@@ -395,11 +414,13 @@ def main():
     ap.add_argument("--copy-probe", action="store_true", help="run only the copy kernel (PMC scaling)")
     ap.add_argument("--no-batched", action="store_true", help="skip the 512-cloud batched leg")
     ap.add_argument("--no-diff", action="store_true", help="skip the differentiable leg")
-    ap.add_argument("--c4-cap", type=int, default=1500, help="largest n_i in the diagrams/sec sample")
-    ap.add_argument("--c4-sample", type=int, default=192, help="clouds in the diagrams/sec sample")
+    ap.add_argument("--c4-cap", type=int, default=2000, help="largest n_i in the batched vs plain sample")
+    ap.add_argument("--c4-sample", type=int, default=512, help="most clouds in the batched vs plain sample")
+    ap.add_argument("--no-c4-full", dest="c4_full", action="store_false",
+                    help="skip the whole batched run of all 512 clouds")
     ap.add_argument("--c5-n", type=int, default=2048, help="points of the differentiable circle")
-    ap.add_argument("--c5-batch", type=int, default=16, help="clouds in the differentiable batch sample")
-    ap.add_argument("--c5-batch-nmax", type=int, default=1024, help="largest n_i in that sample")
+    ap.add_argument("--c5-batch", type=int, default=64, help="clouds in the differentiable batch")
+    ap.add_argument("--c5-batch-nmax", type=int, default=4096, help="largest n_i taken into that batch")
     args = ap.parse_args()
     if args.gpus != 1:
         raise SystemExit("bench.py measures one GPU: this build runs the front end on a single MI355X "
