@@ -16,6 +16,7 @@ Prints ONE JSON line. Besides the contract keys it carries:
   cpu_baseline    the reference's own make_dense + enclosing_radius (compiled
                   from its sources into oracle/_ref, SIMD build, all allotted
                   host cores) on the same cloud
+  whole_run       persistence() of the same cloud at maxdim 1, end to end
   batched         the 512-cloud batched front end (device time, GB/s), whole
                   runs of all 512 clouds (diagrams/sec), and the batched form
                   against the plain per-cloud path on the clouds with n <= 2000
@@ -148,6 +149,16 @@ def run_dense(args):
     r = torch.empty(1)
     r.copy_(radius.cpu())
     out["config"]["enclosing_radius"] = float(r[0])
+    if args.whole_run:
+        # the same cloud end to end at maxdim 1 (the configuration's own
+        # persistence call): GPU front end + host copies + host reduction
+        from turbotops_amd import persistence
+        t0 = time.perf_counter()
+        dgms, info = persistence(x, maxdim=1, metric=metric)
+        out["whole_run"] = {"seconds": round(time.perf_counter() - t0, 3), "engine": info["engine"],
+                            "seconds_front": round(info["seconds_front"], 3),
+                            "seconds_reduce": round(info["seconds_reduce"], 3), "edges": int(info["edges"]),
+                            "h1_bars": int(len(dgms[1]))}
     return out
 
 
@@ -411,6 +422,8 @@ def main():
                     help="c2a | c2b-l1 | c2b-linf | c2b-lp (distance matrix); c2a-edges (dimension-0 edges); "
                          "c3a-knn | c3b-radius | c2c-mst (graph stages)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-whole-run", dest="whole_run", action="store_false",
+                    help="skip the end-to-end persistence() run of the single cloud")
     ap.add_argument("--copy-probe", action="store_true", help="run only the copy kernel (PMC scaling)")
     ap.add_argument("--no-batched", action="store_true", help="skip the 512-cloud batched leg")
     ap.add_argument("--no-diff", action="store_true", help="skip the differentiable leg")

@@ -181,6 +181,18 @@ tt_status tt_build_edges(const float* pts, uint32_t n, uint32_t dim, int kind, f
                          float threshold, int layout, float* mat, uint64_t* m, int32_t** verts,
                          float** diam, float* threshold_out);
 
+/* tt_build_edges, and per edge the boundary engine's ready-witness flag in
+ * bit 31 of its larger vertex id (verts[2e + 1]): whether a triangle {lo, hi,
+ * w} with max(d(lo, w), d(hi, w)) <= d exists whose youngest face in this
+ * order is the edge (w < lo; or lo < w < hi and d(hi, w) < d; or w > hi and
+ * both < d). No reference counterpart: the flag lets the host engine
+ * (turbotops_amd/host/zerodiff.hpp) skip its witness search for an edge with
+ * no active face around it, the search RedZeD runs per edge
+ * (redzed.hpp:1237-1241, has_leftover_coface). */
+tt_status tt_build_edges_witness(const float* pts, uint32_t n, uint32_t dim, int kind, float p,
+                                 float threshold, int layout, float* mat, uint64_t* m,
+                                 int32_t** verts, float** diam, float* threshold_out);
+
 /* build_edges over a graph (redzed.hpp:544-613 with SparseGeom, :831-979;
  * redzed::compute_graph :1452-1461): edges u < v with w <= threshold of a CSR
  * graph (rows sorted by neighbour id), in the same order and layout. */
@@ -211,21 +223,26 @@ tt_status tt_batch_wait(tt_batch* b, uint32_t cloud);
  * of dimension 0 need no others. */
 tt_status tt_batch_cloud(tt_batch* b, uint32_t cloud, const float** mat, float* threshold,
                          uint64_t* m, const uint64_t** idx, const float** d);
-/* Which edges a run hands over. comb = 0 (the default): the kept edges that
+/* Which edges a run hands over. order = 0 (the default): the kept edges that
  * are not in an apparent pair, in the coboundary engine's order (d ascending,
- * index descending; rips.hpp:116-118), read with tt_batch_cloud. comb = 1:
+ * index descending; rips.hpp:116-118), read with tt_batch_cloud. order = 1:
  * every kept edge in the boundary engine's order (d ascending, combinatorial
  * index ascending; redzed.hpp:544-613), read with tt_batch_cloud_edges.
+ * order = 2: as 1, plus each edge's ready-witness flag (as
+ * tt_build_edges_witness) where the batch's keys have a bit to spare.
  * Takes effect at the next run (the recorded sequence is rebuilt). No
  * reference counterpart: the reference has no batch API. */
-tt_status tt_batch_set_order(tt_batch* b, int comb);
-/* Cloud results in the comb = 1 order: the full matrix, its threshold, and m
- * edges as 2 m vertex ids (lo < hi per edge) with their diameters. */
+tt_status tt_batch_set_order(tt_batch* b, int order);
+/* Cloud results in orders 1 and 2: the full matrix, its threshold, and m
+ * edges as 2 m vertex ids (lo < hi per edge) with their diameters. When
+ * *witness_bit is 1, bit 31 of each hi id is the edge's ready-witness flag,
+ * as tt_build_edges_witness hands it over (batches whose keys have no bit to
+ * spare: 0, and the ids are plain). */
 tt_status tt_batch_cloud_edges(tt_batch* b, uint32_t cloud, const float** mat, float* threshold,
-                               uint64_t* m, const uint32_t** verts, const float** d);
+                               uint64_t* m, const uint32_t** verts, const float** d, int* witness_bit);
 /* Device time of the last run, the autotuned rows per wave (8/16/32), and
- * the kept / non-apparent edge counts (both the kept count in the comb = 1
- * order). */
+ * the kept / non-apparent edge counts (both the kept count in orders 1 and
+ * 2). */
 tt_status tt_batch_info(tt_batch* b, float* device_ms, int* rows_per_wave, uint64_t* edges_total,
                         uint64_t* edges_kept);
 /* The distance stage of the batch alone, re-run on the points of the last

@@ -63,13 +63,14 @@ const char* hh_last_error() { return g_err.c_str(); }
 
 // H0 and H1 by the boundary engine: full n x n matrix and m edges as 2 m
 // vertex ids (lo < hi) with diameters, in (diameter, comb index) order.
-int hh_boundary(const float* mat, uint32_t n, const uint32_t* verts, const float* d, uint64_t m,
+// witness_bit: bit 31 of each hi id carries the ready-witness flag.
+int hh_boundary(const float* mat, uint32_t n, const uint32_t* verts, const float* d, uint64_t m, int witness_bit,
                 uint64_t* nbars, uint32_t** dims, float** b, float** e) {
     try {
         Diagram out;
         out.ensure(1);
         tops::zd::BoundaryH1 eng(n, mat, n);
-        if (n) eng.run(m, verts, d, out);
+        if (n) eng.run(m, verts, d, out, witness_bit != 0);
         flatten(out, nbars, dims, b, e);
         return 0;
     } catch (const std::exception& ex) {

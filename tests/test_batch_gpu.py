@@ -40,20 +40,23 @@ def test_batch_equals_single(gpu, metric):
 
 def test_batch_engines_agree(gpu):
     """The boundary engine (every edge in (diameter, comb index) order, the
-    default) and the cohomology engine (the non-apparent edges in its own
-    order) on the same batch: the same diagrams, with and without the graph,
-    and switching back and forth between handles."""
+    default; also with the GPU's ready-witness flags) and the cohomology
+    engine (the non-apparent edges in its own order) on the same batch: the
+    same diagrams, with and without the graph, and switching back and forth
+    between handles."""
     from turbotops_amd import BatchPersistence
     clouds = _clouds(21, count=20, nmax=500)
     print(f"case batch engines clouds={len(clouds)}", flush=True)
     bd = BatchPersistence([c.shape for c in clouds])
+    bw = BatchPersistence([c.shape for c in clouds], witness_flags=True)
     bc = BatchPersistence([c.shape for c in clouds], mode="coboundary")
     for use_graph in (True, False, True):
         rd = bd(clouds, use_graph=use_graph)
+        rw = bw(clouds, use_graph=use_graph)
         rc = bc(clouds, use_graph=use_graph)
-        for (dd, id_), (dc, ic) in zip(rd, rc):
-            assert canon(dd) == canon(dc)
-            assert id_["engine"] == "boundary" and ic["engine"] == "coboundary"
+        for (dd, id_), (dw, iw), (dc, ic) in zip(rd, rw, rc):
+            assert canon(dd) == canon(dc) == canon(dw)
+            assert id_["engine"] == iw["engine"] == "boundary" and ic["engine"] == "coboundary"
 
 
 def test_graph_replays_with_new_points(gpu):

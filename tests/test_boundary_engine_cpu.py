@@ -9,6 +9,10 @@ the restatement's full matrix and every edge up to the enclosing radius in
   edge and triangle, columns added left to right over Z/2) on clouds small
   enough to list all triangles;
 * the compiled reference's diagrams, in both of its modes, bar for bar.
+
+The engine's second input form, the ready-witness flag per edge (what
+tt_build_edges_witness computes on the GPU), is checked the same ways, with
+the flags from their definition; a planted mistake in the flags is seen.
 """
 import ctypes as C
 import itertools
@@ -31,7 +35,7 @@ def harness():
     if not os.path.exists(path):
         pytest.skip("tests/libhost_harness.so not built (make -C tests)")
     L = C.CDLL(path)
-    L.hh_boundary.argtypes = [_f32p, C.c_uint32, _u32p, _f32p, C.c_uint64, _u64p, C.POINTER(_u32p),
+    L.hh_boundary.argtypes = [_f32p, C.c_uint32, _u32p, _f32p, C.c_uint64, C.c_int, _u64p, C.POINTER(_u32p),
                               C.POINTER(_f32p), C.POINTER(_f32p)]
     L.hh_last_error.restype = C.c_char_p
     L.hh_free.argtypes = [C.c_void_p]
@@ -49,13 +53,38 @@ def comb_edges(mat, thr):
     return np.stack([lo[o], hi[o]], 1).astype(np.uint32), np.ascontiguousarray(d[o], np.float32)
 
 
-def engine_bars(L, mat, verts, d):
+def witness_flags(mat, verts, d):
+    """The ready-witness flag of every edge (lo, hi) of diameter D, from its
+    definition: some w with d(lo, w) <= D and d(hi, w) <= D for which the
+    edge is the youngest face of {lo, hi, w} in (diameter, comb index) order:
+    w < lo; or lo < w < hi and d(hi, w) < D; or w > hi and both < D."""
+    n = mat.shape[0]
+    w = np.arange(n)[None, :]
+    out = np.zeros(len(d), bool)
+    for s0 in range(0, len(d), 1024):
+        lo = verts[s0:s0 + 1024, 0].astype(np.int64)[:, None]
+        hi = verts[s0:s0 + 1024, 1].astype(np.int64)[:, None]
+        D = d[s0:s0 + 1024, None]
+        ra, rb = mat[lo[:, 0]], mat[hi[:, 0]]
+        both = (ra <= D) & (rb <= D) & (w != lo) & (w != hi)
+        ready = (w < lo) | ((w > lo) & (w < hi) & (rb < D)) | ((w > hi) & (ra < D) & (rb < D))
+        out[s0:s0 + 1024] = (both & ready).any(1)
+    return out
+
+
+def with_flags(verts, flags):
+    v = np.array(verts, np.uint32)
+    v[:, 1] |= np.where(flags, np.uint32(0x80000000), np.uint32(0))
+    return v
+
+
+def engine_bars(L, mat, verts, d, witness=False):
     mat = np.ascontiguousarray(mat, np.float32)
     verts = np.ascontiguousarray(verts, np.uint32)
     n = mat.shape[0]
     nb, dims, b, e = C.c_uint64(), _u32p(), _f32p(), _f32p()
     rc = L.hh_boundary(mat.ctypes.data_as(_f32p), n, verts.ctypes.data_as(_u32p), d.ctypes.data_as(_f32p),
-                       d.size, C.byref(nb), C.byref(dims), C.byref(b), C.byref(e))
+                       d.size, int(witness), C.byref(nb), C.byref(dims), C.byref(b), C.byref(e))
     assert rc == 0, L.hh_last_error()
     k = nb.value
     out = []
@@ -70,10 +99,12 @@ def canon(dims, b, e):
     return [f"{d} {x:.9g} inf" if math.isinf(y) else f"{d} {x:.9g} {y:.9g}" for d, x, y in bars]
 
 
-def engine_diagram(L, restatement, x, metric):
+def engine_diagram(L, restatement, x, metric, witness=False):
     mat = restatement.make_dense(x, metric)
     thr = restatement.enclosing_radius(mat, x.shape[0]) if x.shape[0] else 0.0
     v, d = comb_edges(mat, thr)
+    if witness:
+        return engine_bars(L, mat, with_flags(v, witness_flags(mat, v, d)), d, True), mat, thr
     return engine_bars(L, mat, v, d), mat, thr
 
 
@@ -177,6 +208,48 @@ def test_vs_reference(harness, restatement, reference, kind, n, dim, metric):
     for mode in ("boundary", "coboundary"):
         res, _ = reference.persistence(x, maxdim=1, metric=metric, mode=mode)
         assert got == canon(*res), mode
+    # with the ready-witness flags: no radius graph until witness searches
+    # become common (the circles switch it on, the rest never need it)
+    fbars, _, _ = engine_diagram(harness, restatement, x, metric, witness=True)
+    assert canon(*fbars) == got
+
+
+@pytest.mark.parametrize("kind,n,dim,metric", [("circle", 14, 2, "l2"), ("grid", 16, 2, "l1"),
+                                                ("grid", 14, 3, "linf"), ("gauss", 12, 3, "l2")])
+def test_witness_flags_brute_force(harness, restatement, kind, n, dim, metric):
+    x = _cloud(kind, n, dim, 5 * n + dim)
+    bars, mat, thr = engine_diagram(harness, restatement, x, metric, witness=True)
+    assert canon(*bars) == canon(*brute_force(mat, thr))
+
+
+def test_witness_flags_against_a_direct_loop(restatement):
+    """The vectorised flag definition used above, against a plain loop over
+    every w (the rule written out once more, case by case)."""
+    x = _cloud("grid", 30, 2, 4)
+    mat = restatement.make_dense(x, "l1")
+    v, d = comb_edges(mat, restatement.enclosing_radius(mat, 30))
+    got = witness_flags(mat, v, d)
+    for (lo, hi), D, f in zip(v.tolist(), d.tolist(), got.tolist()):
+        want = False
+        for w in range(30):
+            if w in (lo, hi) or mat[lo, w] > D or mat[hi, w] > D:
+                continue
+            if w < lo or (lo < w < hi and mat[hi, w] < D) or (w > hi and mat[lo, w] < D and mat[hi, w] < D):
+                want = True
+        assert f == want
+
+
+def test_wrong_flags_change_the_diagram(harness, restatement):
+    """A planted mistake in the flags (every flag cleared: no edge seems to
+    have a witness) leaves classes alive that a triangle kills, and the
+    brute force sees it."""
+    x = _cloud("gauss", 14, 2, 8)
+    mat = restatement.make_dense(x, "l2")
+    thr = restatement.enclosing_radius(mat, 14)
+    v, d = comb_edges(mat, thr)
+    want = canon(*brute_force(mat, thr))
+    assert canon(*engine_bars(harness, mat, with_flags(v, witness_flags(mat, v, d)), d, True)) == want
+    assert canon(*engine_bars(harness, mat, with_flags(v, np.zeros(len(d), bool)), d, True)) != want
 
 
 def test_ties_in_one_diameter(harness, restatement, reference):

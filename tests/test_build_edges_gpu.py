@@ -89,3 +89,37 @@ def test_build_edges_csr_vs_reference(gpu, reference, case):
     rv, rd = reference.build_edges_csr(n, rp, col, w, thr)
     assert np.array_equal(v, rv) and np.array_equal(d.view(np.uint32), rd.view(np.uint32))
     assert v.shape[0] > 0 and (v[:, 0] < v[:, 1]).all()
+
+
+@pytest.mark.parametrize("kind,n,dim,metric", [("gauss", 700, 3, "l2"), ("grid", 7, 3, "linf"), ("grid", 8, 3, "l1"),
+                                                ("uniform", 300, 8, "l2"), ("gauss", 2, 3, "l2")])
+def test_witness_flags_vs_definition(gpu, kind, n, dim, metric):
+    """tt_build_edges_witness: the same list as tt_build_edges, and every
+    edge's ready-witness flag equal to its definition (numpy, every w). The
+    grids put hundreds of equal edges in one diameter; n = 700 reaches the
+    block-filtered second phase of the search."""
+    from test_boundary_engine_cpu import witness_flags
+    from turbotops_amd import build_edges
+    rng = np.random.default_rng(3 * n + dim)
+    x = {"gauss": lambda: rng.normal(size=(n, dim)), "uniform": lambda: rng.uniform(size=(n, dim)),
+         "grid": lambda: _grid(n, dim)}[kind]().astype(np.float32)
+    print(f"case witness {kind} n={x.shape[0]} dim={dim} {metric}", flush=True)
+    v, d, t, mat = build_edges(x, metric, with_matrix=True)
+    vw, dw, flags, tw = build_edges(x, metric, witness=True)
+    assert np.array_equal(v, vw) and np.array_equal(d.view(np.uint32), dw.view(np.uint32)) and t == tw
+    want = witness_flags(mat, v.astype(np.uint32), d)
+    assert np.array_equal(flags, want)
+
+
+def test_witness_flags_sampled_large(gpu):
+    """n = 4200 (more than one pass over the block minima): 30000 sampled
+    edges against the definition."""
+    from test_boundary_engine_cpu import witness_flags
+    from turbotops_amd import build_edges
+    x = np.random.default_rng(17).normal(size=(4200, 3)).astype(np.float32)
+    print("case witness sampled n=4200", flush=True)
+    v, d, flags, t, mat = build_edges(x, "l2", witness=True, with_matrix=True)
+    pick = np.random.default_rng(1).choice(len(d), 30000, replace=False)
+    want = witness_flags(mat, v[pick].astype(np.uint32), d[pick])
+    assert np.array_equal(flags[pick], want)
+    assert flags.mean() > 0.9

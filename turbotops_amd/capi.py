@@ -53,6 +53,8 @@ _SIGS = {
     "tt_sparse_edges": ([C.c_uint32, _vp, _vp, _vp, C.c_float, _u64p, _vp, _vp, _vp], None),
     "tt_build_edges": ([_vp, C.c_uint32, C.c_uint32, C.c_int, C.c_float, C.c_float, C.c_int, _vp, _u64p, _vp,
                         _vp, _f32p], None),
+    "tt_build_edges_witness": ([_vp, C.c_uint32, C.c_uint32, C.c_int, C.c_float, C.c_float, C.c_int, _vp, _u64p,
+                                _vp, _vp, _f32p], None),
     "tt_build_edges_csr": ([C.c_uint32, _vp, _vp, _vp, C.c_float, _u64p, _vp, _vp], None),
     "tt_free": ([_vp], C.c_int),
 }

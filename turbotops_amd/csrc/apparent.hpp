@@ -51,6 +51,15 @@ __device__ __forceinline__ bool facet_rule(u32 w, u32 i, u32 j, float d, float d
     return djw != d && diw != d;                // (j, w), (i, w) precede
 }
 
+// The boundary engine's ready test (host/zerodiff.hpp) for the triangle
+// {lo, hi, w} with max(d(lo, w), d(hi, w)) <= D: the edge (lo, hi) of
+// diameter D is its youngest face in (diameter, comb index) order.
+__device__ __forceinline__ bool witness_rule(u32 w, u32 lo, u32 hi, float D, float dlo, float dhi) {
+    if (w < lo) return true;                    // both other faces come first
+    if (w < hi) return dhi < D;                 // face {w, hi} comes later
+    return dlo < D && dhi < D;                  // faces {lo, w} and {hi, w} come later
+}
+
 __device__ __forceinline__ const float* shfl_ptr(const float* p, int src) {
     const u64 v = reinterpret_cast<u64>(p);
     const u32 lo = __shfl(u32(v), src), hi = __shfl(u32(v >> 32), src);
@@ -69,7 +78,11 @@ __device__ __forceinline__ const float* shfl_ptr(const float* p, int src) {
 // w with max(d(i,w), d(j,w)) <= d, highest first, and a ballot over each such
 // block finds the first cofacet. The filter is conservative (a block's minimum
 // includes w = i and w = j), so the first w found is exactly the plain walk's.
-template <u32 TOPK>
+//
+// WITNESS = true: the same search for the boundary engine's ready witness of
+// the edge (j, i), j < i: any w with max(d(i, w), d(j, w)) <= d that passes
+// witness_rule; the result is whether one exists.
+template <u32 TOPK, bool WITNESS = false>
 __device__ __forceinline__ bool edges_apparent_lanes(const float* __restrict__ mat, const float* __restrict__ bmin,
                                                      u32 n, u32 ld, u32 nb, bool valid, u32 i, u32 j,
                                                      float d, u32 lane) {
@@ -84,8 +97,15 @@ __device__ __forceinline__ bool edges_apparent_lanes(const float* __restrict__ m
                     const float a = mat[u64(i) * ld + w];
                     const float c = mat[u64(w) * ld + j];
                     if (fmax_ref(a, c) <= d) {
-                        ap = facet_rule(w, i, j, d, a, c);
-                        done = true;
+                        if constexpr (WITNESS) {
+                            if (witness_rule(w, j, i, d, c, a)) {
+                                ap = true;
+                                done = true;
+                            }
+                        } else {
+                            ap = facet_rule(w, i, j, d, a, c);
+                            done = true;
+                        }
                     }
                 }
             }
@@ -147,7 +167,8 @@ __device__ __forceinline__ bool edges_apparent_lanes(const float* __restrict__ m
                     }
                     av[k] = a;
                     cv[k] = c;
-                    const bool ok = in && u32(w) != ei && u32(w) != ej && fmax_ref(a, c) <= ed;
+                    bool ok = in && u32(w) != ei && u32(w) != ej && fmax_ref(a, c) <= ed;
+                    if constexpr (WITNESS) ok = ok && witness_rule(u32(w), ej, ei, ed, c, a);
                     hit |= ((__ballot(ok) >> (16 * g)) & 0xffffull) << (16 * k);
                 }
                 const int q = hit ? __builtin_ctzll(hit) : 0;     // lowest bit = largest w
@@ -169,7 +190,9 @@ __device__ __forceinline__ bool edges_apparent_lanes(const float* __restrict__ m
             }
             if (live && wf < 0) top -= 64;
         }
-        const bool r = wf >= 0 && facet_rule(u32(wf), ei, ej, ed, diw, djw);
+        bool r;
+        if constexpr (WITNESS) r = wf >= 0;
+        else r = wf >= 0 && facet_rule(u32(wf), ei, ej, ed, diw, djw);
 #pragma unroll
         for (int t = 0; t < 4; ++t) {
             const bool rt = __shfl(int(r), 16 * t) != 0;

@@ -31,8 +31,9 @@
 //                  orders every cloud by `precedes` and keeps clouds contiguous
 //   batch_unpack   (idx, d) per non-apparent edge
 // In the other edge order (tt_batch_set_order), for the boundary engine, the
-// compaction keys hold the combinatorial index itself, the apparent stages
-// are skipped, and every kept edge is sorted and unpacked to its vertex pair.
+// compaction keys hold the combinatorial index itself, the ready-witness test
+// (batch_witness) replaces the apparent test and appends its flag below the
+// key, and every kept edge is sorted and unpacked to its vertex pair.
 // Host copies then run per cloud, each followed by an event, so the host
 // reducer starts on a cloud as soon as its data has landed.
 #include <algorithm>
@@ -258,17 +259,54 @@ __global__ void batch_unpack(const u64* __restrict__ keys, const u32* __restrict
     }
 }
 
+// The boundary engine's ready-witness flag of every compacted comb-order key
+// (cloud, diameter bits, comb index), 64 edges per wave (apparent.hpp); the
+// flag becomes the key's lowest bit.
+__global__ __launch_bounds__(256) void batch_witness(BatchShape b, const float* __restrict__ mats,
+                                                     const float* __restrict__ bmins, u64* __restrict__ keys,
+                                                     const u32* __restrict__ total, int ib) {
+    const u32 lane = threadIdx.x & 63;
+    const u64 gw = (blockIdx.x * 256ull + threadIdx.x) >> 6;
+    const u64 nw = (u64(gridDim.x) * 256) >> 6;
+    const u64 m = *total;
+    const u64 imask = (u64(1) << ib) - 1;
+    for (u64 e0 = gw * 64; e0 < m; e0 += nw * 64) {
+        const u64 e = e0 + lane;
+        const bool valid = e < m;
+        u32 i = 1, j = 0, n = 2, nb = 1;
+        float d = 0.0f;
+        u64 key = 0;
+        const float* mat = mats;
+        const float* bm = bmins;
+        if (valid) {
+            key = keys[e];
+            const u32 c = u32(key >> (31 + ib));
+            d = __uint_as_float(u32(key >> ib) & 0x7fffffffu);
+            n = b.n[c];
+            nb = (n + BLK - 1) / BLK;
+            decode_edge_b(key & imask, i, j);
+            mat = mats + b.mat_off[c];
+            bm = bmins + b.bmin_off[c];
+        }
+        const bool wf = edges_apparent_lanes<TOPK, true>(mat, bm, n, row_ld(n), nb, valid, i, j, d, lane);
+        if (valid) keys[e] = (key << 1) | (wf ? 1u : 0u);
+    }
+}
+
 // Sorted keys of every kept edge in (diameter, combinatorial index) order ->
 // (lo | hi << 32, d): the vertex pair reads as two u32 (lo, hi) in memory.
-__global__ void batch_unpack_comb(const u64* __restrict__ keys, const u32* __restrict__ total, int ib,
+// fb = 1: the key carries the witness flag in bit 0, which goes to bit 31 of hi.
+__global__ void batch_unpack_comb(const u64* __restrict__ keys, const u32* __restrict__ total, int ib, int fb,
                                   u64* __restrict__ out_pair, float* __restrict__ out_d) {
     const u64 m = *total;
     const u64 imask = (u64(1) << ib) - 1;
     for (u64 e = blockIdx.x * 256ull + threadIdx.x; e < m; e += u64(gridDim.x) * 256) {
-        const u64 key = keys[e];
+        const u64 raw = keys[e];
+        const u64 key = raw >> fb;
+        const u32 flag = fb ? u32(raw & 1) : 0u;
         u32 hi, lo;
         decode_edge_b(key & imask, hi, lo);
-        out_pair[e] = u64(lo) | (u64(hi) << 32);
+        out_pair[e] = u64(lo) | (u64(hi | (flag << 31)) << 32);
         out_d[e] = __uint_as_float(u32(key >> ib) & 0x7fffffffu);
     }
 }
@@ -327,6 +365,8 @@ struct BatchState {
     int rpw = 16, ib = 1, cb = 0;
     bool tuned = false;
     bool comb = false;   // hand over every kept edge in (diameter, comb index) order
+    bool witness = false;   // ... with the ready-witness flags (asked for, and the keys have a bit to spare)
+    bool witness_fits = false;
     std::vector<u32> n, dim;
     std::vector<u64> pt_off, mat_off, dmat_off, row_base, tile_base, pair_base, bmin_off;
     u64 total_pts = 0, total_mat = 0, total_rows = 0, total_tiles = 0, capacity = 0, h_cap = 0;
@@ -385,8 +425,13 @@ struct BatchState {
         if (comb) {
             // every kept edge, sorted by (cloud, diameter, comb index); the
             // cloud segments are those of the compaction
-            radix_sort_u64_counted(d_keys, capacity, d_total, 0, cb + 31 + ib, d_alt, d_hist, d_stmp, s);
-            batch_unpack_comb<<<ge, 256, 0, s>>>(d_keys, d_total, ib, d_out_idx, d_out_d);
+            const int fb = witness ? 1 : 0;
+            if (witness) {
+                batch_block_min<<<gw, 256, 0, s>>>(shape, d_mats, d_bmin);
+                batch_witness<<<ge, 256, 0, s>>>(shape, d_mats, d_bmin, d_keys, d_total, ib);
+            }
+            radix_sort_u64_counted(d_keys, capacity, d_total, 0, cb + 31 + ib + fb, d_alt, d_hist, d_stmp, s);
+            batch_unpack_comb<<<ge, 256, 0, s>>>(d_keys, d_total, ib, fb, d_out_idx, d_out_d);
             copy_u32<<<1, 256, 0, s>>>(d_seg, nclouds + 1, d_kseg);
             TTG_LAUNCH_CHECK();
             return;
@@ -471,6 +516,7 @@ void batch_setup(BatchState& b, u32 nclouds, const uint32_t* n, const uint32_t* 
     b.ib = std::max(1, bit_length(maxM - 1));
     b.cb = bit_length(u64(nclouds - 1));
     if (b.cb + 31 + b.ib > 64) fail(TT_ELENGTH, "tt_batch_create: edge keys exceed 64 bits");
+    b.witness_fits = b.cb + 31 + b.ib + 1 <= 64;
     b.grid_dist = u32(std::max<u64>(1, std::min<u64>(b.total_tiles, 256 * 16)));
 
     Workspace& ws = b.ws;
@@ -682,12 +728,15 @@ tt_status tt_batch_cloud(tt_batch* b, uint32_t cloud, const float** mat, float* 
     });
 }
 
-tt_status tt_batch_set_order(tt_batch* b, int comb) {
+tt_status tt_batch_set_order(tt_batch* b, int order) {
     return guarded([&] {
         if (!b) fail(TT_EINVAL, "tt_batch_set_order: null handle");
+        if (order < 0 || order > 2) fail(TT_EINVAL, "tt_batch_set_order: order must be 0, 1 or 2");
         BatchState& s = b->st;
-        if (s.comb == (comb != 0)) return;
-        s.comb = comb != 0;
+        const bool comb = order >= 1, witness = order == 2 && s.witness_fits;
+        if (s.comb == comb && s.witness == witness) return;
+        s.comb = comb;
+        s.witness = witness;
         // the recorded sequence belongs to the other order
         if (s.exec) TTG_HIP(hipGraphExecDestroy(s.exec));
         if (s.graph) TTG_HIP(hipGraphDestroy(s.graph));
@@ -697,7 +746,7 @@ tt_status tt_batch_set_order(tt_batch* b, int comb) {
 }
 
 tt_status tt_batch_cloud_edges(tt_batch* b, uint32_t cloud, const float** mat, float* threshold, uint64_t* m,
-                               const uint32_t** verts, const float** d) {
+                               const uint32_t** verts, const float** d, int* witness_bit) {
     return guarded([&] {
         if (!b || cloud >= b->st.nclouds) fail(TT_EINVAL, "tt_batch_cloud_edges: bad cloud");
         BatchState& s = b->st;
@@ -708,6 +757,7 @@ tt_status tt_batch_cloud_edges(tt_batch* b, uint32_t cloud, const float** mat, f
         if (m) *m = e1 - e0;
         if (verts) *verts = reinterpret_cast<const uint32_t*>(s.h_idx + e0);
         if (d) *d = s.h_d + e0;
+        if (witness_bit) *witness_bit = s.witness ? 1 : 0;
     });
 }
 

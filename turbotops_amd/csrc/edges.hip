@@ -120,13 +120,13 @@ __global__ __launch_bounds__(EB) void edge_unpack(const u64* __restrict__ keys, 
 }
 
 // EdgeList vertex pairs (smaller id first) of sorted edge indices.
-__global__ __launch_bounds__(EB) void edge_verts(const u64* __restrict__ idx, u64 m,
+__global__ __launch_bounds__(EB) void edge_verts(const u64* __restrict__ idx, u64 m, const u8* __restrict__ flags,
                                                  int32_t* __restrict__ verts) {
     for (u64 t = blockIdx.x * u64(EB) + threadIdx.x; t < m; t += u64(gridDim.x) * EB) {
         u32 i, j;
         decode_edge(idx[t], i, j);
         verts[2 * t] = int32_t(j);
-        verts[2 * t + 1] = int32_t(i);
+        verts[2 * t + 1] = int32_t(i | (flags && flags[t] ? 0x80000000u : 0u));
     }
 }
 
@@ -166,6 +166,32 @@ __global__ __launch_bounds__(EB) void apparent_scan(const float* __restrict__ ma
     }
 }
 
+// The boundary engine's ready-witness flag of the compacted comb-order edges
+// (keys hold the combinatorial index itself), 64 edges per wave; the flag
+// becomes the key's lowest bit, as in apparent_scan.
+__global__ __launch_bounds__(EB) void witness_scan(const float* __restrict__ mat, u32 n,
+                                                   const float* __restrict__ bmin, u32 nb,
+                                                   u64* __restrict__ keys, u64 m, int ib) {
+    const u32 lane = threadIdx.x & 63;
+    const u64 gw = (blockIdx.x * u64(EB) + threadIdx.x) >> 6;
+    const u64 nw = (u64(gridDim.x) * EB) >> 6;
+    const u64 mask = (u64(1) << ib) - 1;
+    for (u64 e0 = gw * 64; e0 < m; e0 += nw * 64) {
+        const u64 e = e0 + lane;
+        const bool valid = e < m;
+        u64 k = 0;
+        u32 i = 1, j = 0;
+        float d = 0.0f;
+        if (valid) {
+            k = keys[e];
+            decode_edge(k & mask, i, j);
+            d = mat[u64(i) * n + j];
+        }
+        const bool wf = edges_apparent_lanes<TOPK, true>(mat, bmin, n, n, nb, valid, i, j, d, lane);
+        if (valid) keys[e] = (k << 1) | (wf ? 1u : 0u);
+    }
+}
+
 // Sorted keys carrying the apparent flag in bit 0 -> flags.
 __global__ __launch_bounds__(EB) void edge_flags(u64* __restrict__ keys, u64 m, u8* __restrict__ flags) {
     for (u64 t = blockIdx.x * u64(EB) + threadIdx.x; t < m; t += u64(gridDim.x) * EB) {
@@ -185,6 +211,7 @@ u32 grid_for(u64 waves) {
 // their apparent flags. Buffers live in ws.
 DenseEdges dense_edges(const float* d_mat, u32 n, float thr, bool want_apparent, Workspace& ws,
                        hipStream_t s, bool comb) {
+    // want_apparent: in comb order, the ready-witness flags
     DenseEdges out;
     if (n < 2) return out;
     if (u64(n) * (n - 1) / 2 >= (u64(1) << 32))
@@ -208,7 +235,6 @@ DenseEdges dense_edges(const float* d_mat, u32 n, float thr, bool want_apparent,
     const int ib = std::max(1, bit_length(M - 1));
     const int db = bit_length(u64(dmax - dmin));
     if (ib + db + 1 > 64) fail(TT_ELENGTH, "dense_edges: key does not fit 64 bits");
-    if (want_apparent && comb) fail(TT_EINVAL, "dense_edges: the apparent test runs in `precedes` order only");
     u64* keys = ws.get<u64>(out.m);
     u64* alt = ws.get<u64>(out.m);
     edge_compact<<<grid_for(n), EB, 0, s>>>(d_mat, n, thr, cnt, dmin, ib, M, comb, keys);
@@ -219,7 +245,8 @@ DenseEdges dense_edges(const float* d_mat, u32 n, float thr, bool want_apparent,
         float* bmin = ws.get<float>(u64(n) * nb);
         block_min<<<grid_for(n), EB, 0, s>>>(d_mat, n, nb, bmin);
         TTG_LAUNCH_CHECK();
-        apparent_scan<<<grid_for((out.m + 63) / 64), EB, 0, s>>>(d_mat, n, bmin, nb, keys, out.m, ib, M);
+        if (comb) witness_scan<<<grid_for((out.m + 63) / 64), EB, 0, s>>>(d_mat, n, bmin, nb, keys, out.m, ib);
+        else apparent_scan<<<grid_for((out.m + 63) / 64), EB, 0, s>>>(d_mat, n, bmin, nb, keys, out.m, ib, M);
         TTG_LAUNCH_CHECK();
     }
     radix_sort_u64(keys, nullptr, out.m, 0, ib + db + fb, alt, nullptr, ws, s);
@@ -316,9 +343,9 @@ void tt_free(void* p) { std::free(p); }
 
 namespace ttg {
 
-void edge_vertices(const u64* d_idx, u64 m, int32_t* d_verts, hipStream_t s) {
+void edge_vertices(const u64* d_idx, u64 m, int32_t* d_verts, hipStream_t s, const u8* d_flags) {
     if (!m) return;
-    edge_verts<<<u32(std::min<u64>((m + EB - 1) / EB, 65536)), EB, 0, s>>>(d_idx, m, d_verts);
+    edge_verts<<<u32(std::min<u64>((m + EB - 1) / EB, 65536)), EB, 0, s>>>(d_idx, m, d_flags, d_verts);
     TTG_LAUNCH_CHECK();
 }
 
@@ -392,9 +419,10 @@ extern "C" tt_status tt_dense_filtration(const float* pts, uint32_t n, uint32_t 
 // RedZeD's build_edges over it (redzed.hpp:544-613 with DenseGeom::bucket_row
 // :677-705): edges i < j with d <= threshold in (diameter ascending,
 // combinatorial index ascending) order, as EdgeList holds them (:436-442).
-extern "C" tt_status tt_build_edges(const float* pts, uint32_t n, uint32_t dim, int kind, float p,
-                                    float threshold, int layout, float* mat, uint64_t* m,
-                                    int32_t** verts, float** diam, float* threshold_out) {
+namespace {
+tt_status build_edges_impl(const float* pts, uint32_t n, uint32_t dim, int kind, float p, float threshold,
+                           int layout, float* mat, uint64_t* m, int32_t** verts, float** diam,
+                           float* threshold_out, bool witness) {
     using namespace ttg;
     return guarded([&] {
         check_metric(kind, p);
@@ -422,9 +450,9 @@ extern "C" tt_status tt_build_edges(const float* pts, uint32_t n, uint32_t dim, 
             TTG_HIP(hipStreamSynchronize(s));
         }
         if (threshold_out) *threshold_out = thr;
-        const DenseEdges e = dense_edges(dm, n, thr, false, ws, s, true);
+        const DenseEdges e = dense_edges(dm, n, thr, witness, ws, s, true);
         int32_t* vd = ws.get<int32_t>(2 * e.m);
-        edge_vertices(e.idx, e.m, vd, s);
+        edge_vertices(e.idx, e.m, vd, s, witness ? e.apparent : nullptr);
         if (mat) {
             if (layout == TT_LAYOUT_FULL) {
                 if (u64(n) * n)
@@ -446,4 +474,17 @@ extern "C" tt_status tt_build_edges(const float* pts, uint32_t n, uint32_t dim, 
         }
         TTG_HIP(hipStreamSynchronize(s));
     });
+}
+}  // namespace
+
+extern "C" tt_status tt_build_edges(const float* pts, uint32_t n, uint32_t dim, int kind, float p,
+                                    float threshold, int layout, float* mat, uint64_t* m,
+                                    int32_t** verts, float** diam, float* threshold_out) {
+    return build_edges_impl(pts, n, dim, kind, p, threshold, layout, mat, m, verts, diam, threshold_out, false);
+}
+
+extern "C" tt_status tt_build_edges_witness(const float* pts, uint32_t n, uint32_t dim, int kind, float p,
+                                            float threshold, int layout, float* mat, uint64_t* m,
+                                            int32_t** verts, float** diam, float* threshold_out) {
+    return build_edges_impl(pts, n, dim, kind, p, threshold, layout, mat, m, verts, diam, threshold_out, true);
 }

@@ -19,14 +19,17 @@ struct DenseEdges {
     u64 m = 0;
     u64* idx = nullptr;      // device, filtration order
     float* d = nullptr;      // device
-    u8* apparent = nullptr;  // device (null unless requested)
+    u8* apparent = nullptr;  // device (null unless requested); the witness flag in comb order
 };
 // comb: ties in diameter broken by ascending index (RedZeD's order)
-// instead of descending (the coboundary reducer's `precedes`).
-DenseEdges dense_edges(const float* d_mat, u32 n, float thr, bool want_apparent, Workspace& ws,
+// instead of descending (the coboundary reducer's `precedes`). want_flags:
+// the apparent flags in `precedes` order, the boundary engine's ready-witness
+// flags (witness_rule) in comb order.
+DenseEdges dense_edges(const float* d_mat, u32 n, float thr, bool want_flags, Workspace& ws,
                        hipStream_t s, bool comb = false);
-// EdgeList vertex pairs (smaller id first) of edge indices.
-void edge_vertices(const u64* d_idx, u64 m, int32_t* d_verts, hipStream_t s);
+// EdgeList vertex pairs (smaller id first) of edge indices; with flags, bit
+// 31 of each larger id carries the edge's flag.
+void edge_vertices(const u64* d_idx, u64 m, int32_t* d_verts, hipStream_t s, const u8* d_flags = nullptr);
 
 // sparse.hip
 struct SparseEdges {

@@ -178,11 +178,14 @@ def sparse_edges(graph, threshold=math.inf, apparent=True):
     return idx, dd, ap
 
 
-def build_edges(x, metric="l2", threshold=math.inf, layout="full", with_matrix=False):
+def build_edges(x, metric="l2", threshold=math.inf, layout="full", with_matrix=False, witness=False):
     """The boundary (RedZeD) engine's front end: make_dense + enclosing
     radius (turbotops.hpp:243-246) and build_edges (redzed.hpp:544-613).
     Returns (verts (m, 2) int32 with verts[:, 0] < verts[:, 1], diam f32,
-    threshold[, mat]) in (diameter asc, combinatorial index asc) order."""
+    threshold[, mat]) in (diameter asc, combinatorial index asc) order.
+    witness=True (tt_build_edges_witness) adds, after diam, each edge's
+    ready-witness flag (bool): a triangle exists whose youngest face in this
+    order is the edge."""
     x = _f32(x)
     if x.ndim != 2:
         raise ValueError("x must be a 2-d array (n, d)")
@@ -196,15 +199,21 @@ def build_edges(x, metric="l2", threshold=math.inf, layout="full", with_matrix=F
     m = C.c_uint64()
     pv, pd = C.POINTER(C.c_int32)(), C.POINTER(C.c_float)()
     thr = C.c_float()
-    lib.call("tt_build_edges", _addr(x), n, dim, kind, p, C.c_float(threshold),
-             capi.LAYOUT_FULL if full else capi.LAYOUT_LOWER, _addr(mat) if mat is not None else None,
-             C.byref(m), C.byref(pv), C.byref(pd), C.byref(thr))
+    lib.call("tt_build_edges_witness" if witness else "tt_build_edges", _addr(x), n, dim, kind, p,
+             C.c_float(threshold), capi.LAYOUT_FULL if full else capi.LAYOUT_LOWER,
+             _addr(mat) if mat is not None else None, C.byref(m), C.byref(pv), C.byref(pd), C.byref(thr))
     k = m.value
     verts = _take(lib, pv, 2 * k, C.c_int32, np.int32).reshape(k, 2)
     diam = _take(lib, pd, k, C.c_float, np.float32)
+    out = [verts, diam]
+    if witness:
+        u = verts.view(np.uint32)
+        out.append((u[:, 1] >> 31).astype(bool))
+        verts[:, 1] = (u[:, 1] & 0x7FFFFFFF).astype(np.int32)
+    out.append(thr.value)
     if with_matrix:
-        return verts, diam, thr.value, (mat.reshape(n, n) if full else mat)
-    return verts, diam, thr.value
+        out.append(mat.reshape(n, n) if full else mat)
+    return tuple(out)
 
 
 def build_edges_csr(graph, threshold=math.inf):

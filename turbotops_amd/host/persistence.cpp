@@ -87,12 +87,14 @@ std::string select_mode(Mode mode, u32 n, u32 maxdim, double mean_degree) {
 
 // H0 and H1 by the boundary engine: full matrix (row stride n) and every
 // edge up to the threshold in (diameter, combinatorial index) order.
-void reduce_boundary(u32 n, const f32* mat, u64 m, const u32* verts, const f32* d, Barcode& out, Stats& st) {
+// witness: bit 31 of each hi id carries the ready-witness flag.
+void reduce_boundary(u32 n, const f32* mat, u64 m, const u32* verts, const f32* d, bool witness, Barcode& out,
+                     Stats& st) {
     const auto t0 = std::chrono::steady_clock::now();
     out.ensure(1);
     if (n) {
         zd::BoundaryH1 eng(n, mat, n);
-        eng.run(m, verts, d, out);
+        eng.run(m, verts, d, out, witness);
     }
     st.edges = m;
     st.seconds_reduce = since(t0);
@@ -345,12 +347,13 @@ Result persistence(const f32* points, u32 n, u32 dim, const Options& o) {
             f32* pd = nullptr;
             f32 thr = INFINITY;
             const auto tf = std::chrono::steady_clock::now();
-            check(tt_build_edges(points, n, dim, cloud.metric.kind, cloud.metric.p, INFINITY, TT_LAYOUT_FULL,
-                                 mat.data(), &m, &pv, &pd, &thr));
+            check(tt_build_edges_witness(points, n, dim, cloud.metric.kind, cloud.metric.p, INFINITY,
+                                         TT_LAYOUT_FULL, mat.data(), &m, &pv, &pd, &thr));
             st.seconds_front = since(tf);
             const std::vector<int32_t> verts = take(pv, 2 * m);
             const std::vector<f32> d = take(pd, m);
-            reduce_boundary(n, mat.data(), m, reinterpret_cast<const u32*>(verts.data()), d.data(), r.barcode, st);
+            reduce_boundary(n, mat.data(), m, reinterpret_cast<const u32*>(verts.data()), d.data(), true,
+                            r.barcode, st);
             st.engine = "boundary";
             st.method = "dense-redzed";
             st.threshold = thr;
@@ -412,7 +415,7 @@ Result persistence(const f32* points, u32 n, u32 dim, const Options& o) {
 
 // ---- batches ---------------------------------------------------------------
 BatchRunner::BatchRunner(const std::vector<u32>& n, const std::vector<u32>& dim, const std::string& metric,
-                         u32 maxdim, Mode mode)
+                         u32 maxdim, Mode mode, bool witness_flags)
     : n_(n), dim_(dim), maxdim_(maxdim) {
     if (n.size() != dim.size()) throw std::invalid_argument("persistence_batch: n and dim differ in length");
     if (maxdim != 1) throw std::invalid_argument("persistence_batch: maxdim must be 1 (H0 and H1)");
@@ -421,7 +424,7 @@ BatchRunner::BatchRunner(const std::vector<u32>& n, const std::vector<u32>& dim,
     for (u32 v : n) nmax = std::max(nmax, v);
     boundary_ = boundary_engine(mode, nmax, maxdim, true);
     check(tt_batch_create(u32(n.size()), n.data(), dim.data(), m.kind, m.p, INFINITY, &h_));
-    if (boundary_) check(tt_batch_set_order(h_, 1));
+    if (boundary_) check(tt_batch_set_order(h_, witness_flags ? 2 : 1));
 }
 
 BatchRunner::~BatchRunner() {
@@ -535,8 +538,9 @@ std::vector<Result> BatchRunner::run(const f32* pts, bool use_graph, u32 threads
                 Result& r = out[c];
                 if (boundary_) {
                     const u32* verts = nullptr;
-                    check(tt_batch_cloud_edges(h_, c, &mat, &thr, &m, &verts, &d));
-                    reduce_boundary(n_[c], mat, m, verts, d, r.barcode, r.stats);
+                    int witness = 0;
+                    check(tt_batch_cloud_edges(h_, c, &mat, &thr, &m, &verts, &d, &witness));
+                    reduce_boundary(n_[c], mat, m, verts, d, witness != 0, r.barcode, r.stats);
                     r.stats.engine = "boundary";
                 } else {
                     const u64* idx = nullptr;

@@ -218,11 +218,11 @@ PYBIND11_MODULE(_turbotops, m) {
           py::arg("metric"), py::arg("graph"), py::arg("k"), py::arg("threads"), py::arg("memory_budget"));
     py::class_<BatchRunner>(m, "BatchRunner")
         .def(py::init([](const std::vector<u32>& n, const std::vector<u32>& dim, const std::string& metric,
-                         u32 maxdim, const std::string& mode) {
-                 return std::make_unique<BatchRunner>(n, dim, metric, maxdim, parse_mode(mode));
+                         u32 maxdim, const std::string& mode, bool witness_flags) {
+                 return std::make_unique<BatchRunner>(n, dim, metric, maxdim, parse_mode(mode), witness_flags);
              }),
              py::arg("n"), py::arg("dim"), py::arg("metric") = "l2", py::arg("maxdim") = 1,
-             py::arg("mode") = "auto")
+             py::arg("mode") = "auto", py::arg("witness_flags") = false)
         .def("front", &batch_front, py::arg("clouds"), py::arg("use_graph") = true)
         .def("run", &batch_run, py::arg("clouds"), py::arg("use_graph") = true, py::arg("threads") = 0,
              py::arg("pairs") = false)

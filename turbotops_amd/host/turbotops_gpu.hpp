@@ -175,9 +175,11 @@ class BatchRunner {
   public:
     // mode: Auto and Boundary reduce with the boundary engine (every kept
     // edge in (diameter, comb index) order), Coboundary with the cohomology
-    // engine (the non-apparent edges in its order).
+    // engine (the non-apparent edges in its order). witness_flags: the
+    // boundary engine's inputs carry the GPU's ready-witness flags (faster
+    // host reductions on clouds without long cycles, one more device scan).
     BatchRunner(const std::vector<u32>& n, const std::vector<u32>& dim, const std::string& metric,
-                u32 maxdim = 1, Mode mode = Mode::Auto);
+                u32 maxdim = 1, Mode mode = Mode::Auto, bool witness_flags = false);
     ~BatchRunner();
     BatchRunner(const BatchRunner&) = delete;
     BatchRunner& operator=(const BatchRunner&) = delete;

@@ -171,13 +171,25 @@ public:
 
     // verts: 2 m vertex ids (lo < hi per edge), diam: m diameters, in
     // (diameter, combinatorial index) order, every edge up to the threshold.
-    void run(u64 m, const u32* verts, const f32* diam, cob::Diagram& out) {
+    // witness_bit: bit 31 of each hi id is the edge's ready-witness flag (a
+    // triangle with the edge as its youngest face exists; the GPU computes
+    // it). With the flags an edge with no active face around it, by far the
+    // most common case on most clouds, needs only its flag, and the radius
+    // graph is not kept: adjacency at radius D is d(u, w) <= D read from the
+    // matrix. Where edges with active faces around them become common (long
+    // cycles: a noisy circle), the engine switches the radius graph on, so
+    // their witness searches run word-parallel again.
+    void run(u64 m, const u32* verts, const f32* diam, cob::Diagram& out, bool witness_bit = false) {
         if (m >= 0xffffffffull) throw std::length_error("boundary engine: more than 2^32 - 2 edges");
         out.ensure(1);
         m_ = m;
         verts_ = verts;
         diam_ = diam;
-        adj_.assign(n_ * W_, 0);
+        flags_ = witness_bit;
+        bits_ = !flags_;
+        win_edges_ = win_cand_ = quiet_ = 0;
+        adj_.clear();
+        if (bits_) adj_.assign(n_ * W_, 0);
         seen_.assign(W_, 0);
         seed_a_.assign(n_, 0);
         seed_b_.assign(n_, 0);
@@ -195,7 +207,7 @@ public:
         gen_free_.clear();
         rptr_ = resolved_ = 0;
         for (u64 e = 0; e < m; ++e) {
-            const u32 a = verts[2 * e], b = verts[2 * e + 1];
+            const u32 a = verts[2 * e], b = hi_of(e);
             const f32 d = diam[e];
             u32 ra = find(a), rb = find(b);
             if (ra != rb) {
@@ -213,8 +225,7 @@ public:
         G_.for_each([&](u32 g, u32) { alive.push_back(g); });
         std::sort(alive.begin(), alive.end());
         for (u32 g : alive)
-            out.add(1, diam_[g - 1], INFINITY, u128(comb2(verts_[2 * (g - 1)], verts_[2 * (g - 1) + 1])),
-                    u128(~u64(0)));
+            out.add(1, diam_[g - 1], INFINITY, u128(comb2(verts_[2 * (g - 1)], hi_of(g - 1))), u128(~u64(0)));
     }
 
     u64 triangles_resolved() const { return resolved_; }
@@ -227,6 +238,9 @@ private:
     const f32* mat_;
     u64 ld_;
     u64 m_ = 0, rptr_ = 0, resolved_ = 0;
+    bool flags_ = false, bits_ = true;   // witness flags given; radius graph kept
+    u32 win_edges_ = 0, win_cand_ = 0;   // cycle edges / witness searches, this window
+    u32 quiet_ = 0;                      // quiet windows in a row with the radius graph on
     const u32* verts_ = nullptr;
     const f32* diam_ = nullptr;
     std::vector<u64> adj_;                               // radius graph rows, W_ words each
@@ -247,6 +261,8 @@ private:
     u32 cur_ = 0;                                        // the edge being processed (not linked yet)
 
     f32 dist(u32 u, u32 v) const { return mat_[u64(u) * ld_ + v]; }
+    u32 hi_of(u64 e) const { return verts_[2 * e + 1] & 0x7fffffffu; }
+    bool witness_of(u64 e) const { return verts_[2 * e + 1] >> 31; }
     u32 find(u32 x) {
         while (parent_[x] != x) {
             parent_[x] = parent_[parent_[x]];
@@ -254,10 +270,36 @@ private:
         }
         return x;
     }
-    bool adjacent(u32 u, u32 w) const { return (adj_[u * W_ + (w >> 6)] >> (w & 63)) & 1; }
+    // in the radius graph at radius D (all edges of diameter <= D)
+    bool adjacent(u32 u, u32 w, f32 D) const {
+        return bits_ ? (adj_[u * W_ + (w >> 6)] >> (w & 63)) & 1 : dist(u, w) <= D;
+    }
+    // Witness searches (edges whose class survived a seeded triangle) cost a
+    // word-parallel AND with the radius graph, or two matrix rows without
+    // it. Per window of 4096 cycle edges: many searches switch the radius
+    // graph on (caught up to radius D), two quiet windows in a row switch it
+    // off again (keeping it costs two bit writes per edge).
+    void note_edge(f32 D) {
+        if (++win_edges_ < 4096) return;
+        if (!bits_ && win_cand_ > 64) {
+            adj_.assign(n_ * W_, 0);
+            rptr_ = 0;
+            bits_ = true;
+            advance(D);
+            quiet_ = 0;
+        } else if (bits_ && win_cand_ < 4 && ++quiet_ >= 2) {
+            bits_ = false;
+            adj_.clear();
+            adj_.shrink_to_fit();
+        } else if (win_cand_ >= 4) {
+            quiet_ = 0;
+        }
+        win_edges_ = win_cand_ = 0;
+    }
     void advance(f32 D) {
+        if (!bits_) return;
         while (rptr_ < m_ && diam_[rptr_] <= D) {
-            const u32 u = verts_[2 * rptr_], v = verts_[2 * rptr_ + 1];
+            const u32 u = verts_[2 * rptr_], v = hi_of(rptr_);
             adj_[u * W_ + (v >> 6)] |= u64(1) << (v & 63);
             adj_[v * W_ + (u >> 6)] |= u64(1) << (u & 63);
             ++rptr_;
@@ -268,18 +310,44 @@ private:
         if (w < b) return dist(w, b) < D;
         return dist(w, a) < D && dist(w, b) < D;
     }
-    // a ready triangle {a, b, w} with no active face besides (a, b), or -1
-    int64_t leftover(u32 a, u32 b, f32 D) const {
+    // a ready triangle {a, b, w} with no active face besides (a, b), or -1.
+    // Any common neighbour w < a is ready without a distance read, so the
+    // scan goes down from a's word first (nearest first: with labels that
+    // follow the geometry, common neighbours sit near a), then up.
+    int64_t leftover(u32 a, u32 b, f32 D) {
+        ++win_cand_;
+        if (!bits_) return leftover_dist(a, b, D);
         const u64* ra = &adj_[a * W_];
         const u64* rb = &adj_[b * W_];
-        for (u64 t = 0; t < W_; ++t) {
+        const u64 ta = a >> 6;
+        const u64 below = (u64(1) << (a & 63)) - 1;   // bits of word ta under a
+        for (u64 t = ta + 1; t-- > 0;) {
             u64 c = ra[t] & rb[t] & ~seen_[t];
+            if (t == ta) c &= below;
+            if (c) return int64_t(t * 64 + 63 - std::countl_zero(c));
+        }
+        for (u64 t = ta; t < W_; ++t) {
+            u64 c = ra[t] & rb[t] & ~seen_[t];
+            if (t == ta) c &= ~below & ~(u64(1) << (a & 63));
             while (c) {
                 const u32 w = u32(t * 64 + std::countr_zero(c));
                 c &= c - 1;
                 if (ready(a, b, D, w)) return w;
             }
         }
+        return -1;
+    }
+
+    // leftover() from the matrix rows: w below a first (ready once a common
+    // neighbour), nearest first, then above
+    int64_t leftover_dist(u32 a, u32 b, f32 D) const {
+        const f32* ra = mat_ + u64(a) * ld_;
+        const f32* rb = mat_ + u64(b) * ld_;
+        auto unseen = [&](u32 w) { return !((seen_[w >> 6] >> (w & 63)) & 1); };
+        for (u32 w = a; w-- > 0;)
+            if (ra[w] <= D && rb[w] <= D && unseen(w)) return w;
+        for (u32 w = a + 1; w < n_; ++w)
+            if (w != b && ra[w] <= D && rb[w] <= D && unseen(w) && ready(a, b, D, w)) return w;
         return -1;
     }
 
@@ -327,7 +395,7 @@ private:
         cls_free_.push_back(c);
         if (z != cur_) {
             unlink_from(verts_[2 * (z - 1)], c);
-            unlink_from(verts_[2 * (z - 1) + 1], c);
+            unlink_from(hi_of(z - 1), c);
         }
     }
     // beta_ (nonzero boundary class of a triangle of diameter D): its
@@ -355,7 +423,7 @@ private:
         gen_free_.push_back(gj);
         G_.erase(j);
         const f32 birth = diam_[j - 1];
-        if (birth != D) out.add(1, birth, D, u128(comb2(verts_[2 * (j - 1)], verts_[2 * (j - 1) + 1])), tri);
+        if (birth != D) out.add(1, birth, D, u128(comb2(verts_[2 * (j - 1)], hi_of(j - 1))), tri);
     }
     // beta_ = sum of the classes in slots p, q, r (1 + slot, 0 = none);
     // false when it is zero. Classes of one generator, the common case, are
@@ -393,7 +461,7 @@ private:
         }
         cand_.clear();
         for (const auto& [w, c] : inc_[b])
-            if (w != a && adjacent(a, w)) {
+            if (w != a && adjacent(a, w, D)) {
                 if (!((seen_[w >> 6] >> (w & 63)) & 1)) {
                     seen_[w >> 6] |= u64(1) << (w & 63);
                     cand_.push_back(w);
@@ -401,7 +469,7 @@ private:
                 seed_b_[w] = c + 1;
             }
         for (const auto& [w, c] : inc_[a])
-            if (w != b && adjacent(b, w)) {
+            if (w != b && adjacent(b, w, D)) {
                 if (!((seen_[w >> 6] >> (w & 63)) & 1)) {
                     seen_[w >> 6] |= u64(1) << (w & 63);
                     cand_.push_back(w);
@@ -409,10 +477,12 @@ private:
                 seed_a_[w] = c + 1;
             }
         cur_ = r;
+        if (flags_) note_edge(D);
+        const bool witness = flags_ ? witness_of(r - 1) : true;
         if (cand_.empty()) {
             // every triangle entering with x has boundary {x}: the first
             // ready one kills x's class as it is born
-            if (leftover(a, b, D) < 0) {
+            if (flags_ ? !witness : leftover(a, b, D) < 0) {
                 const u32 c = create(r);
                 inc_[a].push_back({b, c});
                 inc_[b].push_back({a, c});
@@ -421,16 +491,27 @@ private:
             return;
         }
         const u32 xc = create(r);
+        bool dirty = false;
         for (u32 w : cand_) {
             if (!boundary_class(head_[xc] ? xc + 1 : 0, seed_a_[w], seed_b_[w])) continue;
             if (!ready(a, b, D, w)) continue;
             resolve(D, comb3(a, b, w), out);
+            dirty = true;
         }
-        if (head_[xc]) {
-            const int64_t w = leftover(a, b, D);
-            if (w >= 0) {
+        if (head_[xc] && witness) {
+            if (flags_ && !dirty) {
+                // x's class is still {x}, so every seeded triangle had a
+                // nonzero boundary and none was ready: the witness is an
+                // unseeded triangle, and it kills x as it is born (a bar of
+                // length 0, recorded by nobody)
                 beta_ = cls_[xc];
-                resolve(D, comb3(a, b, u32(w)), out);
+                resolve(D, 0, out);
+            } else {
+                const int64_t w = leftover(a, b, D);
+                if (w >= 0) {
+                    beta_ = cls_[xc];
+                    resolve(D, comb3(a, b, u32(w)), out);
+                }
             }
         }
         if (head_[xc]) {
