@@ -193,15 +193,20 @@ public:
         seen_.assign(W_, 0);
         seed_a_.assign(n_, 0);
         seed_b_.assign(n_, 0);
+        hseed_a_.assign(n_, 0);
+        hseed_b_.assign(n_, 0);
+        cand_.assign(n_, 0);
+        ncand_ = 0;
         inc_.assign(n_, {});
         parent_.resize(n_);
         for (u32 v = 0; v < n_; ++v) parent_[v] = v;
         size_.assign(n_, 1);
-        cand_.clear();
         G_ = RankMap();
         cls_.clear();
         head_.clear();
         owner_.clear();
+        pos_lo_.clear();
+        pos_hi_.clear();
         cls_free_.clear();
         gen_.clear();
         gen_free_.clear();
@@ -246,8 +251,17 @@ private:
     std::vector<u64> adj_;                               // radius graph rows, W_ words each
     std::vector<u64> seen_;                              // candidate vertices of the current edge
     std::vector<u32> seed_a_, seed_b_;                   // 1 + class slot of edge (a, w) / (b, w)
-    std::vector<u32> cand_;
-    std::vector<std::vector<std::pair<u32, u32>>> inc_;  // per vertex: (other end, class slot)
+    std::vector<u32> hseed_a_, hseed_b_;                 // their class heads when collected
+    std::vector<u32> cand_;                              // candidates: cand_[0, ncand_)
+    u32 ncand_ = 0;
+    // per vertex, its active edges: other end, class slot, and the class's
+    // head word, kept current (set_head), so that collecting the candidates
+    // reads the classes in order instead of one scattered load each
+    struct Inc {
+        u32 w, c, h;
+    };
+    std::vector<std::vector<Inc>> inc_;
+    std::vector<u32> pos_lo_, pos_hi_;                   // per class slot: its entries in inc_[lo], inc_[hi]
     std::vector<u32> parent_, size_;
     // classes of active edges, by slot: generator set, its single element (0
     // when empty, MULTI when larger), and the edge's rank
@@ -362,7 +376,19 @@ private:
     }
     void set_head(u32 c) {
         const auto& v = cls_[c];
-        head_[c] = v.empty() ? 0 : v.size() == 1 ? v[0] : MULTI;
+        const u32 h = v.empty() ? 0 : v.size() == 1 ? v[0] : MULTI;
+        head_[c] = h;
+        const u32 z = owner_[c];
+        if (h && z != cur_) {   // linked: refresh its two incident entries
+            inc_[verts_[2 * (z - 1)]][pos_lo_[c]].h = h;
+            inc_[hi_of(z - 1)][pos_hi_[c]].h = h;
+        }
+    }
+    void link(u32 c, u32 lo, u32 hi) {
+        pos_lo_[c] = u32(inc_[lo].size());
+        inc_[lo].push_back({hi, c, head_[c]});
+        pos_hi_[c] = u32(inc_[hi].size());
+        inc_[hi].push_back({lo, c, head_[c]});
     }
     // class {x} and G[x] = {its slot} for a new non-tree edge x; returns the slot
     u32 create(u32 x) {
@@ -370,6 +396,10 @@ private:
         if (head_.size() < cls_.size()) {
             head_.resize(cls_.size());
             owner_.resize(cls_.size());
+        }
+        if (pos_lo_.size() < cls_.size()) {
+            pos_lo_.resize(cls_.size());
+            pos_hi_.resize(cls_.size());
         }
         cls_[c].assign(1, x);
         head_[c] = x;
@@ -379,14 +409,16 @@ private:
         G_.put(x, g + 1);
         return c;
     }
-    void unlink_from(u32 v, u32 c) {
+    // removes class c's entry at position k of inc_[v]; the last entry moves
+    // into its place, and its own position is updated
+    void unlink_at(u32 v, u32 k) {
         auto& l = inc_[v];
-        for (size_t k = 0; k < l.size(); ++k)
-            if (l[k].second == c) {
-                l[k] = l.back();
-                l.pop_back();
-                return;
-            }
+        l[k] = l.back();
+        l.pop_back();
+        if (k < l.size()) {
+            const Inc& mv = l[k];
+            (v < mv.w ? pos_lo_ : pos_hi_)[mv.c] = k;
+        }
     }
     void drop_class(u32 c) {
         const u32 z = owner_[c];
@@ -394,8 +426,8 @@ private:
         head_[c] = 0;
         cls_free_.push_back(c);
         if (z != cur_) {
-            unlink_from(verts_[2 * (z - 1)], c);
-            unlink_from(hi_of(z - 1), c);
+            unlink_at(verts_[2 * (z - 1)], pos_lo_[c]);
+            unlink_at(hi_of(z - 1), pos_hi_[c]);
         }
     }
     // beta_ (nonzero boundary class of a triangle of diameter D): its
@@ -424,6 +456,15 @@ private:
         G_.erase(j);
         const f32 birth = diam_[j - 1];
         if (birth != D) out.add(1, birth, D, u128(comb2(verts_[2 * (j - 1)], hi_of(j - 1))), tri);
+    }
+    // three class heads (0 = no class) of one generator each that cancel in
+    // pairs: the boundary class is zero without looking at the sets
+    static bool singles_cancel(u32 x, u32 y, u32 z) {
+        if (x == MULTI || y == MULTI || z == MULTI) return false;
+        if (!x) return y == z;
+        if (!y) return x == z;
+        if (!z) return x == y;
+        return false;   // three nonzero singles never cancel
     }
     // beta_ = sum of the classes in slots p, q, r (1 + slot, 0 = none);
     // false when it is zero. Classes of one generator, the common case, are
@@ -455,48 +496,66 @@ private:
 
     void cycle_edge(u32 r, u32 a, u32 b, f32 D, cob::Diagram& out) {
         advance(D);
-        for (u32 w : cand_) {
+        for (u32 t = 0; t < ncand_; ++t) {
+            const u32 w = cand_[t];
             seen_[w >> 6] = 0;
             seed_a_[w] = seed_b_[w] = 0;
         }
-        cand_.clear();
-        for (const auto& [w, c] : inc_[b])
-            if (w != a && adjacent(a, w, D)) {
+        u32 k = 0;
+        for (const Inc& e : inc_[b])
+            if (e.w != a && adjacent(a, e.w, D)) {
+                const u32 w = e.w;
                 if (!((seen_[w >> 6] >> (w & 63)) & 1)) {
                     seen_[w >> 6] |= u64(1) << (w & 63);
-                    cand_.push_back(w);
+                    cand_[k++] = w;
+                    hseed_a_[w] = 0;
                 }
-                seed_b_[w] = c + 1;
+                seed_b_[w] = e.c + 1;
+                hseed_b_[w] = e.h;
             }
-        for (const auto& [w, c] : inc_[a])
-            if (w != b && adjacent(b, w, D)) {
+        for (const Inc& e : inc_[a])
+            if (e.w != b && adjacent(b, e.w, D)) {
+                const u32 w = e.w;
                 if (!((seen_[w >> 6] >> (w & 63)) & 1)) {
                     seen_[w >> 6] |= u64(1) << (w & 63);
-                    cand_.push_back(w);
+                    cand_[k++] = w;
+                    hseed_b_[w] = 0;
                 }
-                seed_a_[w] = c + 1;
+                seed_a_[w] = e.c + 1;
+                hseed_a_[w] = e.h;
             }
+        ncand_ = k;
         cur_ = r;
         if (flags_) note_edge(D);
         const bool witness = flags_ ? witness_of(r - 1) : true;
-        if (cand_.empty()) {
+        if (!ncand_) {
             // every triangle entering with x has boundary {x}: the first
             // ready one kills x's class as it is born
             if (flags_ ? !witness : leftover(a, b, D) < 0) {
                 const u32 c = create(r);
-                inc_[a].push_back({b, c});
-                inc_[b].push_back({a, c});
+                cur_ = 0;
+                link(c, a, b);
             }
             cur_ = 0;
             return;
         }
         const u32 xc = create(r);
         bool dirty = false;
-        for (u32 w : cand_) {
-            if (!boundary_class(head_[xc] ? xc + 1 : 0, seed_a_[w], seed_b_[w])) continue;
+        // The heads recorded at collection stay current until a resolve
+        // changes a class other than x's own (the usual resolve kills x's
+        // fresh generator and touches x's class alone).
+        bool recorded = true;
+        for (u32 t = 0; t < ncand_; ++t) {
+            const u32 w = cand_[t];
+            const u32 hx = head_[xc];
+            const u32 ha = recorded ? hseed_a_[w] : seed_a_[w] ? head_[seed_a_[w] - 1] : 0;
+            const u32 hb = recorded ? hseed_b_[w] : seed_b_[w] ? head_[seed_b_[w] - 1] : 0;
+            if (singles_cancel(hx, ha, hb)) continue;
+            if (!boundary_class(hx ? xc + 1 : 0, seed_a_[w], seed_b_[w])) continue;
             if (!ready(a, b, D, w)) continue;
             resolve(D, comb3(a, b, w), out);
             dirty = true;
+            if (!(touched_.size() == 1 && touched_[0] == xc)) recorded = false;
         }
         if (head_[xc] && witness) {
             if (flags_ && !dirty) {
@@ -514,11 +573,8 @@ private:
                 }
             }
         }
-        if (head_[xc]) {
-            inc_[a].push_back({b, xc});
-            inc_[b].push_back({a, xc});
-        }
         cur_ = 0;
+        if (head_[xc]) link(xc, a, b);
     }
 };
 
