@@ -233,6 +233,17 @@ def run_graph(args):
         tflops = pairs * (3 * d + 1) / per / 1e12
         out["roofline"] = {"bound": "valu", "achieved": round(tflops, 2), "peak": VALU_PEAK_TFLOPS,
                            "unit": "TFLOP/s", "frac": round(tflops / VALU_PEAK_TFLOPS, 4), "traffic": None}
+    if args.whole_run and w in ("c3a-knn", "c3b-radius"):
+        # the configuration's own persistence call end to end: the kNN graph
+        # complex at maxdim 2, the radius-graph complex at maxdim 1
+        from turbotops_amd import persistence
+        kw = dict(maxdim=2, graph="knn", k=15) if w == "c3a-knn" else dict(maxdim=1, threshold=0.15)
+        t0 = time.perf_counter()
+        dgms, info = persistence(x, metric="l2", **kw)
+        out["whole_run"] = {"seconds": round(time.perf_counter() - t0, 3), "maxdim": kw["maxdim"],
+                            "engine": info["engine"], "seconds_graph": round(info["seconds_graph"], 3),
+                            "seconds_reduce": round(info["seconds_reduce"], 3),
+                            "bars": [int(len(a)) for a in dgms]}
     if not args.no_cpu:
         from oracle import Reference, reference_available
         if reference_available(simd=True):

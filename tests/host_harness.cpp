@@ -40,12 +40,15 @@ int run(const Cx& cx, uint32_t maxdim, const uint64_t* idx, const float* d, cons
         Diagram out;
         out.ensure(maxdim);
         Telemetry tel;
+        // several threads, so the CPU tests also cover the parallel assembly
         if (index_fits<u64>(cx.n, maxdim)) {
             Cohomology<u64, Cx> r(cx, maxdim, out, tel);
+            r.set_threads(4);
             r.dim0(idx, d, ap, m);
             r.run_from_dim1();
         } else {
             Cohomology<u128, Cx> r(cx, maxdim, out, tel);
+            r.set_threads(4);
             r.dim0(idx, d, ap, m);
             r.run_from_dim1();
         }

@@ -152,3 +152,17 @@ def test_sparse_vs_reference(harness, restatement, reference, kind, n, dim, maxd
     got = canon(*sparse_diagram(harness, n, rp, col, w, maxdim))
     res, _ = reference.persistence(x, maxdim=maxdim, metric=metric, threshold=tau, mode="coboundary")
     assert got == canon(*res)
+
+
+def test_sparse_knn_graph_larger(harness, restatement, reference):
+    """A kNN-12 graph of 3000 points in [0,1]^8 at maxdim 2: many columns
+    per dimension, so later columns keep meeting earlier pivots (the stored
+    reduced coboundaries), the working columns run on the radix heap, and the
+    next dimension is assembled on four threads; bar for bar against the
+    compiled reference."""
+    x = np.random.default_rng(31).uniform(size=(3000, 8)).astype(np.float32)
+    rp, col, w = reference.knn_graph(x, "l2", 12)
+    got = canon(*sparse_diagram(harness, 3000, rp, col, w, 2))
+    res, _ = reference.persistence(x, maxdim=2, metric="l2", graph="knn", k=12, mode="coboundary")
+    assert got == canon(*res)
+    assert sum(1 for s in got if s.startswith("1 ")) > 1000

@@ -101,9 +101,10 @@ void reduce_boundary(u32 n, const f32* mat, u64 m, const u32* verts, const f32* 
 }
 
 // Runs the reducer from dimension 1 after dim0 on the given edges.
+// threads: host threads for the assembly of each next dimension's columns.
 template <class Cx>
 void reduce_all(const Cx& cx, u32 maxdim, const u64* idx, const f32* d, const u8* apparent, u64 m,
-                Barcode& out, Stats& st) {
+                Barcode& out, Stats& st, u32 threads) {
     const auto t0 = std::chrono::steady_clock::now();
     if (maxdim + 2 > 32) throw std::invalid_argument("rips: maxdim must be <= 30");
     out.ensure(maxdim);
@@ -111,11 +112,13 @@ void reduce_all(const Cx& cx, u32 maxdim, const u64* idx, const f32* d, const u8
     cob::Telemetry tel;
     if (cob::index_fits<u64>(cx.n, maxdim)) {
         cob::Cohomology<u64, Cx> r(cx, maxdim, out, tel);
+        r.set_threads(threads);
         r.dim0(idx, d, apparent, m);
         r.run_from_dim1();
     } else if (cob::index_fits<cob::u128>(cx.n, maxdim)) {
         tel.wide_index = true;
         cob::Cohomology<cob::u128, Cx> r(cx, maxdim, out, tel);
+        r.set_threads(threads);
         r.dim0(idx, d, apparent, m);
         r.run_from_dim1();
     } else {
@@ -305,7 +308,7 @@ Result persistence(const CsrGraph& g, u32 maxdim, f32 threshold, u32 threads, Mo
     cx.col = g.col_idx.data();
     cx.w = g.weights.data();
     const SparseEdgeList e(g, threshold, maxdim >= 1);
-    reduce_all(cx, maxdim, e.idx, e.d, e.apparent, e.m, r.barcode, r.stats);
+    reduce_all(cx, maxdim, e.idx, e.d, e.apparent, e.m, r.barcode, r.stats, num_threads());
     r.barcode.ensure(maxdim);
     r.stats.method = "sparse";
     r.stats.threshold = threshold;
@@ -380,7 +383,8 @@ Result persistence(const f32* points, u32 n, u32 dim, const Options& o) {
         cx.threshold = thr;
         cx.full = layout == TT_LAYOUT_FULL;
         cx.mat = mat.data();
-        reduce_all(cx, o.maxdim, idx.data(), d.data(), ap.empty() ? nullptr : ap.data(), m, r.barcode, st);
+        reduce_all(cx, o.maxdim, idx.data(), d.data(), ap.empty() ? nullptr : ap.data(), m, r.barcode, st,
+                   num_threads());
         st.method = "dense";
         st.threshold = thr;
     } else {
@@ -550,7 +554,7 @@ std::vector<Result> BatchRunner::run(const f32* pts, bool use_graph, u32 threads
                     cx.threshold = thr;
                     cx.full = true;
                     cx.mat = mat;
-                    reduce_all(cx, maxdim_, idx, d, nullptr, m, r.barcode, r.stats);
+                    reduce_all(cx, maxdim_, idx, d, nullptr, m, r.barcode, r.stats, 1);   // one cloud per thread
                     r.stats.engine = "coboundary";
                 }
                 r.barcode.ensure(maxdim_);

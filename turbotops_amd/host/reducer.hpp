@@ -29,6 +29,7 @@
 #include <cstring>
 #include <stdexcept>
 #include <string>
+#include <thread>
 #include <type_traits>
 #include <vector>
 
@@ -359,7 +360,9 @@ class RadixHeap {
 template <class I, class Cx>
 class Cohomology {
   public:
-    struct S {          // a simplex in the filtration
+    // A simplex in the filtration. Packed: 12 bytes with 64-bit indices, 20
+    // with 128-bit ones (32 unpacked), which the column heaps move around.
+    struct __attribute__((packed)) S {
         I idx;
         f32 d;
     };
@@ -368,6 +371,9 @@ class Cohomology {
 
     Cohomology(const Cx& cx, u32 maxdim, Diagram& out, Telemetry& tel)
         : cx_(cx), n_(cx.n), maxdim_(maxdim), B_(cx.n, maxdim + 2), out_(out), tel_(tel) {}
+
+    // host threads for assemble() (the columns of the next dimension)
+    void set_threads(u32 t) { threads_ = std::max<u32>(1, t); }
 
     // Dimension 0 from the GPU's edges (filtration order) and apparent flags.
     void dim0(const u64* idx, const f32* d, const u8* apparent, u64 m) {
@@ -563,11 +569,74 @@ class Cohomology {
         }
     };
 
+    // The working coboundary of reduce(): a binary heap, or, where every
+    // cofacet index fits 64 bits (so a simplex is one 128-bit order key), a
+    // radix heap. Its pivot only moves later in the filtration: an added
+    // column's coboundary has the current pivot as its own, so its entries
+    // before the pivot cancel among themselves and are not pushed.
+    bool radix_ = false;
+    RadixHeap col_;
+
+    void col_push(const S& c) {
+        if (!radix_) {
+            heap_.push(c);
+            return;
+        }
+        const u128 k = order_key(c);
+        if (k >= col_.last()) col_.push(k, 0);
+    }
+    void col_load(std::vector<S>& v) {
+        if (!radix_) {
+            heap_.load(v);
+            return;
+        }
+        col_.clear();
+        for (const S& c : v) col_.push(order_key(c), 0);
+    }
+    // the whole working column in filtration order after mod-2 cancellation,
+    // taken out of it
+    template <class F>
+    void col_drain(F&& f) {
+        if (!radix_) {
+            for (S p = heap_.pivot(); p.idx != kNone<I>; p = heap_.pivot()) {
+                heap_.pop_top();
+                f(p);
+            }
+            return;
+        }
+        while (!col_.empty()) {
+            const u128 k = col_.top_key();
+            u32 copies = 0;
+            while (col_.has_last()) {
+                col_.pop_last();
+                ++copies;
+            }
+            if (copies & 1) f(key_simplex(k));
+        }
+    }
+    // earliest surviving entry after mod-2 cancellation, left in the column
+    S col_pivot() {
+        if (!radix_) return heap_.pivot();
+        while (!col_.empty()) {
+            const u128 k = col_.top_key();
+            u32 copies = 0;
+            while (col_.has_last()) {
+                col_.pop_last();
+                ++copies;
+            }
+            if (copies & 1) {
+                col_.push(k, 0);
+                return key_simplex(k);
+            }
+        }
+        return S{kNone<I>, 0.0f};
+    }
+
     void push_coboundary(const S& s, u32 nv) {
         u32 vd[KM];
         decode(s.idx, nv, vd);
         cofacet_simplices(vd, nv, s.d, true, cx_.threshold, [&](const S& c, u32, u32) {
-            heap_.push(c);
+            col_push(c);
             return true;
         });
     }
@@ -899,11 +968,23 @@ class Cohomology {
     RadixHeap lheap_;
     PivotTable<I> zc_;
 
+    // What a finished column j keeps for the later columns that meet its
+    // pivot: nothing (an emergent pair: its coboundary is its own, enumerated
+    // again), its reduction (the simplices added to it: their coboundaries
+    // enumerated again), or, on sparse complexes, where coboundaries are
+    // short, its reduced coboundary itself (pushed as it is).
+    enum : u8 { KEEP_NONE, KEEP_REDUCTION, KEEP_COBOUNDARY };
+    std::vector<u8> keep_;
+
     void reduce(u32 dim) {
         const u32 nv = dim + 1;
+        // cofacets have nv + 1 vertices: indices below C(n, nv + 1)
+        radix_ = std::is_same_v<I, u64> || B_(n_, nv + 1) <= I(~u64(0));
+        constexpr bool store_cob = std::is_same_v<Cx, SparseView>;
         pivots_.reset(cols_.size());
         red_off_.assign(1, 0);
         red_.clear();
+        keep_.assign(cols_.size(), KEEP_NONE);
         u64 emergent = 0;
         std::vector<S> buf;
         std::vector<S> added;
@@ -931,8 +1012,8 @@ class Cohomology {
             if (fresh) {
                 ++emergent;
             } else {
-                heap_.load(buf);
-                piv = heap_.pivot();
+                col_load(buf);
+                piv = col_pivot();
             }
             for (;;) {
                 if (piv.idx == kNone<I>) {
@@ -945,29 +1026,40 @@ class Cohomology {
                 }
                 const u64 j = pivots_.find(piv.idx);
                 if (j != ~0ull) {
-                    added.push_back(cols_[j]);
-                    push_coboundary(cols_[j], nv);
-                    for (u64 t = red_off_[j]; t < red_off_[j + 1]; ++t) {
-                        added.push_back(red_[t]);
-                        push_coboundary(red_[t], nv);
+                    if (keep_[j] == KEEP_COBOUNDARY) {
+                        for (u64 t = red_off_[j]; t < red_off_[j + 1]; ++t) col_push(red_[t]);
+                    } else {
+                        added.push_back(cols_[j]);
+                        push_coboundary(cols_[j], nv);
+                        for (u64 t = red_off_[j]; t < red_off_[j + 1]; ++t) {
+                            added.push_back(red_[t]);
+                            push_coboundary(red_[t], nv);
+                        }
                     }
-                    piv = heap_.pivot();
+                    piv = col_pivot();
                     continue;
                 }
                 const S fa = apparent_facet_of(piv, nv + 1);
                 if (fa.idx != kNone<I>) {
                     added.push_back(fa);
                     push_coboundary(fa, nv);
-                    piv = heap_.pivot();
+                    piv = col_pivot();
                     continue;
                 }
                 if (piv.d > col.d) out_.add(dim, col.d, piv.d, u128(col.idx), u128(piv.idx));
                 pivots_.put(piv.idx, c);
-                // store the reduction column modulo 2
-                std::sort(added.begin(), added.end(), [](const S& a, const S& b) { return a.idx < b.idx; });
-                for (size_t x = 0; x < added.size();) {
-                    if (x + 1 < added.size() && added[x].idx == added[x + 1].idx) x += 2;
-                    else red_.push_back(added[x++]);
+                if constexpr (store_cob) {
+                    // the reduced coboundary: the pivot and what follows it
+                    keep_[c] = KEEP_COBOUNDARY;
+                    col_drain([&](const S& e) { red_.push_back(e); });
+                } else {
+                    // the reduction column modulo 2
+                    keep_[c] = KEEP_REDUCTION;
+                    std::sort(added.begin(), added.end(), [](const S& a, const S& b) { return a.idx < b.idx; });
+                    for (size_t x = 0; x < added.size();) {
+                        if (x + 1 < added.size() && added[x].idx == added[x + 1].idx) x += 2;
+                        else red_.push_back(added[x++]);
+                    }
                 }
                 break;
             }
@@ -981,34 +1073,59 @@ class Cohomology {
     void assemble(u32 dim) {
         const u32 nv = dim + 1;
         const bool keep = dim + 1 < maxdim_;
-        std::vector<S> next;
-        cols_.clear();
-        u64 skipped = 0;
-        u32 vd[KM], cv[KM + 1];
-        for (const S& s : simplices_) {
-            decode(s.idx, nv, vd);
-            cofacet_simplices(vd, nv, s.d, false, cx_.threshold, [&](const S& cf, u32 w, u32 t) {
-                if (keep) next.push_back(cf);
-                if (pivots_.has(cf.idx)) return true;          // cleared
-                for (u32 i = 0; i < t; ++i) cv[i] = vd[i];
-                cv[t] = w;
-                for (u32 i = t; i < nv; ++i) cv[i + 1] = vd[i];
-                if (apparent_cofacet_of(cf, cv, nv + 1).idx != kNone<I> ||
-                    apparent_facet_of(cf, cv, nv + 1).idx != kNone<I>) {
-                    ++skipped;
+        const u64 ns = simplices_.size();
+        // Contiguous ranges of the simplices, one per thread; the outputs are
+        // concatenated in range order, so columns and next simplices come out
+        // in the serial order.
+        const u32 nt = u32(std::max<u64>(1, std::min<u64>(threads_, ns / 4096)));
+        std::vector<std::vector<S>> cols(nt), nexts(nt);
+        std::vector<u64> skipped(nt, 0);
+        auto range = [&](u32 t) {
+            const u64 lo = ns * t / nt, hi = ns * (t + 1) / nt;
+            std::vector<S>& oc = cols[t];
+            std::vector<S>& on = nexts[t];
+            u32 vd[KM], cv[KM + 1];
+            for (u64 x = lo; x < hi; ++x) {
+                const S& s = simplices_[x];
+                decode(s.idx, nv, vd);
+                cofacet_simplices(vd, nv, s.d, false, cx_.threshold, [&](const S& cf, u32 w, u32 t2) {
+                    if (keep) on.push_back(cf);
+                    if (pivots_.has(cf.idx)) return true;          // cleared
+                    for (u32 i = 0; i < t2; ++i) cv[i] = vd[i];
+                    cv[t2] = w;
+                    for (u32 i = t2; i < nv; ++i) cv[i + 1] = vd[i];
+                    if (apparent_cofacet_of(cf, cv, nv + 1).idx != kNone<I> ||
+                        apparent_facet_of(cf, cv, nv + 1).idx != kNone<I>) {
+                        ++skipped[t];
+                        return true;
+                    }
+                    oc.push_back(cf);
                     return true;
-                }
-                cols_.push_back(cf);
-                return true;
-            });
+                });
+            }
+        };
+        if (nt == 1) {
+            range(0);
+        } else {
+            std::vector<std::thread> pool;
+            for (u32 t = 1; t < nt; ++t) pool.emplace_back(range, t);
+            range(0);
+            for (auto& th : pool) th.join();
         }
+        cols_.clear();
+        for (auto& v : cols) cols_.insert(cols_.end(), v.begin(), v.end());
+        std::vector<S> next;
+        for (auto& v : nexts) next.insert(next.end(), v.begin(), v.end());
         simplices_.swap(next);
         std::sort(cols_.begin(), cols_.end(), [](const S& a, const S& b) { return before(b, a); });
-        tel_.apparent.push_back(skipped);
+        u64 sk = 0;
+        for (u64 v : skipped) sk += v;
+        tel_.apparent.push_back(sk);
     }
 
     const Cx& cx_;
     const u32 n_, maxdim_;
+    u32 threads_ = 1;
     const Binomials<I> B_;
     Diagram& out_;
     Telemetry& tel_;
