@@ -10,6 +10,7 @@
 #include <chrono>
 #include <cstdlib>
 #include <cstring>
+#include <memory>
 #include <thread>
 
 #include "turbotops_gpu.hpp"
@@ -344,19 +345,22 @@ Result persistence(const f32* points, u32 n, u32 dim, const Options& o) {
         int layout = 0;
         check(tt_dense_layout(n, budget, &layout));
         if (layout == TT_LAYOUT_FULL && boundary_engine(o.mode, n, o.maxdim, true)) {
-            std::vector<f32> mat(u64(n) * n);
+            // the matrix is written whole by the library: no zero fill; the
+            // edge arrays are used where the library put them
+            std::unique_ptr<f32[]> mat(new f32[std::max<u64>(1, u64(n) * n)]);
             u64 m = 0;
             int32_t* pv = nullptr;
             f32* pd = nullptr;
             f32 thr = INFINITY;
             const auto tf = std::chrono::steady_clock::now();
-            check(tt_build_edges_witness(points, n, dim, cloud.metric.kind, cloud.metric.p, INFINITY,
-                                         TT_LAYOUT_FULL, mat.data(), &m, &pv, &pd, &thr));
+            const tt_status rc = tt_build_edges_witness(points, n, dim, cloud.metric.kind, cloud.metric.p, INFINITY,
+                                                        TT_LAYOUT_FULL, mat.get(), &m, &pv, &pd, &thr);
+            const std::unique_ptr<int32_t, void (*)(void*)> verts(pv, tt_free);
+            const std::unique_ptr<f32, void (*)(void*)> d(pd, tt_free);
+            check(rc);
             st.seconds_front = since(tf);
-            const std::vector<int32_t> verts = take(pv, 2 * m);
-            const std::vector<f32> d = take(pd, m);
-            reduce_boundary(n, mat.data(), m, reinterpret_cast<const u32*>(verts.data()), d.data(), true,
-                            r.barcode, st);
+            reduce_boundary(n, mat.get(), m, reinterpret_cast<const u32*>(verts.get()), d.get(), true, r.barcode,
+                            st);
             st.engine = "boundary";
             st.method = "dense-redzed";
             st.threshold = thr;
