@@ -332,7 +332,11 @@ def run_batched(args, metric="l2"):
     res = bs(sample)
     t_batch = time.perf_counter() - t0
     t0 = time.perf_counter()
-    plain = [persistence(c, maxdim=1, metric=metric)[0] for c in sample]
+    plain = []
+    for i, c in enumerate(sample):
+        plain.append(persistence(c, maxdim=1, metric=metric)[0])
+        if (i + 1) % 64 == 0:
+            print(f"bench: plain path {i + 1}/{len(sample)} clouds", file=sys.stderr, flush=True)
     t_plain = time.perf_counter() - t0
     same = all(all(np.array_equal(np.asarray(a), np.asarray(b)) for a, b in zip(r[0], p))
                for r, p in zip(res, plain))
