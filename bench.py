@@ -20,6 +20,7 @@ Prints ONE JSON line. Besides the contract keys it carries:
   batched         the 512-cloud batched front end (device time, GB/s), whole
                   runs of all 512 clouds (diagrams/sec), and the batched form
                   against the plain per-cloud path on the clouds with n <= 2000
+  batched_linf    the same set under L-inf: front end and one whole run
   differentiable  forward + backward of a top-8 H1 persistence loss on a
                   noisy circle (n = 2048), the backward kernel alone, and the
                   64-circle batch
@@ -349,6 +350,27 @@ def run_batched(args, metric="l2"):
     return out
 
 
+def run_batched_linf(args):
+    """The 512-cloud set under L-inf (the configuration's second metric):
+    front-end device time, the distance stage, and one whole run of every
+    cloud (front end + host reductions, all host threads)."""
+    from turbotops_amd import BatchPersistence
+    clouds = c4_clouds()
+    algo = float(sum(4.0 * c.shape[0] ** 2 + 4.0 * c.shape[0] * c.shape[1] for c in clouds))
+    bp = BatchPersistence([c.shape for c in clouds], metric="linf")
+    wall, dev_ms = _front_timing(bp, clouds)
+    dist_ms = bp.time_distance_stage()
+    t0 = time.perf_counter()
+    bp(clouds)
+    tf = time.perf_counter() - t0
+    out = {"front_end_device_ms": round(dev_ms, 3), "front_end_wall_ms": round(wall * 1e3, 3),
+           "distance_stage_ms": round(dist_ms, 3), "distance_stage_gbs": round(algo / (dist_ms * 1e-3) / 1e9, 1),
+           "edges_kept": int(bp.edges[0]), "rows_per_wave": bp.rows_per_wave,
+           "diagrams_per_sec_all_512": {"batched": round(len(clouds) / tf, 3), "whole_run_s": round(tf, 3)}}
+    del bp
+    return out
+
+
 def run_differentiable(args):
     """Forward + backward of -sum(death - birth) over the top-8 H1 bars on a
     noisy circle (radius 1, sigma 0.05, R^3, seed 7), L2; and the backward
@@ -446,6 +468,8 @@ def main():
     ap.add_argument("--c4-sample", type=int, default=512, help="most clouds in the batched vs plain sample")
     ap.add_argument("--no-c4-full", dest="c4_full", action="store_false",
                     help="skip the whole batched run of all 512 clouds")
+    ap.add_argument("--no-c4-linf", dest="c4_linf", action="store_false",
+                    help="skip the 512-cloud set under L-inf")
     ap.add_argument("--c5-n", type=int, default=2048, help="points of the differentiable circle")
     ap.add_argument("--c5-batch", type=int, default=64, help="clouds in the differentiable batch")
     ap.add_argument("--c5-batch-nmax", type=int, default=4096, help="largest n_i taken into that batch")
@@ -465,6 +489,9 @@ def main():
     if not args.no_batched:
         print("bench: batched leg", file=sys.stderr, flush=True)
         out["batched"] = run_batched(args)
+        if args.c4_linf:
+            print("bench: batched leg, L-inf", file=sys.stderr, flush=True)
+            out["batched_linf"] = run_batched_linf(args)
     if not args.no_diff:
         print("bench: differentiable leg", file=sys.stderr, flush=True)
         out["differentiable"] = run_differentiable(args)
