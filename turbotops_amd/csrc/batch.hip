@@ -42,7 +42,6 @@
 #include <cstring>
 #include <map>
 #include <mutex>
-#include <numeric>
 #include <vector>
 
 #include "apparent.hpp"
@@ -66,7 +65,6 @@ struct BatchShape {
     const u32* tile_cloud;  // total_tiles: the cloud of each tile
     const u32* row_cloud;   // total_rows: the cloud of each row
     u64 total_tiles, total_rows;
-    u64 tile_mul;           // coprime with total_tiles: the tile visiting order
 };
 
 // Row stride of a device matrix: n rounded up to 32 floats, so that every
@@ -81,11 +79,11 @@ __global__ __launch_bounds__(64 * (64 / RPW)) void batch_dist(BatchShape b, Metr
                                                               float* __restrict__ mats,
                                                               u32* __restrict__ rowmax) {
     __shared__ TileSmem<RPW> sm;
-    for (u64 s = blockIdx.x; s < b.total_tiles; s += gridDim.x) {
-        // visit the tiles in a scattered order, so the tiles in flight at one
-        // time belong to many clouds and the row-maximum atomics of a small
-        // cloud do not pile onto a few cache lines at once
-        const u64 t = (s * b.tile_mul) % b.total_tiles;
+    // Tiles in order: cloud by cloud, row-major lower triangle within one.
+    // (A scattered order, t * A mod T, meant to spread the row-maximum
+    // atomics of small clouds over more cache lines, measured 6% slower on
+    // the 512-cloud set: 3.01 against 2.83 ms.)
+    for (u64 t = blockIdx.x; t < b.total_tiles; t += gridDim.x) {
         const u32 c = __builtin_amdgcn_readfirstlane(b.tile_cloud[t]);
         const u32 n = b.n[c], dim = b.dim[c];
         const float* x = pts + b.pt_off[c];
@@ -461,15 +459,6 @@ struct tt_batch {
 
 namespace {
 
-// An odd multiplier near 0.618 T that is coprime with T (a permutation of
-// the tile ids when used as t * A mod T).
-u64 tile_stride(u64 T) {
-    if (T <= 2) return 1;
-    u64 a = u64(double(T) * 0.6180339887) | 1;
-    while (std::gcd(a, T) != 1) a += 2;
-    return a;
-}
-
 void batch_setup(BatchState& b, u32 nclouds, const uint32_t* n, const uint32_t* dim, int kind, float p,
                  float threshold) {
     check_metric(kind, p);
@@ -573,7 +562,7 @@ void batch_setup(BatchState& b, u32 nclouds, const uint32_t* n, const uint32_t* 
     TTG_HIP(hipStreamSynchronize(s));
     b.shape = BatchShape{nclouds, b.d_n, b.d_dim, b.d_pt_off, b.d_mat_off, b.d_row_base, b.d_tile_base,
                          b.d_pair_base, b.d_bmin_off, b.d_tile_cloud, b.d_row_cloud, b.total_tiles,
-                         b.total_rows, tile_stride(b.total_tiles)};
+                         b.total_rows};
     TTG_HIP(hipHostMalloc(reinterpret_cast<void**>(&b.h_pts), std::max<u64>(b.total_pts, 1) * 4, hipHostMallocDefault));
     TTG_HIP(hipHostMalloc(reinterpret_cast<void**>(&b.h_mats), std::max<u64>(b.total_mat, 1) * 4, hipHostMallocDefault));
     b.ev.resize(nclouds);
