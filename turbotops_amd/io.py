@@ -12,40 +12,80 @@ the other. Errors raise ValueError with the reference's message text.
   canonical order (dimension, birth, death).
 """
 import math
+import re
+from fractions import Fraction
 
 import numpy as np
 
 from .frontend import CsrGraph
 
+_U64 = re.compile(r"([+-]?)([0-9]+)")
+_U64_MAX = (1 << 64) - 1
+_SPACE = re.compile(r"[ \t\n\v\f\r]+")
+
 
 def _parse_f32(tok, what):
+    """The C library's strtof on a whole token: the decimal (or hexadecimal)
+    value rounded once to float32. Going through float64 first rounds twice,
+    which differs where the float64 lands exactly halfway between two
+    float32 values; that case is settled on the exact decimal value."""
     try:
-        v = np.float32(float(tok))
+        if "_" in tok or not tok.isascii():
+            raise ValueError
+        low = tok.lower().lstrip("+-")
+        d = float.fromhex(tok) if low.startswith("0x") else float(tok)
     except ValueError:
         raise ValueError(f"bad {what} '{tok}'") from None
-    return v
+    with np.errstate(over="ignore"):
+        f = np.float32(d)
+        g = np.nextafter(f, np.float32(math.copysign(math.inf, d - float(f)))) if np.isfinite(f) else f
+    if np.isfinite(f) and float(f) != d:
+        mid = (Fraction(float(f)) + Fraction(float(g))) / 2 if np.isfinite(g) else None
+        if mid is not None and Fraction(d) == mid and not low.startswith("0x"):
+            exact = Fraction(tok)
+            if exact != mid:
+                f = min(f, g) if exact < mid else max(f, g)
+    return f
 
 
 def _parse_u64(tok, what):
-    if not tok.isdigit():
+    """strtoull on a whole token: an optional sign and decimal digits; a
+    minus sign wraps modulo 2^64 and values past 2^64 - 1 are errors."""
+    m = _U64.fullmatch(tok)
+    if m is None or int(m.group(2)) > _U64_MAX:
         raise ValueError(f"bad {what} '{tok}'")
-    return int(tok)
+    v = int(m.group(2))
+    return (-v) % (_U64_MAX + 1) if m.group(1) == "-" else v
+
+
+def _lines(path):
+    """The file's lines, split at '\n' only, as getline does; bytes that are
+    not UTF-8 are kept (they fail the numeric conversion, as in C)."""
+    try:
+        with open(path, "rb") as fh:
+            raw = fh.read()
+    except OSError:
+        raise ValueError(f"cannot open {path}") from None
+    return raw, raw.decode("utf-8", "surrogateescape").split("\n")
+
+
+def _tokens(line, seps):
+    """Cut at '#', map each character of seps to a space and split at the C
+    library's white space (space, \\t, \\n, \\v, \\f, \\r)."""
+    line = line.split("#", 1)[0]
+    for ch in seps:
+        line = line.replace(ch, " ")
+    return [t for t in _SPACE.split(line) if t]
 
 
 def read_text_points(path):
     """read_text_points (src/turbotops.cpp:126-155) -> (n, dim) float32."""
-    try:
-        raw = open(path, "rb").read()
-    except OSError:
-        raise ValueError(f"cannot open {path}") from None
+    raw, lines = _lines(path)
     if raw.startswith(b"\xef\xbb\xbf"):
-        raw = raw[3:]
+        lines[0] = lines[0][1:]   # the BOM decodes to one character
     rows, dim = [], 0
-    for line in raw.decode("utf-8", "replace").splitlines():
-        line = line.split("#", 1)[0]
-        for ch in ",\t;":
-            line = line.replace(ch, " ")
-        toks = line.split()
+    for line in lines:
+        toks = _tokens(line, ",\t;")
         if not toks:
             continue
         vals = [_parse_f32(t, "coordinate") for t in toks]
@@ -61,10 +101,7 @@ def read_bin_points(path, dim):
     """read_bin_points (src/turbotops.cpp:157-169) -> (n, dim) float32."""
     if not dim:
         raise ValueError("--dim is required for .bin input")
-    try:
-        raw = open(path, "rb").read()
-    except OSError:
-        raise ValueError(f"cannot open {path}") from None
+    raw, _ = _lines(path)
     if len(raw) % (dim * 4):
         raise ValueError(f"{path}: size is not a multiple of 4 * dim")
     return np.frombuffer(raw, dtype="<f4").astype(np.float32).reshape(-1, dim)
@@ -72,14 +109,10 @@ def read_bin_points(path, dim):
 
 def read_edges(path):
     """read_edges (src/turbotops.cpp:171-203) -> CsrGraph."""
-    try:
-        lines = open(path).read().splitlines()
-    except OSError:
-        raise ValueError(f"cannot open {path}") from None
+    _, lines = _lines(path)
     rows = {}
     for lineno, line in enumerate(lines, 1):
-        line = line.split("#", 1)[0].replace(",", " ").replace("\t", " ")
-        toks = line.split()
+        toks = _tokens(line, ",\t")
         if not toks:
             continue
         if len(toks) < 3:
