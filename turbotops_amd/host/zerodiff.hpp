@@ -54,6 +54,9 @@
 #include <cstring>
 #include <utility>
 #include <vector>
+#if defined(__AVX2__)
+#include <immintrin.h>
+#endif
 
 #include "reducer.hpp"
 
@@ -196,7 +199,9 @@ public:
         hseed_a_.assign(n_, 0);
         hseed_b_.assign(n_, 0);
         cand_.assign(n_, 0);
+        stamp_.assign(n_, 0);
         ncand_ = 0;
+        epoch_ = 0;
         inc_.assign(n_, {});
         parent_.resize(n_);
         for (u32 v = 0; v < n_; ++v) parent_[v] = v;
@@ -249,11 +254,14 @@ private:
     const u32* verts_ = nullptr;
     const f32* diam_ = nullptr;
     std::vector<u64> adj_;                               // radius graph rows, W_ words each
-    std::vector<u64> seen_;                              // candidate vertices of the current edge
-    std::vector<u32> seed_a_, seed_b_;                   // 1 + class slot of edge (a, w) / (b, w)
-    std::vector<u32> hseed_a_, hseed_b_;                 // their class heads when collected
-    std::vector<u32> cand_;                              // candidates: cand_[0, ncand_)
+    std::vector<u64> seen_;                              // candidate vertices (for witness searches)
+    // candidates of the current edge, t < ncand_: the third vertex, 1 + class
+    // slot of edge (a, w) / (b, w) (0 = not active), their heads when collected
+    std::vector<u32> cand_, seed_a_, seed_b_, hseed_a_, hseed_b_;
     u32 ncand_ = 0;
+    // per vertex: (epoch << 32) | candidate index, for the current edge's epoch
+    std::vector<u64> stamp_;
+    u64 epoch_ = 0;
     // per vertex, its active edges: other end, class slot, and the class's
     // head word, kept current (set_head), so that collecting the candidates
     // reads the classes in order instead of one scattered load each
@@ -494,35 +502,69 @@ private:
         return !beta_.empty();
     }
 
+    // a ready triangle {a, b, w} without an active face other than (a, b):
+    // leftover() with the current edge's candidates masked out
+    int64_t unseeded_witness(u32 a, u32 b, f32 D) {
+        for (u32 t = 0; t < ncand_; ++t) seen_[cand_[t] >> 6] |= u64(1) << (cand_[t] & 63);
+        const int64_t w = leftover(a, b, D);
+        for (u32 t = 0; t < ncand_; ++t) seen_[cand_[t] >> 6] = 0;
+        return w;
+    }
+
+    // The first candidate t' >= t whose recorded heads and x's head hx do not
+    // cancel in pairs (ncand_ if none): almost every candidate cancels once
+    // the edge's first resolve has run, so they are tested 8 at a time.
+    u32 next_uncancelled(u32 t, u32 hx) const {
+#if defined(__AVX2__)
+        const __m256i X = _mm256_set1_epi32(int(hx)), Z = _mm256_setzero_si256(),
+                      M = _mm256_set1_epi32(int(MULTI));
+        const __m256i xz = _mm256_cmpeq_epi32(X, Z), xm = _mm256_cmpeq_epi32(X, M);
+        for (; t + 8 <= ncand_; t += 8) {
+            const __m256i ha = _mm256_loadu_si256(reinterpret_cast<const __m256i*>(&hseed_a_[t]));
+            const __m256i hb = _mm256_loadu_si256(reinterpret_cast<const __m256i*>(&hseed_b_[t]));
+            const __m256i pairs = _mm256_or_si256(
+                _mm256_or_si256(_mm256_and_si256(xz, _mm256_cmpeq_epi32(ha, hb)),
+                                _mm256_and_si256(_mm256_cmpeq_epi32(ha, Z), _mm256_cmpeq_epi32(X, hb))),
+                _mm256_and_si256(_mm256_cmpeq_epi32(hb, Z), _mm256_cmpeq_epi32(X, ha)));
+            const __m256i multi = _mm256_or_si256(xm, _mm256_or_si256(_mm256_cmpeq_epi32(ha, M),
+                                                                      _mm256_cmpeq_epi32(hb, M)));
+            const u32 keep = ~u32(_mm256_movemask_ps(_mm256_castsi256_ps(_mm256_andnot_si256(multi, pairs)))) & 0xffu;
+            if (keep) return t + u32(std::countr_zero(keep));
+        }
+#endif
+        for (; t < ncand_; ++t)
+            if (!singles_cancel(hx, hseed_a_[t], hseed_b_[t])) return t;
+        return ncand_;
+    }
+
     void cycle_edge(u32 r, u32 a, u32 b, f32 D, cob::Diagram& out) {
         advance(D);
-        for (u32 t = 0; t < ncand_; ++t) {
-            const u32 w = cand_[t];
-            seen_[w >> 6] = 0;
-            seed_a_[w] = seed_b_[w] = 0;
-        }
+        // candidates in scan order (b's active edges, then a's), written
+        // sequentially; a vertex met on both sides is found by its stamp
+        const u64 ep = ++epoch_ << 32;
         u32 k = 0;
         for (const Inc& e : inc_[b])
             if (e.w != a && adjacent(a, e.w, D)) {
-                const u32 w = e.w;
-                if (!((seen_[w >> 6] >> (w & 63)) & 1)) {
-                    seen_[w >> 6] |= u64(1) << (w & 63);
-                    cand_[k++] = w;
-                    hseed_a_[w] = 0;
-                }
-                seed_b_[w] = e.c + 1;
-                hseed_b_[w] = e.h;
+                stamp_[e.w] = ep | k;
+                cand_[k] = e.w;
+                seed_a_[k] = hseed_a_[k] = 0;
+                seed_b_[k] = e.c + 1;
+                hseed_b_[k] = e.h;
+                ++k;
             }
         for (const Inc& e : inc_[a])
             if (e.w != b && adjacent(b, e.w, D)) {
-                const u32 w = e.w;
-                if (!((seen_[w >> 6] >> (w & 63)) & 1)) {
-                    seen_[w >> 6] |= u64(1) << (w & 63);
-                    cand_[k++] = w;
-                    hseed_b_[w] = 0;
+                const u64 st = stamp_[e.w];
+                u32 t = k;
+                if ((st & ~u64(0xffffffffu)) == ep) {
+                    t = u32(st);
+                } else {
+                    cand_[k] = e.w;
+                    seed_b_[k] = hseed_b_[k] = 0;
+                    ++k;
                 }
-                seed_a_[w] = e.c + 1;
-                hseed_a_[w] = e.h;
+                seed_a_[t] = e.c + 1;
+                hseed_a_[t] = e.h;
             }
         ncand_ = k;
         cur_ = r;
@@ -546,12 +588,16 @@ private:
         // fresh generator and touches x's class alone).
         bool recorded = true;
         for (u32 t = 0; t < ncand_; ++t) {
+            if (recorded) {
+                t = next_uncancelled(t, head_[xc]);
+                if (t >= ncand_) break;
+            }
             const u32 w = cand_[t];
             const u32 hx = head_[xc];
-            const u32 ha = recorded ? hseed_a_[w] : seed_a_[w] ? head_[seed_a_[w] - 1] : 0;
-            const u32 hb = recorded ? hseed_b_[w] : seed_b_[w] ? head_[seed_b_[w] - 1] : 0;
+            const u32 ha = recorded ? hseed_a_[t] : seed_a_[t] ? head_[seed_a_[t] - 1] : 0;
+            const u32 hb = recorded ? hseed_b_[t] : seed_b_[t] ? head_[seed_b_[t] - 1] : 0;
             if (singles_cancel(hx, ha, hb)) continue;
-            if (!boundary_class(hx ? xc + 1 : 0, seed_a_[w], seed_b_[w])) continue;
+            if (!boundary_class(hx ? xc + 1 : 0, seed_a_[t], seed_b_[t])) continue;
             if (!ready(a, b, D, w)) continue;
             resolve(D, comb3(a, b, w), out);
             dirty = true;
@@ -566,7 +612,7 @@ private:
                 beta_ = cls_[xc];
                 resolve(D, 0, out);
             } else {
-                const int64_t w = leftover(a, b, D);
+                const int64_t w = unseeded_witness(a, b, D);
                 if (w >= 0) {
                     beta_ = cls_[xc];
                     resolve(D, comb3(a, b, u32(w)), out);
