@@ -110,6 +110,13 @@ void check_points(const float* pts, u64 count, const char* what);
 // std::max(m, v) == (m < v) ? v : m
 __device__ __forceinline__ float fmax_ref(float m, float v) { return (m < v) ? v : m; }
 
+// fmax_ref for a running maximum m that starts at +0 and is never NaN, over
+// terms v that are >= +0 or NaN (|a - b|): one v_max_f32. IEEE maxNum returns
+// m when v is a (quiet) NaN, as (m < v) ? v : m does, and the two agree on
+// every other pair (no -0 can occur); the compare-and-select form costs two
+// instructions per coordinate.
+__device__ __forceinline__ float fmax_acc(float m, float v) { return __builtin_fmaxf(m, v); }
+
 // kern::ipow (metric.hpp:176-181)
 __device__ __forceinline__ float ipow_ref(float x, u32 p) {
     float r = 1.0f;

@@ -45,7 +45,7 @@ __device__ __forceinline__ float pair_dist(const float* __restrict__ a, const fl
     } else {
         float m = 0.0f;                               // metric.hpp:172
 #pragma unroll
-        for (int k = 0; k < DIM; ++k) m = fmax_ref(m, __builtin_fabsf(a[k] - b[k]));
+        for (int k = 0; k < DIM; ++k) m = fmax_acc(m, __builtin_fabsf(a[k] - b[k]));
         if constexpr (V == MV_LINF) {
             return m;
         } else {                                      // metric.hpp:183-207
@@ -91,7 +91,7 @@ __device__ __forceinline__ void rows_runtime(const float* __restrict__ x, u32 n,
                     const float d = a[t] - b[t];
                     if constexpr (V == MV_L2) s = __builtin_fmaf(d, d, s);
                     else if constexpr (V == MV_L1) s += __builtin_fabsf(d);
-                    else s = fmax_ref(s, __builtin_fabsf(d));
+                    else s = fmax_acc(s, __builtin_fabsf(d));
                 }
             }
             acc[r] = s;
@@ -180,7 +180,7 @@ __device__ __forceinline__ float pdist(const float* __restrict__ a, const float*
     } else {
         float m = 0.0f;
 #pragma unroll
-        for (u32 k = 0; k < dim; ++k) m = fmax_ref(m, __builtin_fabsf(a[k] - b[k]));
+        for (u32 k = 0; k < dim; ++k) m = fmax_acc(m, __builtin_fabsf(a[k] - b[k]));
         if constexpr (V == MV_LINF) {
             return m;
         } else {
