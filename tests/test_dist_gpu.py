@@ -116,6 +116,22 @@ def test_denormals_and_overflow(gpu, restatement):
     assert got.reshape(3, 3)[0, 1] == np.float32(1e-40)   # a subnormal |difference| survives (no FTZ)
 
 
+@pytest.mark.parametrize("d", [3, 16, 21])
+def test_linf_nan_and_inf_terms(gpu, restatement, d):
+    """The L-inf fold runs as IEEE maxNum on the GPU and as the reference's
+    (m < v) ? v : m in the oracle: NaN terms (a NaN coordinate, inf - inf)
+    are skipped by both, inf terms win in both. d = 3 and 16 take the
+    compile-time paths, d = 21 the runtime-dimension one."""
+    rng = np.random.default_rng(40 + d)
+    x = rng.normal(size=(70, d)).astype(np.float32)
+    x[rng.integers(0, 70, 12), rng.integers(0, d, 12)] = np.nan
+    x[rng.integers(0, 70, 6), rng.integers(0, d, 6)] = np.inf
+    x[rng.integers(0, 70, 6), rng.integers(0, d, 6)] = -np.inf
+    want = restatement.make_dense(x, "linf", True)
+    got, _, _ = _dev("linf", x, "full")
+    assert np.array_equal(bits(got.reshape(70, 70)), bits(want))
+
+
 def test_ties_grid_linf_l1(gpu, restatement):
     x = _cloud(257, 3, seed=5, kind="grid")
     for metric in ["l1", "linf"]:
