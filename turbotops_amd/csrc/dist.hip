@@ -191,7 +191,7 @@ tt_status tt_make_dense(const float* pts, uint32_t n, uint32_t dim, int kind, fl
         if (dx.n) TTG_HIP(hipMemcpy(dx.p, pts, dx.n * sizeof(float), hipMemcpyHostToDevice));
         dist_matrix(dx.p, n, dim, kind, p, layout, dm.p, radius ? rowmax.p : nullptr,
                     radius ? dr.p : nullptr, s);
-        if (cells) TTG_HIP(hipMemcpy(mat, dm.p, cells * sizeof(float), hipMemcpyDeviceToHost));
+        copy_to_host(mat, dm.p, cells * sizeof(float), s);
         if (radius) TTG_HIP(hipMemcpy(radius, dr.p, sizeof(float), hipMemcpyDeviceToHost));
     });
 }

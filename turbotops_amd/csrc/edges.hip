@@ -329,9 +329,9 @@ tt_status tt_dense_edges(const float* pts, uint32_t n, uint32_t dim, int kind, f
         if (apparent) *apparent = static_cast<uint8_t*>(std::malloc(std::max<u64>(1, e.m)));
         if (!*idx || !*d || (apparent && !*apparent)) fail(TT_EDEVICE, "out of host memory");
         if (e.m) {
-            TTG_HIP(hipMemcpyAsync(*idx, e.idx, e.m * sizeof(u64), hipMemcpyDeviceToHost, s));
-            TTG_HIP(hipMemcpyAsync(*d, e.d, e.m * sizeof(float), hipMemcpyDeviceToHost, s));
-            if (apparent) TTG_HIP(hipMemcpyAsync(*apparent, e.apparent, e.m, hipMemcpyDeviceToHost, s));
+            copy_to_host(*idx, e.idx, e.m * sizeof(u64), s);
+            copy_to_host(*d, e.d, e.m * sizeof(float), s);
+            if (apparent) copy_to_host(*apparent, e.apparent, e.m, s);
         }
         TTG_HIP(hipStreamSynchronize(s));
     });
@@ -392,12 +392,12 @@ extern "C" tt_status tt_dense_filtration(const float* pts, uint32_t n, uint32_t 
         if (mat) {
             if (layout == TT_LAYOUT_FULL) {
                 if (u64(n) * n)
-                    TTG_HIP(hipMemcpyAsync(mat, dm, u64(n) * n * sizeof(float), hipMemcpyDeviceToHost, s));
+                    copy_to_host(mat, dm, u64(n) * n * sizeof(float), s);
             } else {
                 const u64 cells = u64(n) * (n ? n - 1 : 0) / 2;
                 float* low = ws.get<float>(cells);
                 dist_matrix(dx, n, dim, kind, p, TT_LAYOUT_LOWER, low, nullptr, nullptr, s);
-                if (cells) TTG_HIP(hipMemcpyAsync(mat, low, cells * sizeof(float), hipMemcpyDeviceToHost, s));
+                if (cells) copy_to_host(mat, low, cells * sizeof(float), s);
             }
         }
         *m = e.m;
@@ -406,9 +406,9 @@ extern "C" tt_status tt_dense_filtration(const float* pts, uint32_t n, uint32_t 
         if (apparent) *apparent = static_cast<uint8_t*>(std::malloc(std::max<u64>(1, e.m)));
         if (!*idx || !*d || (apparent && !*apparent)) fail(TT_EDEVICE, "out of host memory");
         if (e.m) {
-            TTG_HIP(hipMemcpyAsync(*idx, e.idx, e.m * sizeof(u64), hipMemcpyDeviceToHost, s));
-            TTG_HIP(hipMemcpyAsync(*d, e.d, e.m * sizeof(float), hipMemcpyDeviceToHost, s));
-            if (apparent) TTG_HIP(hipMemcpyAsync(*apparent, e.apparent, e.m, hipMemcpyDeviceToHost, s));
+            copy_to_host(*idx, e.idx, e.m * sizeof(u64), s);
+            copy_to_host(*d, e.d, e.m * sizeof(float), s);
+            if (apparent) copy_to_host(*apparent, e.apparent, e.m, s);
         }
         TTG_HIP(hipStreamSynchronize(s));
     });
@@ -456,12 +456,12 @@ tt_status build_edges_impl(const float* pts, uint32_t n, uint32_t dim, int kind,
         if (mat) {
             if (layout == TT_LAYOUT_FULL) {
                 if (u64(n) * n)
-                    TTG_HIP(hipMemcpyAsync(mat, dm, u64(n) * n * sizeof(float), hipMemcpyDeviceToHost, s));
+                    copy_to_host(mat, dm, u64(n) * n * sizeof(float), s);
             } else {
                 const u64 cells = u64(n) * (n ? n - 1 : 0) / 2;
                 float* low = ws.get<float>(cells);
                 dist_matrix(dx, n, dim, kind, p, TT_LAYOUT_LOWER, low, nullptr, nullptr, s);
-                if (cells) TTG_HIP(hipMemcpyAsync(mat, low, cells * sizeof(float), hipMemcpyDeviceToHost, s));
+                if (cells) copy_to_host(mat, low, cells * sizeof(float), s);
             }
         }
         *m = e.m;
@@ -469,8 +469,8 @@ tt_status build_edges_impl(const float* pts, uint32_t n, uint32_t dim, int kind,
         *diam = static_cast<float*>(std::malloc(std::max<u64>(1, e.m) * sizeof(float)));
         if (!*verts || !*diam) fail(TT_EDEVICE, "out of host memory");
         if (e.m) {
-            TTG_HIP(hipMemcpyAsync(*verts, vd, 2 * e.m * sizeof(int32_t), hipMemcpyDeviceToHost, s));
-            TTG_HIP(hipMemcpyAsync(*diam, e.d, e.m * sizeof(float), hipMemcpyDeviceToHost, s));
+            copy_to_host(*verts, vd, 2 * e.m * sizeof(int32_t), s);
+            copy_to_host(*diam, e.d, e.m * sizeof(float), s);
         }
         TTG_HIP(hipStreamSynchronize(s));
     });

@@ -14,6 +14,10 @@ void fill_device_u32(u32* p, u64 count, u32 v, hipStream_t s);
 int tile_rows_per_wave();
 void set_tile_rows_per_wave(int rpw);
 
+// hostcopy.hip: device -> pageable host memory, staged through pinned
+// buffers for large sizes; returns when the data is in dst
+void copy_to_host(void* dst, const void* src, u64 bytes, hipStream_t s);
+
 // edges.hip
 struct DenseEdges {
     u64 m = 0;
