@@ -116,6 +116,23 @@ def test_denormals_and_overflow(gpu, restatement):
     assert got.reshape(3, 3)[0, 1] == np.float32(1e-40)   # a subnormal |difference| survives (no FTZ)
 
 
+@pytest.mark.parametrize("n", [4099, 4500])
+def test_host_matrix_staged_copy(gpu, restatement, n):
+    """Host matrices of 32 MiB and more come back through the pinned staging
+    buffers (64 MiB chunks): n = 4099 is one full chunk and 67 KB more,
+    n = 4500 one full chunk and a larger partial one; the lower layout of
+    n = 4500 (40.5 MB) is one partial chunk."""
+    from turbotops_amd import make_dense
+    x = _cloud(n, 3, seed=n)
+    full, mat, r = make_dense(x, "l2", with_radius=True)
+    want = restatement.make_dense(x, "l2", True)
+    assert full and np.array_equal(bits(mat.reshape(-1)), bits(want.reshape(-1)))
+    assert bits(np.float32(r)) == bits(restatement.enclosing_radius(want, n))
+    if n == 4500:
+        full, low, _ = make_dense(x, "l2", budget=2 * n * (n - 1), with_radius=False)
+        assert not full and np.array_equal(bits(low), bits(restatement.make_dense(x, "l2", False)))
+
+
 @pytest.mark.parametrize("d", [3, 16, 21])
 def test_linf_nan_and_inf_terms(gpu, restatement, d):
     """The L-inf fold runs as IEEE maxNum on the GPU and as the reference's
