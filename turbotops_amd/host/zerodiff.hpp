@@ -216,7 +216,16 @@ public:
         gen_.clear();
         gen_free_.clear();
         rptr_ = resolved_ = 0;
+        linked_ = 0;
         for (u64 e = 0; e < m; ++e) {
+            if (flags_ && !linked_) {
+                // No edge is active, so a flagged edge has no candidates and
+                // its class dies as it is born; nor is it a tree edge (the
+                // other two edges of its witness triangle come before it).
+                // Skip such edges reading only their flag bits.
+                while (e < m && (verts[2 * e + 1] >> 31)) ++e;
+                if (e == m) break;
+            }
             const u32 a = verts[2 * e], b = hi_of(e);
             const f32 d = diam[e];
             u32 ra = find(a), rb = find(b);
@@ -248,6 +257,7 @@ private:
     const f32* mat_;
     u64 ld_;
     u64 m_ = 0, rptr_ = 0, resolved_ = 0;
+    u64 linked_ = 0;                     // active edges (classes linked into inc_)
     bool flags_ = false, bits_ = true;   // witness flags given; radius graph kept
     u32 win_edges_ = 0, win_cand_ = 0;   // cycle edges / witness searches, this window
     u32 quiet_ = 0;                      // quiet windows in a row with the radius graph on
@@ -393,6 +403,7 @@ private:
         }
     }
     void link(u32 c, u32 lo, u32 hi) {
+        ++linked_;
         pos_lo_[c] = u32(inc_[lo].size());
         inc_[lo].push_back({hi, c, head_[c]});
         pos_hi_[c] = u32(inc_[hi].size());
@@ -434,6 +445,7 @@ private:
         head_[c] = 0;
         cls_free_.push_back(c);
         if (z != cur_) {
+            --linked_;
             unlink_at(verts_[2 * (z - 1)], pos_lo_[c]);
             unlink_at(hi_of(z - 1), pos_hi_[c]);
         }
