@@ -287,7 +287,8 @@ def run_batched(args, metric="l2"):
     front end: all 512 clouds through the recorded launch sequence (distances,
     enclosing radii, the edge sort, copies to host memory), timed with the
     library's HIP events on its own stream, in the boundary engine's edge
-    order (the default: every kept edge by (diameter, comb index)) and in the
+    order (the default: every kept edge by (diameter, comb index), with its
+    ready-witness flag) and in the
     cohomology engine's (apparent test, then the non-apparent edges sorted);
     algorithmic bytes of the distance stage are sum(4 n_i^2 + 4 n_i d_i).
     diagrams/sec: whole runs (front end + host reductions) of the batched
@@ -306,7 +307,8 @@ def run_batched(args, metric="l2"):
     out = {"workload": f"512 clouds, n_i ~ U{{200..4000}}, d_i in {{2,3,8}}, circles/spheres sigma 0.02, seed 6, "
                        f"{metric}, maxdim 1",
            "front_end_device_ms": round(dev_ms, 3), "front_end_wall_ms": round(wall * 1e3, 3),
-           "front_end_edge_order": "boundary engine: every kept edge by (diameter, comb index)",
+           "front_end_edge_order": "boundary engine: every kept edge by (diameter, comb index), "
+                                   "with its ready-witness flag",
            "distance_stage_bytes": algo, "distance_stage_ms": round(dist_ms, 3),
            "distance_stage_gbs": round(algo / (dist_ms * 1e-3) / 1e9, 1),
            "distance_stage_frac": round(algo / (dist_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),

@@ -222,7 +222,7 @@ PYBIND11_MODULE(_turbotops, m) {
                  return std::make_unique<BatchRunner>(n, dim, metric, maxdim, parse_mode(mode), witness_flags);
              }),
              py::arg("n"), py::arg("dim"), py::arg("metric") = "l2", py::arg("maxdim") = 1,
-             py::arg("mode") = "auto", py::arg("witness_flags") = false)
+             py::arg("mode") = "auto", py::arg("witness_flags") = true)
         .def("front", &batch_front, py::arg("clouds"), py::arg("use_graph") = true)
         .def("run", &batch_run, py::arg("clouds"), py::arg("use_graph") = true, py::arg("threads") = 0,
              py::arg("pairs") = false)

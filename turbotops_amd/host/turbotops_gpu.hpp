@@ -177,9 +177,10 @@ class BatchRunner {
     // edge in (diameter, comb index) order), Coboundary with the cohomology
     // engine (the non-apparent edges in its order). witness_flags: the
     // boundary engine's inputs carry the GPU's ready-witness flags (faster
-    // host reductions on clouds without long cycles, one more device scan).
+    // host reductions on clouds without long cycles, one more device scan;
+    // used where the flag bit fits the batch's edge keys).
     BatchRunner(const std::vector<u32>& n, const std::vector<u32>& dim, const std::string& metric,
-                u32 maxdim = 1, Mode mode = Mode::Auto, bool witness_flags = false);
+                u32 maxdim = 1, Mode mode = Mode::Auto, bool witness_flags = true);
     ~BatchRunner();
     BatchRunner(const BatchRunner&) = delete;
     BatchRunner& operator=(const BatchRunner&) = delete;
