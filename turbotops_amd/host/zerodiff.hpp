@@ -227,6 +227,12 @@ public:
                 if (e == m) break;
             }
             const u32 a = verts[2 * e], b = hi_of(e);
+            // the same for one flagged edge whose two ends have no active edge
+            // (still counted in the radius graph's switching window)
+            if (flags_ && (verts[2 * e + 1] >> 31) && inc_[a].empty() && inc_[b].empty()) {
+                note_edge(diam[e]);
+                continue;
+            }
             const f32 d = diam[e];
             u32 ra = find(a), rb = find(b);
             if (ra != rb) {
