@@ -155,6 +155,28 @@ inline void xor_into(std::vector<u32>& x, const std::vector<u32>& v, std::vector
         else x.insert(it, v[0]);
         return;
     }
+    if (v.size() * 16 <= x.size()) {
+        // a few elements into a large set (a generator's set of class slots
+        // holds thousands on 16-D clouds): find each by binary search and
+        // copy the runs between them whole
+        tmp.resize(x.size() + v.size());
+        u32* out = tmp.data();
+        const u32* base = x.data();
+        size_t i = 0;
+        for (u32 y : v) {
+            const size_t p = size_t(std::lower_bound(x.begin() + i, x.end(), y) - x.begin());
+            std::memcpy(out, base + i, (p - i) * sizeof(u32));
+            out += p - i;
+            i = p;
+            if (i < x.size() && base[i] == y) ++i;   // present: it cancels
+            else *out++ = y;                         // absent: it joins
+        }
+        std::memcpy(out, base + i, (x.size() - i) * sizeof(u32));
+        out += x.size() - i;
+        tmp.resize(size_t(out - tmp.data()));
+        x.swap(tmp);
+        return;
+    }
     xor_sorted(x, v, tmp);
     x.swap(tmp);
 }
