@@ -237,6 +237,7 @@ public:
         cls_free_.clear();
         gen_.clear();
         gen_free_.clear();
+        gcnt_.clear();
         rptr_ = resolved_ = 0;
         linked_ = 0;
         for (u64 e = 0; e < m; ++e) {
@@ -313,9 +314,14 @@ private:
     // when empty, MULTI when larger), and the edge's rank
     std::vector<std::vector<u32>> cls_;
     std::vector<u32> head_, owner_, cls_free_;
-    // G sets (class slots holding a generator), by slot; G_: generator -> 1 + slot
+    // G sets (class slots holding a generator), by slot; G_: generator -> 1 + slot.
+    // A G set is kept as a list that only grows between compactions: a slot
+    // is appended when its class gains the generator and left in place when
+    // the class loses it; the class's own sorted set decides membership, and
+    // gcnt_ counts the true members. (Summing into sorted G sets copied
+    // thousands of slots per resolve on 16-D clouds.)
     std::vector<std::vector<u32>> gen_;
-    std::vector<u32> gen_free_;
+    std::vector<u32> gen_free_, gcnt_, gys_;
     RankMap G_;
     std::vector<u32> beta_, touched_, tmp_;
     u32 cur_ = 0;                                        // the edge being processed (not linked yet)
@@ -452,7 +458,9 @@ private:
         head_[c] = x;
         owner_[c] = x;
         const u32 g = take(gen_, gen_free_);
+        if (gcnt_.size() < gen_.size()) gcnt_.resize(gen_.size());
         gen_[g].assign(1, c);
+        gcnt_[g] = 1;
         G_.put(x, g + 1);
         return c;
     }
@@ -480,23 +488,47 @@ private:
     }
     // beta_ (nonzero boundary class of a triangle of diameter D): its
     // youngest generator dies
+    bool holds(u32 c, u32 y) const { return std::binary_search(cls_[c].begin(), cls_[c].end(), y); }
+    // the live members of generator y's list, sorted, without repeats
+    void members(u32 g, u32 y, std::vector<u32>& out) const {
+        out.clear();
+        for (u32 c : gen_[g])
+            if (holds(c, y)) out.push_back(c);
+        std::sort(out.begin(), out.end());
+        out.erase(std::unique(out.begin(), out.end()), out.end());
+    }
     void resolve(f32 D, u128 tri, cob::Diagram& out) {
         ++resolved_;
         const u32 j = beta_.back();
         const u32 gj = G_.get(j) - 1;
-        touched_ = gen_[gj];
+        members(gj, j, touched_);
+        const size_t others = beta_.size() - 1;
+        gys_.resize(others);
+        for (size_t t = 0; t < others; ++t) gys_[t] = G_.get(beta_[t]) - 1;
         for (u32 c : touched_) {
             xor_into(cls_[c], beta_, tmp_);
+            // the class drops j and toggles each other generator of beta
+            for (size_t t = 0; t < others; ++t) {
+                const u32 gy = gys_[t];
+                if (holds(c, beta_[t])) {
+                    gen_[gy].push_back(c);
+                    ++gcnt_[gy];
+                } else {
+                    --gcnt_[gy];
+                }
+            }
             set_head(c);
             if (cls_[c].empty()) drop_class(c);
         }
-        for (size_t t = 0; t + 1 < beta_.size(); ++t) {
-            const u32 y = beta_[t];
-            const u32 gy = G_.get(y) - 1;
-            xor_into(gen_[gy], touched_, tmp_);
-            if (gen_[gy].empty()) {
+        for (size_t t = 0; t < others; ++t) {
+            const u32 y = beta_[t], gy = gys_[t];
+            if (!gcnt_[gy]) {
+                gen_[gy].clear();
                 gen_free_.push_back(gy);
                 G_.erase(y);
+            } else if (gen_[gy].size() > 2 * size_t(gcnt_[gy]) + 16) {
+                members(gy, y, tmp_);
+                gen_[gy].swap(tmp_);
             }
         }
         gen_[gj].clear();
