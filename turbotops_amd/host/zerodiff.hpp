@@ -579,7 +579,9 @@ private:
     int64_t unseeded_witness(u32 a, u32 b, f32 D) {
         for (u32 t = 0; t < ncand_; ++t) seen_[cand_[t] >> 6] |= u64(1) << (cand_[t] & 63);
         const int64_t w = leftover(a, b, D);
-        for (u32 t = 0; t < ncand_; ++t) seen_[cand_[t] >> 6] = 0;
+        // clear: word by word when the candidates are many, else each one's word
+        if (u64(ncand_) * 4 >= W_) std::memset(seen_.data(), 0, W_ * sizeof(u64));
+        else for (u32 t = 0; t < ncand_; ++t) seen_[cand_[t] >> 6] = 0;
         return w;
     }
 
