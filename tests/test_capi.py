@@ -54,3 +54,18 @@ def test_errors_are_thread_local_and_status_mapped():
     rc = lib.dll.tt_metric_parse(b"wat", C.byref(k), C.byref(p))
     assert rc == capi.TT_EINVAL
     assert "unknown 'wat'" in lib.last_error()
+
+
+def test_backward_host_form_rejects_out_of_range_endpoint():
+    """tt_diagram_backward checks every endpoint on the host before any
+    device work: an endpoint >= n is TT_EINVAL."""
+    import numpy as np
+    lib = capi.lib()
+    x = np.zeros((4, 2), dtype=np.float32)
+    edges = np.array([0, 1, 2, 4], dtype=np.uint32)   # second edge ends at 4 == n
+    g = np.ones(2, dtype=np.float32)
+    dx = np.zeros_like(x)
+    rc = lib.dll.tt_diagram_backward(x.ctypes.data, 4, 2, capi.METRIC_L2, 2.0, edges.ctypes.data, g.ctypes.data, 2,
+                                     dx.ctypes.data)
+    assert rc == capi.TT_EINVAL
+    assert lib.last_error() == "tt_diagram_backward: edge endpoint out of range"

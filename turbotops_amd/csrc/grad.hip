@@ -24,16 +24,18 @@
 namespace ttg {
 namespace {
 
-__global__ void grad_keys(const u32* __restrict__ edges, u32 m, int cb, u64* __restrict__ keys) {
+// An endpoint >= n is clamped to n: such keys sort after every real point,
+// no segment start is recorded for them, and their edges contribute nothing.
+__global__ void grad_keys(const u32* __restrict__ edges, u32 m, u32 n, int cb, u64* __restrict__ keys) {
     for (u32 t = blockIdx.x * blockDim.x + threadIdx.x; t < 2 * m; t += gridDim.x * blockDim.x)
-        keys[t] = (u64(edges[t]) << cb) | t;   // edges[2e + side] is the endpoint
+        keys[t] = (u64(min(edges[t], n)) << cb) | t;   // edges[2e + side] is the endpoint
 }
 
-__global__ void grad_count(const u64* __restrict__ keys, u32 total, int cb, u32* __restrict__ start) {
+__global__ void grad_count(const u64* __restrict__ keys, u32 total, u32 n, int cb, u32* __restrict__ start) {
     // start[p] = first sorted slot of point p (segment starts, rest untouched)
     for (u32 t = blockIdx.x * blockDim.x + threadIdx.x; t < total; t += gridDim.x * blockDim.x) {
         const u32 p = u32(keys[t] >> cb);
-        if (t == 0 || u32(keys[t - 1] >> cb) != p) start[p] = t;
+        if (p < n && (t == 0 || u32(keys[t - 1] >> cb) != p)) start[p] = t;
     }
 }
 
@@ -84,7 +86,7 @@ __global__ void grad_gather(const float* __restrict__ x, u32 n, u32 dim, int kin
             const u32 slot = u32(keys[t] & mask);
             const u32 e = slot >> 1, side = slot & 1;
             const u32 a = edges[2 * e + side], b = edges[2 * e + (side ^ 1)];
-            if (a == b) continue;
+            if (a == b || b >= n) continue;   // a == pnt < n
             acc += double(g[e]) * ddist(kind, double(p), x + u64(a) * dim, x + u64(b) * dim, dim, k);
         }
         dx[c] = float(acc);
@@ -100,17 +102,18 @@ void diagram_backward(const float* d_x, u32 n, u32 dim, int kind, float p, const
     fill_device_u32(reinterpret_cast<u32*>(d_dx), cells, 0u, s);
     if (!m || !cells) return;
     const int cb = std::max(1, bit_length(2 * u64(m) - 1));
-    const int pb = std::max(1, bit_length(u64(n) - 1));
+    const int pb = std::max(1, bit_length(u64(n)));   // endpoints clamped to n
     if (cb + pb > 64) fail(TT_ELENGTH, "diagram_backward: too many contributions");
     u64* keys = ws.get<u64>(2 * u64(m));
     u64* alt = ws.get<u64>(2 * u64(m));
     u32* start = ws.get<u32>(n);
     fill_device_u32(start, n, 0xffffffffu, s);
     const u32 blocks = u32(std::min<u64>((2 * u64(m) + 255) / 256, 4096));
-    grad_keys<<<blocks, 256, 0, s>>>(d_edges, m, cb, keys);
+    grad_keys<<<blocks, 256, 0, s>>>(d_edges, m, n, cb, keys);
     TTG_LAUNCH_CHECK();
     radix_sort_u64(keys, nullptr, 2 * u64(m), 0, cb + pb, alt, nullptr, ws, s);
-    grad_count<<<blocks, 256, 0, s>>>(keys, 2 * m, cb, start);
+    grad_count<<<blocks, 256, 0, s>>>(keys, 2 * m, n, cb, start);
+    TTG_LAUNCH_CHECK();
     const u32 gblocks = u32(std::min<u64>((cells + 255) / 256, 16384));
     grad_gather<<<gblocks, 256, 0, s>>>(d_x, n, dim, kind, p, d_edges, d_g, keys, 2 * m, cb, start, d_dx);
     TTG_LAUNCH_CHECK();
