@@ -123,6 +123,16 @@ tt_status tt_mst_deaths(const float* pts, uint32_t n, uint32_t dim, int kind, fl
 tt_status tt_mst_deaths_dev(const float* d_pts, uint32_t n, uint32_t dim, int kind, float p,
                             float* d_deaths, uint32_t* rounds, uint64_t* scans, void* stream);
 
+/* tt_mst_deaths_dev without any device allocation. No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, serving loops, harnesses that forbid hipMalloc).
+ * Every temporary lives in one static 56 MiB scratch region per device, held
+ * until the process ends; n <= 1048576 (TT_ELENGTH beyond). The stream must
+ * belong to the current device. Calls that overlap on one device race on
+ * that region: serialize them. Same results as tt_mst_deaths_dev. */
+tt_status tt_mst_deaths_dev_noalloc(const float* d_pts, uint32_t n, uint32_t dim, int kind, float p,
+                                    float* d_deaths, void* stream);
+
 /* ---- dimension 0 of the coboundary reducer -------------------------------- */
 
 /* Reducer::dim0's kept edges and sort (rips.hpp:685-839) and its parallel
@@ -269,7 +279,10 @@ tt_status tt_diagram_backward(const float* pts, uint32_t n, uint32_t dim, int ki
                               const uint32_t* edges, const float* grad, uint32_t m, float* dx);
 
 /* Device form (HBM pointers, stream as hipStream_t); returns after the
- * stream has finished the backward. */
+ * stream has finished the backward. The endpoints are not checked on the
+ * host here: an edge with an endpoint >= n contributes nothing to either end
+ * (the host form rejects it with TT_EINVAL "tt_diagram_backward: edge
+ * endpoint out of range"). */
 tt_status tt_diagram_backward_dev(const float* d_pts, uint32_t n, uint32_t dim, int kind, float p,
                                   const uint32_t* d_edges, const float* d_grad, uint32_t m,
                                   float* d_dx, void* stream);

@@ -44,6 +44,7 @@ _SIGS = {
                            None),
     "tt_mst_deaths": ([_vp, C.c_uint32, C.c_uint32, C.c_int, C.c_float, _vp], None),
     "tt_mst_deaths_dev": ([_vp, C.c_uint32, C.c_uint32, C.c_int, C.c_float, _vp, _u32p, _u64p, _vp], None),
+    "tt_mst_deaths_dev_noalloc": ([_vp, C.c_uint32, C.c_uint32, C.c_int, C.c_float, _vp, _vp], None),
     "tt_dense_edges": ([_vp, C.c_uint32, C.c_uint32, C.c_int, C.c_float, C.c_float, _u64p, _vp, _vp, _vp,
                         _f32p], None),
     "tt_dense_edges_dev": ([_vp, C.c_uint32, C.c_float, C.c_int, C.c_uint64, _u64p, _vp, _vp, _vp, _vp], None),

@@ -24,6 +24,8 @@
 //               recorded once
 //   jump        pointer jumping until every label names its root
 #include <algorithm>
+#include <mutex>
+#include <string>
 #include <vector>
 
 #include "pairs.hpp"
@@ -185,27 +187,76 @@ struct MstStats {
     u64 scanned = 0;    // vertex scans (each n - 1 candidates)
 };
 
-// Sorted MST edge weights (n - 1 floats, device) of the complete graph.
-void mst_deaths_dev(const float* d_x, u32 n, u32 dim, int kind, float p, float* d_deaths,
-                    Workspace& ws, hipStream_t s, MstStats* st) {
+namespace {
+
+// Sub-buffers of one device region, handed out in order at 256-byte
+// alignment. With base == nullptr it only measures (the bytes a run needs).
+struct Carve {
+    char* base = nullptr;
+    u64 cap = ~0ull, used = 0;
+    template <typename T>
+    T* get(u64 count) {
+        const u64 off = (used + 255) & ~u64(255);
+        const u64 need = count * sizeof(T);
+        if (off + need > cap) fail(TT_ELENGTH, "mst: scratch region too small");
+        used = off + need;
+        return base ? reinterpret_cast<T*>(base + off) : nullptr;
+    }
+};
+
+// Every device buffer a Boruvka run on n vertices uses, carved from `c` in
+// a fixed order (a measuring Carve gives the total).
+struct MstBuffers {
+    u32 *comp, *parent, *cw, *bj, *todo, *scal, *hist, *scan_tmp;
+    unsigned long long* cp;
+    float *bk, *keys;
+    u64 *sk, *alt;
+    MstBuffers(Carve& c, u32 n) {
+        comp = c.get<u32>(n);
+        parent = c.get<u32>(n);
+        cw = c.get<u32>(n);
+        cp = c.get<unsigned long long>(n);
+        bk = c.get<float>(n);
+        bj = c.get<u32>(n);
+        todo = c.get<u32>(n);
+        scal = c.get<u32>(4);      // [0] ntodo, [1] changed, [2] out count, [3] n - 1
+        keys = c.get<float>(n);
+        sk = c.get<u64>(n);
+        alt = c.get<u64>(n);
+        const u64 he = radix_hist_entries(n);
+        hist = c.get<u32>(he);
+        scan_tmp = c.get<u32>(std::max<u64>(1, scan_tmp_entries(he)));
+    }
+};
+
+}  // namespace
+
+u64 mst_scratch_bytes(u32 n) {
+    Carve c;
+    MstBuffers b(c, std::max<u32>(n, 2));
+    (void)b;
+    return c.used;
+}
+
+// Sorted MST edge weights (n - 1 floats, device) of the complete graph, with
+// every temporary carved from the device region [base, base + cap).
+void mst_deaths_in(const float* d_x, u32 n, u32 dim, int kind, float p, float* d_deaths, char* base, u64 cap,
+                   hipStream_t s, MstStats* st) {
     if (n < 2) return;
     const MetricVariant v = variant_of(kind, p);
     const MetricArgs ma = metric_args(kind, p);
-    u32* comp = ws.get<u32>(n);
-    u32* parent = ws.get<u32>(n);
-    u32* cw = ws.get<u32>(n);
-    unsigned long long* cp = ws.get<unsigned long long>(n);
-    float* bk = ws.get<float>(n);
-    u32* bj = ws.get<u32>(n);
-    u32* todo = ws.get<u32>(n);
-    u32* scal = ws.get<u32>(4);      // [0] ntodo, [1] changed, [2] out count
-    float* keys = ws.get<float>(n);
+    Carve c{base, cap, 0};
+    const MstBuffers B(c, n);
+    u32 *comp = B.comp, *parent = B.parent, *cw = B.cw, *bj = B.bj, *todo = B.todo, *scal = B.scal;
+    unsigned long long* cp = B.cp;
+    float *bk = B.bk, *keys = B.keys;
     iota_u32<<<blocks_of(n), MB, 0, s>>>(comp, n);
     iota_u32<<<blocks_of(n), MB, 0, s>>>(parent, n);
     fill_device_u32(cw, n, 0xffffffffu, s);
     fill_device_u32(reinterpret_cast<u32*>(cp), 2 * u64(n), 0xffffffffu, s);
     fill_device_u32(bj, n, 0xffffffffu, s);
-    fill_device_u32(scal, 4, 0u, s);
+    fill_device_u32(scal, 3, 0u, s);
+    fill_device_u32(scal + 3, 1, n - 1, s);
     TTG_LAUNCH_CHECK();
     u32 edges = 0;
     for (u32 round = 0; edges < n - 1; ++round) {
@@ -242,17 +293,56 @@ void mst_deaths_dev(const float* d_x, u32 n, u32 dim, int kind, float p, float* 
         TTG_HIP(hipStreamSynchronize(s));
     }
     if (edges != n - 1) fail(TT_EDEVICE, "mst: spanning tree has the wrong edge count");
-    u64* sk = ws.get<u64>(n - 1);
-    u64* alt = ws.get<u64>(n - 1);
     switch (v) {
-        case MV_L2: mst_finish<MV_L2><<<blocks_of(n), MB, 0, s>>>(keys, n - 1, sk); break;
-        default: mst_finish<MV_L1><<<blocks_of(n), MB, 0, s>>>(keys, n - 1, sk); break;
+        case MV_L2: mst_finish<MV_L2><<<blocks_of(n), MB, 0, s>>>(keys, n - 1, B.sk); break;
+        default: mst_finish<MV_L1><<<blocks_of(n), MB, 0, s>>>(keys, n - 1, B.sk); break;
     }
     TTG_LAUNCH_CHECK();
-    radix_sort_u64(sk, nullptr, n - 1, 0, 31, alt, nullptr, ws, s);
-    mst_unpack<<<blocks_of(n), MB, 0, s>>>(sk, n - 1, d_deaths);
+    radix_sort_u64_counted(B.sk, n - 1, scal + 3, 0, 31, B.alt, B.hist, B.scan_tmp, s);
+    mst_unpack<<<blocks_of(n), MB, 0, s>>>(B.sk, n - 1, d_deaths);
     TTG_LAUNCH_CHECK();
 }
+
+// The same with the region from a Workspace.
+void mst_deaths_dev(const float* d_x, u32 n, u32 dim, int kind, float p, float* d_deaths, Workspace& ws,
+                    hipStream_t s, MstStats* st) {
+    if (n < 2) return;
+    const u64 bytes = mst_scratch_bytes(n);
+    mst_deaths_in(d_x, n, dim, kind, p, d_deaths, ws.get<char>(bytes), bytes, s, st);
+}
+
+// ---- the no-allocation form ---------------------------------------------
+// One scratch region per device for tt_mst_deaths_dev_noalloc: a file-scope
+// device array, so a call allocates nothing. Host code never passes the
+// array's name to a kernel (on the host it is only a stand-in for the
+// symbol); it passes the device address hipGetSymbolAddress returns, looked
+// up once per device with that device current.
+constexpr u32 kMstNoAllocMaxN = 1u << 20;
+constexpr u64 kMstScratchBytes = u64(56) << 20;
+
+static __device__ __align__(256) unsigned char scratch[kMstScratchBytes];
+
+namespace {
+
+std::mutex g_scratch_mu;
+std::vector<char*> g_scratch_ptr;   // per device ordinal; nullptr until looked up
+
+char* scratch_ptr() {
+    int dev = 0;
+    TTG_HIP(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (g_scratch_ptr.size() <= size_t(dev)) g_scratch_ptr.resize(size_t(dev) + 1, nullptr);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            fail(TT_EDEVICE, std::string("hip: hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        g_scratch_ptr[dev] = static_cast<char*>(p);
+    }
+    return g_scratch_ptr[dev];
+}
+
+}  // namespace
 
 }  // namespace ttg
 
@@ -294,6 +384,24 @@ tt_status tt_mst_deaths_dev(const float* d_pts, uint32_t n, uint32_t dim, int ki
         TTG_HIP(hipStreamSynchronize(s));
         if (rounds) *rounds = st.rounds;
         if (scans) *scans = st.scanned;
+    });
+}
+
+tt_status tt_mst_deaths_dev_noalloc(const float* d_pts, uint32_t n, uint32_t dim, int kind, float p,
+                                    float* d_deaths, void* stream) {
+    return guarded([&] {
+        check_metric(kind, p);
+        if (n > kMstNoAllocMaxN)
+            fail(TT_ELENGTH, "tt_mst_deaths_dev_noalloc: n exceeds " + std::to_string(kMstNoAllocMaxN) +
+                                 " points (the static scratch region)");
+        if (n >= 2 && (!d_pts || !d_deaths)) fail(TT_EINVAL, "tt_mst_deaths_dev_noalloc: null buffer");
+        if (n >= 2 && dim == 0) fail(TT_EINVAL, "tt_mst_deaths_dev_noalloc: dim must be >= 1");
+        if (n < 2) return;
+        const u64 bytes = mst_scratch_bytes(n);
+        if (bytes > kMstScratchBytes) fail(TT_ELENGTH, "tt_mst_deaths_dev_noalloc: scratch region too small");
+        hipStream_t s = as_stream(stream);
+        mst_deaths_in(d_pts, n, dim, kind, p, d_deaths, scratch_ptr(), kMstScratchBytes, s, nullptr);
+        TTG_HIP(hipStreamSynchronize(s));
     });
 }
 
