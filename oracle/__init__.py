@@ -14,6 +14,9 @@ import this package, and only as the checker. The product never does.
 """
 from .bindings import (  # noqa: F401
     METRIC_KINDS,
+    Complex,
+    index_values,
+    index_words,
     Reference,
     Restatement,
     parse_metric,

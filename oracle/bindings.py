@@ -51,6 +51,15 @@ class Restatement:
             fn.restype = C.c_int
         L.tto_apparent_edges.argtypes = [_f32p, C.c_uint32, _u64p, _f32p, C.c_uint64, _u8p]
         L.tto_decode.argtypes = [C.c_uint32, C.c_uint32, C.c_uint64, C.c_uint64, _u32p]
+        cx = [C.c_int, C.c_uint32, _f32p, _u64p, _u32p, _f32p]
+        L.tto_apparent_cx.argtypes = cx + [C.c_uint32, C.c_uint32, _u64p, _f32p, C.c_uint64, _u8p]
+        L.tto_emergent_cx.argtypes = cx + [C.c_float, C.c_uint32, C.c_uint32, _u64p, _f32p, C.c_uint64, _u64p,
+                                           _u8p]
+        L.tto_assemble_cx.argtypes = cx + [C.c_float, C.c_uint32, C.c_uint32, _u64p, _f32p, C.c_uint64, _u64p,
+                                           C.c_uint64, C.c_int, _u64p, C.POINTER(_u64p), C.POINTER(_f32p),
+                                           C.POINTER(_u64p), C.POINTER(_u8p), _u64p, C.POINTER(_u64p),
+                                           C.POINTER(_f32p), _u64p]
+        L.tto_free.argtypes = [C.c_void_p]
 
     def parse(self, metric):
         k, p = C.c_int(), C.c_float()
@@ -153,6 +162,92 @@ class Restatement:
         self.lib.tto_decode(n, nv, idx & (2**64 - 1), idx >> 64, _ptr(vd, _u32p))
         return vd
 
+    def apparent_cx(self, cx, nv, idx, d, words=1):
+        """bit 0: zero_apparent_cofacet, bit 1: zero_apparent_facet, per simplex."""
+        iw = index_words(idx, words)
+        d = _f32(d)
+        out = np.zeros(max(len(d), 1), np.uint8)
+        self.lib.tto_apparent_cx(*cx.args(), nv, words, _ptr(iw, _u64p), _ptr(d, _f32p), len(d), _ptr(out, _u8p))
+        return out[: len(d)]
+
+    def emergent_cx(self, cx, threshold, nv, idx, d, words=1):
+        """(cofacet indices, states) of compute_pairs' emergent shortcut."""
+        iw = index_words(idx, words)
+        d = _f32(d)
+        cof = np.zeros((max(len(d), 1), words), np.uint64)
+        st = np.zeros(max(len(d), 1), np.uint8)
+        self.lib.tto_emergent_cx(*cx.args(), threshold, nv, words, _ptr(iw, _u64p), _ptr(d, _f32p), len(d),
+                                 _ptr(cof, _u64p), _ptr(st, _u8p))
+        return index_values(cof[: len(d)], words), st[: len(d)]
+
+    def assemble_cx(self, cx, threshold, nv, idx, d, pivots, keep_next, words=1):
+        """(columns [(idx, d)], emergent (cof, state) per column, next
+        simplices [(idx, d)], skipped) of Reducer::assemble."""
+        iw = index_words(idx, words)
+        d = _f32(d)
+        pv = index_words(sorted(int(v) for v in pivots), words)
+        nc, nn, sk = C.c_uint64(), C.c_uint64(), C.c_uint64()
+        ci, cf, nix = _u64p(), _u64p(), _u64p()
+        cd, nd = _f32p(), _f32p()
+        cs = _u8p()
+        self.lib.tto_assemble_cx(*cx.args(), threshold, nv, words, _ptr(iw, _u64p), _ptr(d, _f32p), len(d),
+                                 _ptr(pv, _u64p) if len(pv) else None, len(pv), int(keep_next), C.byref(nc),
+                                 C.byref(ci), C.byref(cd), C.byref(cf), C.byref(cs), C.byref(nn), C.byref(nix),
+                                 C.byref(nd), C.byref(sk))
+
+        def take(p, k, dt):
+            out = np.ctypeslib.as_array(p, shape=(max(k, 1),))[:k].astype(dt, copy=True)
+            self.lib.tto_free(p)
+            return out
+
+        k, m = nc.value, nn.value
+        cols = list(zip(index_values(take(ci, k * words, np.uint64), words), take(cd, k, np.float32).tolist()))
+        em = (index_values(take(cf, k * words, np.uint64), words), take(cs, k, np.uint8))
+        nxt = list(zip(index_values(take(nix, m * words, np.uint64), words), take(nd, m, np.float32).tolist()))
+        return cols, em, nxt, sk.value
+
+
+class Complex:
+    """A dense (full n x n matrix) or sparse (CSR graph) complex for the
+    reducer's per-simplex tests, marshalled for the C entry points."""
+
+    def __init__(self, mat=None, n=None, row_ptr=None, col=None, w=None):
+        if mat is not None:
+            self.sparse = 0
+            self.mat = _f32(mat)
+            self.n = self.mat.shape[0]
+            self.rp = self.col = self.w = None
+        else:
+            self.sparse = 1
+            self.n = int(n)
+            self.mat = None
+            self.rp = np.ascontiguousarray(row_ptr, np.uint64)
+            self.col = np.ascontiguousarray(col, np.uint32)
+            self.w = _f32(w)
+
+    def args(self):
+        return (self.sparse, self.n,
+                None if self.mat is None else _ptr(self.mat, _f32p),
+                None if self.rp is None else _ptr(self.rp, _u64p),
+                None if self.col is None else _ptr(self.col, _u32p),
+                None if self.w is None else _ptr(self.w, _f32p))
+
+
+def index_words(values, words):
+    """Python ints -> (count,) or (count, 2) uint64 words, low word first."""
+    vals = [int(v) for v in values]
+    if words == 1:
+        return np.array(vals, dtype=np.uint64)
+    return np.array([[v & (2**64 - 1), v >> 64] for v in vals], dtype=np.uint64).reshape(-1, 2)
+
+
+def index_values(words_arr, words):
+    a = np.asarray(words_arr, dtype=np.uint64)
+    if words == 1:
+        return [int(v) for v in a.ravel()]
+    a = a.reshape(-1, 2)
+    return [int(lo) | (int(hi) << 64) for lo, hi in a]
+
 
 def parse_metric(metric):
     """(kind, p) exactly as the reference's Metric::parse (metric.hpp:240-253)."""
@@ -202,6 +297,14 @@ class Reference:
                                      C.POINTER(C.c_double), C.POINTER(C.c_double), _f32p]
         L.ref_time_graph.argtypes = [_f32p, C.c_uint32, C.c_uint32, C.c_char_p, C.c_int, C.c_float,
                                      C.c_uint32, C.c_uint32, C.POINTER(C.c_double), _u64p]
+        cx = [C.c_int, C.c_uint32, _f32p, _u64p, _u32p, _f32p]
+        L.ref_apparent_cx.argtypes = cx + [C.c_uint32, C.c_uint32, _u64p, _f32p, C.c_uint64, _u8p]
+        L.ref_emergent_cx.argtypes = cx + [C.c_float, C.c_uint32, C.c_uint32, _u64p, _f32p, C.c_uint64, _u64p,
+                                           _u8p]
+        L.ref_assemble_cx.argtypes = cx + [C.c_float, C.c_uint32, C.c_uint32, _u64p, _f32p, C.c_uint64, _u64p,
+                                           C.c_uint64, C.c_int, C.c_uint32, _u64p, C.POINTER(_u64p),
+                                           C.POINTER(_f32p), _u64p, C.POINTER(_u64p), C.POINTER(_f32p), _u64p]
+        L.ref_compute_cx.argtypes = cx + [C.c_float, C.c_uint32, C.c_int] + bars_out + [C.POINTER(C.c_int)]
 
     _ERR = {1: ValueError, 2: MemoryError, 3: OverflowError, 4: RuntimeError}
 
@@ -336,6 +439,54 @@ class Reference:
         md = {"auto": 0, "boundary": 1, "coboundary": 2}[mode]
         return self._bars(self.lib.ref_persistence_csr, n, _ptr(rp, _u64p), _ptr(col, _u32p), _ptr(w, _f32p),
                           maxdim, threshold, md)
+
+    def apparent_cx(self, cx, nv, idx, d, words=1):
+        iw = index_words(idx, words)
+        d = _f32(d)
+        out = np.zeros(max(len(d), 1), np.uint8)
+        self._check(self.lib.ref_apparent_cx(*cx.args(), nv, words, _ptr(iw, _u64p), _ptr(d, _f32p), len(d),
+                                             _ptr(out, _u8p)))
+        return out[: len(d)]
+
+    def emergent_cx(self, cx, threshold, nv, idx, d, words=1):
+        iw = index_words(idx, words)
+        d = _f32(d)
+        cof = np.zeros((max(len(d), 1), words), np.uint64)
+        st = np.zeros(max(len(d), 1), np.uint8)
+        self._check(self.lib.ref_emergent_cx(*cx.args(), threshold, nv, words, _ptr(iw, _u64p), _ptr(d, _f32p),
+                                             len(d), _ptr(cof, _u64p), _ptr(st, _u8p)))
+        return index_values(cof[: len(d)], words), st[: len(d)]
+
+    def assemble_cx(self, cx, threshold, nv, idx, d, pivots, keep_next, words=1, threads=1):
+        """(columns [(idx, d)], next simplices [(idx, d)], skipped) of
+        Reducer::assemble."""
+        iw = index_words(idx, words)
+        d = _f32(d)
+        pv = index_words(sorted(int(v) for v in pivots), words)
+        nc, nn, sk = C.c_uint64(), C.c_uint64(), C.c_uint64()
+        ci, nix = _u64p(), _u64p()
+        cd, nd = _f32p(), _f32p()
+        self._check(self.lib.ref_assemble_cx(*cx.args(), threshold, nv, words, _ptr(iw, _u64p), _ptr(d, _f32p),
+                                             len(d), _ptr(pv, _u64p) if len(pv) else None, len(pv), int(keep_next),
+                                             threads, C.byref(nc), C.byref(ci), C.byref(cd), C.byref(nn),
+                                             C.byref(nix), C.byref(nd), C.byref(sk)))
+        k, m = nc.value, nn.value
+        cols = list(zip(index_values(self._take(ci, k * words, np.uint64), words),
+                        self._take(cd, k, np.float32).tolist()))
+        nxt = list(zip(index_values(self._take(nix, m * words, np.uint64), words),
+                       self._take(nd, m, np.float32).tolist()))
+        return cols, nxt, sk.value
+
+    def compute_cx(self, cx, threshold, maxdim, force_wide=False):
+        """rips::compute with force_wide: ((dims, births, deaths), wide)."""
+        nb = C.c_uint64()
+        dims, b, e = _u32p(), _f32p(), _f32p()
+        wide = C.c_int()
+        self._check(self.lib.ref_compute_cx(*cx.args(), threshold, maxdim, int(force_wide), C.byref(nb),
+                                            C.byref(dims), C.byref(b), C.byref(e), C.byref(wide)))
+        k = nb.value
+        return ((self._take(dims, k, np.uint32), self._take(b, k, np.float32), self._take(e, k, np.float32)),
+                bool(wide.value))
 
     def time_dense(self, x, metric="l2", threads=0):
         x = _f32(x)

@@ -13,6 +13,8 @@
 #include <cstdlib>
 #include <cstring>
 #include <exception>
+#include <memory>
+#include <type_traits>
 #include <string>
 
 #include "turbotops/turbotops.hpp"
@@ -304,6 +306,193 @@ int ref_time_graph(const float* pts, uint32_t n, uint32_t dim, const char* metri
             *size = mst::h0(c, tau).birth[0].size();
         }
         *secs = secs_since(t0);
+    });
+}
+
+}  // extern "C"
+
+// ---- the reducer's per-simplex tests (rips.hpp:585-665, :903-928, :981-1023)
+// Complexes: sparse = 0 -> DenseComplex (full layout, n x n matrix), 1 ->
+// SparseComplex over a CSR graph. Simplex indices as `words` 64-bit words
+// (1: Reducer<u64>, 2: Reducer<u128>, low word first).
+namespace {
+
+template <typename I>
+I get_idx(const uint64_t* w, uint32_t words, uint64_t t) {
+    if constexpr (sizeof(I) > 8) return words == 2 ? (I(w[2 * t + 1]) << 64) | w[2 * t] : I(w[t]);
+    else return I(w[t]);
+}
+
+template <typename I>
+void put_idx(uint64_t* w, uint32_t words, uint64_t t, I v) {
+    if (words == 2) {
+        w[2 * t] = uint64_t(v);
+        if constexpr (sizeof(I) > 8) w[2 * t + 1] = uint64_t(v >> 64);
+        else w[2 * t + 1] = 0;
+    } else {
+        w[t] = uint64_t(v);
+    }
+}
+
+struct Cx {
+    rips::DenseComplex dense;
+    CsrGraph g;
+    std::unique_ptr<rips::SparseComplex> sp;
+    Cx(int sparse, uint32_t n, const float* mat, const uint64_t* rp, const uint32_t* col, const float* w,
+       float threshold) {
+        if (!sparse) {
+            dense.n = n;
+            dense.full = true;
+            dense.mat.assign(mat, mat + u64(n) * n);
+            dense.threshold = threshold;
+        } else {
+            g.n = n;
+            g.row_ptr.assign(rp, rp + u64(n) + 1);
+            g.m = g.row_ptr[n] / 2;
+            g.col_idx.assign(col, col + g.row_ptr[n]);
+            g.weights.assign(w, w + g.row_ptr[n]);
+            sp = std::make_unique<rips::SparseComplex>(g, threshold);
+        }
+    }
+};
+
+template <typename F>
+void with_reducer(const Cx& cx, uint32_t words, uint32_t maxdim, F&& f) {
+    rips::Barcode out;
+    rips::RipsStats st;
+    if (cx.sp) {
+        if (words == 2) {
+            rips::Reducer<rips::u128, rips::SparseComplex> r(*cx.sp, maxdim, out, st);
+            f(r, st);
+        } else {
+            rips::Reducer<u64, rips::SparseComplex> r(*cx.sp, maxdim, out, st);
+            f(r, st);
+        }
+    } else {
+        if (words == 2) {
+            rips::Reducer<rips::u128, rips::DenseComplex> r(cx.dense, maxdim, out, st);
+            f(r, st);
+        } else {
+            rips::Reducer<u64, rips::DenseComplex> r(cx.dense, maxdim, out, st);
+            f(r, st);
+        }
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+// Reducer::zero_apparent_cofacet / zero_apparent_facet (rips.hpp:644-654) of
+// simplices with nv vertices: bit 0 cofacet, bit 1 facet.
+int ref_apparent_cx(int sparse, uint32_t n, const float* mat, const uint64_t* rp, const uint32_t* col,
+                    const float* w, uint32_t nv, uint32_t words, const uint64_t* idx, const float* d,
+                    uint64_t count, uint8_t* out) {
+    return guard([&] {
+        const Cx cx(sparse, n, mat, rp, col, w, INFINITY);
+        with_reducer(cx, words, nv, [&](auto& r, rips::RipsStats&) {
+            using R = std::remove_reference_t<decltype(r)>;
+            using I = decltype(R::E::idx);
+            for (uint64_t t = 0; t < count; ++t) {
+                const typename R::E e{get_idx<I>(idx, words, t), d[t]};
+                out[t] = uint8_t((r.zero_apparent_cofacet(e, nv).idx != rips::kNone<I> ? 1 : 0) |
+                                 (r.zero_apparent_facet(e, nv).idx != rips::kNone<I> ? 2 : 0));
+            }
+        });
+    });
+}
+
+// compute_pairs' emergent shortcut without the pivot lookup
+// (rips.hpp:903-928): the first cofacet of diameter col.d in the
+// enumeration order (descending index, cap = threshold) -> cof, state 1,
+// or 2 when zero_apparent_facet finds that cofacet's apparent facet; 0 when
+// there is none.
+int ref_emergent_cx(int sparse, uint32_t n, const float* mat, const uint64_t* rp, const uint32_t* col,
+                    const float* w, float threshold, uint32_t nv, uint32_t words, const uint64_t* idx,
+                    const float* d, uint64_t count, uint64_t* cof, uint8_t* state) {
+    return guard([&] {
+        const Cx cx(sparse, n, mat, rp, col, w, threshold);
+        with_reducer(cx, words, nv, [&](auto& r, rips::RipsStats&) {
+            using R = std::remove_reference_t<decltype(r)>;
+            using E = typename R::E;
+            using I = decltype(E::idx);
+            u32 vd[R::KMAX];
+            for (uint64_t t = 0; t < count; ++t) {
+                const E col_e{get_idx<I>(idx, words, t), d[t]};
+                r.decode(col_e.idx, nv, vd);
+                E cand{rips::kNone<I>, 0.0f};
+                r.cofacet_entries(vd, nv, col_e.d, true, r.cx.threshold, [&](const E& cf) {
+                    if (cf.d == col_e.d) {
+                        cand = cf;
+                        return false;
+                    }
+                    return true;
+                });
+                if (cand.idx == rips::kNone<I>) {
+                    state[t] = 0;
+                    put_idx<I>(cof, words, t, I(0));
+                } else {
+                    state[t] = r.zero_apparent_facet(cand, nv + 1).idx == rips::kNone<I> ? 1 : 2;
+                    put_idx<I>(cof, words, t, cand.idx);
+                }
+            }
+        });
+    });
+}
+
+// Reducer::assemble(nv - 1) (rips.hpp:981-1023) on the given simplices (nv
+// vertices) with the given pivots cleared; columns in its (sorted) order,
+// the next simplices in its order when keep_next, the skipped count from
+// RipsStats::apparent. Outputs malloc'd (ref_free).
+int ref_assemble_cx(int sparse, uint32_t n, const float* mat, const uint64_t* rp, const uint32_t* col,
+                    const float* w, float threshold, uint32_t nv, uint32_t words, const uint64_t* idx,
+                    const float* d, uint64_t count, const uint64_t* pivots, uint64_t npivots, int keep_next,
+                    uint32_t threads, uint64_t* ncols, uint64_t** cidx, float** cd, uint64_t* nnext,
+                    uint64_t** nidx, float** nd, uint64_t* skipped) {
+    return guard([&] {
+        if (threads) set_num_threads(threads);
+        const Cx cx(sparse, n, mat, rp, col, w, threshold);
+        const u32 dim = nv - 1;
+        with_reducer(cx, words, keep_next ? dim + 2 : dim + 1, [&](auto& r, rips::RipsStats& st) {
+            using R = std::remove_reference_t<decltype(r)>;
+            using E = typename R::E;
+            using I = decltype(E::idx);
+            r.simplices.clear();
+            for (uint64_t t = 0; t < count; ++t) r.simplices.push_back(E{get_idx<I>(idx, words, t), d[t]});
+            r.pivots.reset(2 * npivots + 16);
+            for (uint64_t t = 0; t < npivots; ++t) r.pivots.insert(get_idx<I>(pivots, words, t), t);
+            r.assemble(dim);
+            *ncols = r.columns.size();
+            *cidx = static_cast<uint64_t*>(std::malloc(std::max<size_t>(1, r.columns.size()) * words * 8));
+            *cd = static_cast<float*>(std::malloc(std::max<size_t>(1, r.columns.size()) * 4));
+            for (size_t t = 0; t < r.columns.size(); ++t) {
+                put_idx<I>(*cidx, words, t, r.columns[t].idx);
+                (*cd)[t] = r.columns[t].d;
+            }
+            *nnext = r.simplices.size();
+            *nidx = static_cast<uint64_t*>(std::malloc(std::max<size_t>(1, r.simplices.size()) * words * 8));
+            *nd = static_cast<float*>(std::malloc(std::max<size_t>(1, r.simplices.size()) * 4));
+            for (size_t t = 0; t < r.simplices.size(); ++t) {
+                put_idx<I>(*nidx, words, t, r.simplices[t].idx);
+                (*nd)[t] = r.simplices[t].d;
+            }
+            *skipped = st.apparent.empty() ? 0 : st.apparent.back();
+        });
+    });
+}
+
+// tops::rips::compute on a dense matrix or a CSR graph with force_wide
+// (rips.hpp:1038-1061): 128-bit indices even where 64 bits suffice.
+int ref_compute_cx(int sparse, uint32_t n, const float* mat, const uint64_t* rp, const uint32_t* col,
+                   const float* w, float threshold, uint32_t maxdim, int force_wide, uint64_t* nbars,
+                   uint32_t** dims, float** births, float** deaths, int* wide) {
+    return guard([&] {
+        const Cx cx(sparse, n, mat, rp, col, w, threshold);
+        rips::RipsStats st;
+        const rips::Barcode bc = sparse ? rips::compute(*cx.sp, maxdim, &st, force_wide != 0)
+                                        : rips::compute(cx.dense, maxdim, &st, force_wide != 0);
+        flatten(bc, nbars, dims, births, deaths);
+        *wide = st.wide_index ? 1 : 0;
     });
 }
 

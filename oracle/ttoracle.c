@@ -477,3 +477,340 @@ void tto_apparent_edges(const float* mat, uint32_t n, const uint64_t* idx, const
                         uint64_t m, uint8_t* flags) {
     for (uint64_t t = 0; t < m; ++t) flags[t] = (uint8_t)tto_apparent_cofacet(mat, n, 2, idx[t], 0, d[t]);
 }
+
+/* ---- the reducer's per-simplex tests on a dense or sparse complex -------
+ * A complex is the full matrix (rips.hpp:262-381, DenseComplex with
+ * full = true) or a CSR graph's flag complex (SparseComplex, rips.hpp:
+ * 418-472), with the filtration threshold. Simplex indices are passed as
+ * `words` 64-bit words each (1: u64, 2: u128 low word first). */
+
+typedef struct {
+    int sparse;
+    uint32_t n;
+    const float* mat;        /* dense: n x n, full layout */
+    const uint64_t* rp;      /* sparse: CSR rows sorted by neighbour id */
+    const uint32_t* col;
+    const float* w;
+    float threshold;
+} cx_t;
+
+/* DenseComplex::dist (rips.hpp:267-273) / SparseComplex::dist (:428):
+ * 0 on the diagonal; a missing graph edge is +inf (CsrGraph::weight,
+ * core.hpp:100-105). */
+static int cx_edge(const cx_t* c, uint32_t i, uint32_t j, float* out) {
+    if (i == j) {
+        *out = 0.0f;
+        return 1;
+    }
+    if (!c->sparse) {
+        *out = c->mat[(uint64_t)i * c->n + j];
+        return 1;
+    }
+    uint64_t lo = c->rp[i], hi = c->rp[i + 1];
+    while (lo < hi) {
+        const uint64_t mid = lo + (hi - lo) / 2;
+        if (c->col[mid] < j) lo = mid + 1;
+        else hi = mid;
+    }
+    if (lo < c->rp[i + 1] && c->col[lo] == j) {
+        *out = c->w[lo];
+        return 1;
+    }
+    *out = INFINITY;
+    return 0;
+}
+
+static float cx_dist(const cx_t* c, uint32_t i, uint32_t j) {
+    float v;
+    cx_edge(c, i, j, &v);
+    return v;
+}
+
+/* Cofacet enumeration of a simplex (vd: k vertices, descending; diameter
+ * diam), in descending w (rips.hpp:326-343, cofacets_full's NO_SIMD loop,
+ * and SparseComplex::cofacets :432-471, the common neighbours of the rows):
+ * w outside the simplex with cd = max(diam, d(v, w) for v in vd) <= cap;
+ * all == 0 keeps only w > vd[0]. Calls emit(ctx, w, t, cd), t = number of
+ * simplex vertices above w, until it returns 0. */
+typedef int (*emit_fn)(void* ctx, uint32_t w, uint32_t t, float cd);
+
+static void cx_cofacets(const cx_t* c, const uint32_t* vd, uint32_t k, float diam, int all, float cap,
+                        emit_fn emit, void* ctx) {
+    const uint32_t stop = all ? 0 : vd[0] + 1;
+    uint32_t t = 0;
+    for (int64_t wi = (int64_t)c->n - 1; wi >= (int64_t)stop; --wi) {
+        const uint32_t w = (uint32_t)wi;
+        while (t < k && vd[t] > w) ++t;
+        if (t < k && vd[t] == w) continue;
+        float cd = diam;
+        int ok = 1;
+        for (uint32_t i = 0; i < k && ok; ++i) {
+            float dv;
+            if (!cx_edge(c, vd[i], w, &dv)) ok = 0;   /* not a common neighbour */
+            else cd = fmax_ref(cd, dv);
+        }
+        if (ok && cd <= cap && !emit(ctx, w, t, cd)) return;
+    }
+}
+
+/* cofacet of vd (k vertices) with w inserted below the t vertices above it */
+static u128 cofacet_index(const uint32_t* vd, uint32_t k, uint32_t w, uint32_t t) {
+    u128 s = 0;
+    for (uint32_t i = 0; i < t; ++i) s += binom(vd[i], k + 1 - i);
+    s += binom(w, k + 1 - t);
+    for (uint32_t i = t; i < k; ++i) s += binom(vd[i], k - i);
+    return s;
+}
+
+typedef struct {
+    int found;
+    uint32_t w, t;
+    float cd;
+} first_t;
+
+static int take_first(void* ctx, uint32_t w, uint32_t t, float cd) {
+    first_t* f = (first_t*)ctx;
+    f->found = 1;
+    f->w = w;
+    f->t = t;
+    f->cd = cd;
+    return 0;
+}
+
+/* zero_pivot_cofacet_v (rips.hpp:585-605): the first cofacet (descending
+ * index) with cap = d; its vertices to cv. */
+static int cx_zero_pivot_cofacet(const cx_t* c, const uint32_t* vd, uint32_t k, float d, uint32_t* cv,
+                                 u128* idx) {
+    first_t f = {0, 0, 0, 0.0f};
+    cx_cofacets(c, vd, k, d, 1, d, take_first, &f);
+    if (!f.found) return 0;
+    for (uint32_t i = 0; i < f.t; ++i) cv[i] = vd[i];
+    cv[f.t] = f.w;
+    for (uint32_t i = f.t; i < k; ++i) cv[i + 1] = vd[i];
+    *idx = cofacet_index(vd, k, f.w, f.t);
+    return 1;
+}
+
+/* zero_pivot_facet_v (rips.hpp:609-636): the first facet in ascending index
+ * order (vertex j = 0, 1, ... removed) keeping an edge of length d. */
+static int cx_zero_pivot_facet(const cx_t* c, const uint32_t* vd, uint32_t k, float d, uint32_t* fv, u128* idx) {
+    if (k < 2) return 0;
+    for (uint32_t j = 0; j < k; ++j) {
+        int hit = 0;
+        for (uint32_t a = 0; a < k && !hit; ++a)
+            for (uint32_t b = a + 1; b < k && !hit; ++b)
+                if (a != j && b != j && cx_dist(c, vd[a], vd[b]) == d) hit = 1;
+        if (hit) {
+            uint32_t o = 0;
+            for (uint32_t i = 0; i < k; ++i)
+                if (i != j) fv[o++] = vd[i];
+            *idx = index_of(fv, k - 1);
+            return 1;
+        }
+    }
+    return 0;
+}
+
+/* zero_apparent_cofacet_v (rips.hpp:644-649) */
+static int cx_apparent_cofacet(const cx_t* c, const uint32_t* vd, uint32_t k, u128 idx, float d) {
+    uint32_t cv[KMAXV + 2], fv[KMAXV + 2];
+    u128 ci, fi;
+    if (!cx_zero_pivot_cofacet(c, vd, k, d, cv, &ci)) return 0;
+    if (!cx_zero_pivot_facet(c, cv, k + 1, d, fv, &fi)) return 0;
+    return fi == idx;
+}
+
+/* zero_apparent_facet_v (rips.hpp:638-643) */
+static int cx_apparent_facet(const cx_t* c, const uint32_t* vd, uint32_t k, u128 idx, float d) {
+    uint32_t cv[KMAXV + 2], fv[KMAXV + 2];
+    u128 ci, fi;
+    if (!cx_zero_pivot_facet(c, vd, k, d, fv, &fi)) return 0;
+    if (!cx_zero_pivot_cofacet(c, fv, k - 1, d, cv, &ci)) return 0;
+    return ci == idx;
+}
+
+static cx_t make_cx(int sparse, uint32_t n, const float* mat, const uint64_t* rp, const uint32_t* col,
+                    const float* w, float threshold) {
+    cx_t c;
+    c.sparse = sparse;
+    c.n = n;
+    c.mat = mat;
+    c.rp = rp;
+    c.col = col;
+    c.w = w;
+    c.threshold = threshold;
+    return c;
+}
+
+static u128 get_index(const uint64_t* words, uint32_t nw, uint64_t t) {
+    return nw == 2 ? (((u128)words[2 * t + 1]) << 64) | words[2 * t] : (u128)words[t];
+}
+
+static void put_index(uint64_t* words, uint32_t nw, uint64_t t, u128 v) {
+    if (nw == 2) {
+        words[2 * t] = (uint64_t)v;
+        words[2 * t + 1] = (uint64_t)(v >> 64);
+    } else {
+        words[t] = (uint64_t)v;
+    }
+}
+
+/* Both apparent tests of simplices with nv vertices: bit 0 of out[t]
+ * zero_apparent_cofacet, bit 1 zero_apparent_facet (rips.hpp:638-665). */
+void tto_apparent_cx(int sparse, uint32_t n, const float* mat, const uint64_t* rp, const uint32_t* col,
+                     const float* w, uint32_t nv, uint32_t words, const uint64_t* idx, const float* d,
+                     uint64_t count, uint8_t* out) {
+    const cx_t c = make_cx(sparse, n, mat, rp, col, w, INFINITY);
+    uint32_t vd[KMAXV + 2];
+    for (uint64_t t = 0; t < count; ++t) {
+        const u128 s = get_index(idx, words, t);
+        decode(n, nv, s, vd);
+        out[t] = (uint8_t)(cx_apparent_cofacet(&c, vd, nv, s, d[t]) | (cx_apparent_facet(&c, vd, nv, s, d[t]) << 1));
+    }
+}
+
+/* The pivot-independent part of compute_pairs' emergent shortcut
+ * (rips.hpp:903-928) per column (nv vertices): the first cofacet of equal
+ * diameter in descending index order, cof[t] (state 1), unless that cofacet
+ * has a zero-apparent facet (state 2); state 0 when there is none. The
+ * column is emergent iff state == 1 and cof[t] is not yet a pivot. */
+static uint8_t emergent_of(const cx_t* c, const uint32_t* vd, uint32_t nv, float d, u128* cof) {
+    uint32_t cv[KMAXV + 2];
+    u128 ci;
+    if (!cx_zero_pivot_cofacet(c, vd, nv, d, cv, &ci)) return 0;
+    *cof = ci;
+    return cx_apparent_facet(c, cv, nv + 1, ci, d) ? 2 : 1;
+}
+
+void tto_emergent_cx(int sparse, uint32_t n, const float* mat, const uint64_t* rp, const uint32_t* col,
+                     const float* w, float threshold, uint32_t nv, uint32_t words, const uint64_t* idx,
+                     const float* d, uint64_t count, uint64_t* cof, uint8_t* state) {
+    const cx_t c = make_cx(sparse, n, mat, rp, col, w, threshold);
+    uint32_t vd[KMAXV + 2];
+    for (uint64_t t = 0; t < count; ++t) {
+        const u128 s = get_index(idx, words, t);
+        decode(n, nv, s, vd);
+        u128 ci = 0;
+        state[t] = emergent_of(&c, vd, nv, d[t], &ci);
+        put_index(cof, words, t, state[t] ? ci : 0);
+    }
+}
+
+typedef struct {
+    u128 idx;
+    float d;
+} simplex_t;
+
+/* descending filtration order (sort_desc, rips.hpp:776-778 with precedes
+ * :116-118 reversed): diameter descending, index ascending */
+static int desc_cmp(const void* x, const void* y) {
+    const simplex_t* a = (const simplex_t*)x;
+    const simplex_t* b = (const simplex_t*)y;
+    if (a->d != b->d) return a->d > b->d ? -1 : 1;
+    return a->idx < b->idx ? -1 : (a->idx > b->idx ? 1 : 0);
+}
+
+static int sorted_has(const uint64_t* keys, uint32_t words, uint64_t np, u128 v) {
+    uint64_t lo = 0, hi = np;
+    while (lo < hi) {
+        const uint64_t mid = lo + (hi - lo) / 2;
+        if (get_index(keys, words, mid) < v) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo < np && get_index(keys, words, lo) == v;
+}
+
+typedef struct {
+    const cx_t* c;
+    const uint32_t* vd;
+    uint32_t nv;
+    const uint64_t* piv;
+    uint32_t words;
+    uint64_t np;
+    int keep;
+    simplex_t *cols, *next;
+    uint64_t ncols, nnext, cap_cols, cap_next, skipped;
+} asm_t;
+
+static void push_simplex(simplex_t** buf, uint64_t* len, uint64_t* cap, u128 idx, float d) {
+    if (*len == *cap) {
+        *cap = *cap ? 2 * *cap : 1024;
+        *buf = (simplex_t*)realloc(*buf, *cap * sizeof(simplex_t));
+    }
+    (*buf)[*len].idx = idx;
+    (*buf)[*len].d = d;
+    ++*len;
+}
+
+static int assemble_emit(void* ctx, uint32_t w, uint32_t t, float cd) {
+    asm_t* a = (asm_t*)ctx;
+    const u128 cf = cofacet_index(a->vd, a->nv, w, t);
+    if (a->keep) push_simplex(&a->next, &a->nnext, &a->cap_next, cf, cd);
+    if (sorted_has(a->piv, a->words, a->np, cf)) return 1;   /* cleared */
+    uint32_t cv[KMAXV + 2];
+    for (uint32_t i = 0; i < t; ++i) cv[i] = a->vd[i];
+    cv[t] = w;
+    for (uint32_t i = t; i < a->nv; ++i) cv[i + 1] = a->vd[i];
+    if (cx_apparent_cofacet(a->c, cv, a->nv + 1, cf, cd) || cx_apparent_facet(a->c, cv, a->nv + 1, cf, cd)) {
+        ++a->skipped;
+        return 1;
+    }
+    push_simplex(&a->cols, &a->ncols, &a->cap_cols, cf, cd);
+    return 1;
+}
+
+/* Reducer::assemble (rips.hpp:981-1023): the columns of dimension nv (nv +
+ * 1 vertices) from the simplices with nv vertices — every cofacet with a
+ * new top vertex (all = false) up to the threshold that is neither a pivot
+ * of the reduction just done (cleared; pivots sorted ascending) nor in a
+ * zero-apparent pair either way — in descending filtration order, with
+ * each column's emergent state (as tto_emergent_cx). keep_next: every such
+ * cofacet, in enumeration order, becomes the next simplices. Outputs are
+ * malloc'd (tto_free). */
+void tto_assemble_cx(int sparse, uint32_t n, const float* mat, const uint64_t* rp, const uint32_t* col,
+                     const float* w, float threshold, uint32_t nv, uint32_t words, const uint64_t* idx,
+                     const float* d, uint64_t count, const uint64_t* pivots, uint64_t npivots, int keep_next,
+                     uint64_t* ncols, uint64_t** cidx, float** cd, uint64_t** ccof, uint8_t** cstate,
+                     uint64_t* nnext, uint64_t** nidx, float** nd, uint64_t* skipped) {
+    const cx_t c = make_cx(sparse, n, mat, rp, col, w, threshold);
+    asm_t a;
+    memset(&a, 0, sizeof(a));
+    a.c = &c;
+    a.nv = nv;
+    a.piv = pivots;
+    a.words = words;
+    a.np = npivots;
+    a.keep = keep_next;
+    uint32_t vd[KMAXV + 2];
+    for (uint64_t t = 0; t < count; ++t) {
+        decode(n, nv, get_index(idx, words, t), vd);
+        a.vd = vd;
+        cx_cofacets(&c, vd, nv, d[t], 0, threshold, assemble_emit, &a);
+    }
+    qsort(a.cols, a.ncols, sizeof(simplex_t), desc_cmp);
+    *ncols = a.ncols;
+    *cidx = (uint64_t*)malloc((a.ncols ? a.ncols : 1) * words * 8);
+    *ccof = (uint64_t*)malloc((a.ncols ? a.ncols : 1) * words * 8);
+    *cd = (float*)malloc((a.ncols ? a.ncols : 1) * 4);
+    *cstate = (uint8_t*)malloc(a.ncols ? a.ncols : 1);
+    for (uint64_t t = 0; t < a.ncols; ++t) {
+        put_index(*cidx, words, t, a.cols[t].idx);
+        (*cd)[t] = a.cols[t].d;
+        decode(n, nv + 1, a.cols[t].idx, vd);
+        u128 ci = 0;
+        (*cstate)[t] = emergent_of(&c, vd, nv + 1, a.cols[t].d, &ci);
+        put_index(*ccof, words, t, (*cstate)[t] ? ci : 0);
+    }
+    *nnext = a.nnext;
+    *nidx = (uint64_t*)malloc((a.nnext ? a.nnext : 1) * words * 8);
+    *nd = (float*)malloc((a.nnext ? a.nnext : 1) * 4);
+    for (uint64_t t = 0; t < a.nnext; ++t) {
+        put_index(*nidx, words, t, a.next[t].idx);
+        (*nd)[t] = a.next[t].d;
+    }
+    *skipped = a.skipped;
+    free(a.cols);
+    free(a.next);
+}
+
+void tto_free(void* p) { free(p); }

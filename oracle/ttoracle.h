@@ -42,6 +42,22 @@ void tto_apparent_edges(const float* mat, uint32_t n, const uint64_t* idx, const
                         uint64_t m, uint8_t* flags);
 int tto_decode(uint32_t n, uint32_t nv, uint64_t idx_lo, uint64_t idx_hi, uint32_t* vd);
 
+/* Complex-generic forms: sparse = 0 with mat (n x n full layout), or
+ * sparse = 1 with a CSR graph (rp, col, w); simplex indices as `words`
+ * 64-bit words each (1 or 2, low word first). */
+void tto_apparent_cx(int sparse, uint32_t n, const float* mat, const uint64_t* rp, const uint32_t* col,
+                     const float* w, uint32_t nv, uint32_t words, const uint64_t* idx, const float* d,
+                     uint64_t count, uint8_t* out);
+void tto_emergent_cx(int sparse, uint32_t n, const float* mat, const uint64_t* rp, const uint32_t* col,
+                     const float* w, float threshold, uint32_t nv, uint32_t words, const uint64_t* idx,
+                     const float* d, uint64_t count, uint64_t* cof, uint8_t* state);
+void tto_assemble_cx(int sparse, uint32_t n, const float* mat, const uint64_t* rp, const uint32_t* col,
+                     const float* w, float threshold, uint32_t nv, uint32_t words, const uint64_t* idx,
+                     const float* d, uint64_t count, const uint64_t* pivots, uint64_t npivots, int keep_next,
+                     uint64_t* ncols, uint64_t** cidx, float** cd, uint64_t** ccof, uint8_t** cstate,
+                     uint64_t* nnext, uint64_t** nidx, float** nd, uint64_t* skipped);
+void tto_free(void* p);
+
 #ifdef __cplusplus
 }
 #endif
