@@ -177,6 +177,54 @@ tt_status tt_dense_filtration(const float* pts, uint32_t n, uint32_t dim, int ki
 tt_status tt_sparse_edges(uint32_t n, const uint64_t* row_ptr, const uint32_t* col, const float* w,
                           float threshold, uint64_t* m, uint64_t** idx, float** d, uint8_t** apparent);
 
+/* ---- columns of the coboundary reducer above dimension 0 ------------------ */
+
+/* A device copy of the complex the coboundary reducer works on, for its
+ * per-simplex work above dimension 0. Simplex indices cross this interface
+ * as index_words 64-bit words each (1: 64-bit indices, 2: 128-bit, low word
+ * first), the width rips::compute picks (rips.hpp:1051-1061: u64 when every
+ * index up to maxdim + 2 vertices fits, else u128; force_wide: u128).
+ * Dense: the n x n matrix in `layout` (TT_LAYOUT_FULL or TT_LAYOUT_LOWER),
+ * host memory, or with on_device (full layout only) a device pointer the
+ * complex borrows until it is destroyed. Sparse: a CSR graph (rows sorted by
+ * neighbour id, CsrGraph::from_rows, core.hpp:113-134). threshold: the
+ * filtration's cap (the enclosing radius for a dense complex at +inf). Fails
+ * with TT_EOVERFLOW when C(n, maxdim + 2) does not fit the index width. */
+typedef struct tt_complex tt_complex;
+tt_status tt_complex_dense(const float* mat, uint32_t n, int layout, int on_device, float threshold,
+                           uint32_t maxdim, int index_words, tt_complex** out);
+tt_status tt_complex_sparse(uint32_t n, const uint64_t* row_ptr, const uint32_t* col, const float* w,
+                            float threshold, uint32_t maxdim, int index_words, tt_complex** out);
+
+/* The pivot-independent part of compute_pairs' emergent-pair shortcut
+ * (rips.hpp:903-928) for columns with nv vertices: per column its zero-pivot
+ * cofacet (zero_pivot_cofacet_v, :585-605: the first cofacet in descending
+ * index order with the column's diameter) in cof and state[t] = 1, or
+ * state[t] = 2 when that cofacet has a zero-apparent facet
+ * (zero_apparent_facet, :638-643), or state[t] = 0 (cof 0) when there is no
+ * such cofacet. The reducer pairs a state-1 column with cof at once when cof
+ * is not yet a pivot. */
+tt_status tt_complex_emergent(tt_complex* cx, uint32_t nv, const uint64_t* idx, const float* d, uint64_t count,
+                              uint64_t* cof, uint8_t* state);
+
+/* The simplices (nv vertices) the next tt_complex_assemble starts from:
+ * dimension 1's are the kept edges (rips.hpp:868, `simplices.swap(edges)`). */
+tt_status tt_complex_seed(tt_complex* cx, uint32_t nv, const uint64_t* idx, const float* d, uint64_t count);
+
+/* Reducer::assemble (rips.hpp:981-1023) from the seeded simplices (nv
+ * vertices): every cofacet with a new top vertex up to the threshold that
+ * is neither one of the npivots pivots of the reduction just done (any
+ * order) nor in a zero-apparent pair either way (:1001-1007), in descending
+ * filtration order (diameter descending, index ascending; :1019), with
+ * each column's emergent state as tt_complex_emergent gives it. *skipped:
+ * the apparent ones. keep_next: every such cofacet becomes the seeded
+ * simplices (nv + 1 vertices). *idx, *cof (ncols x index_words), *d and
+ * *state are malloc'd; release them with tt_free. */
+tt_status tt_complex_assemble(tt_complex* cx, const uint64_t* pivots, uint64_t npivots, int keep_next,
+                              uint64_t* ncols, uint64_t** idx, float** d, uint64_t** cof, uint8_t** state,
+                              uint64_t* skipped);
+void tt_complex_destroy(tt_complex* cx);
+
 /* ---- edges of the boundary (RedZeD) engine -------------------------------- */
 
 /* persistence()'s boundary branch front end (turbotops.hpp:243-246:

@@ -1,15 +1,18 @@
 // SPDX-License-Identifier: MIT
 // host_harness.cpp — TEST INFRASTRUCTURE ONLY: a C ABI around the host
-// reducers (turbotops_amd/host/reducer.hpp, the cohomology engine, and
-// turbotops_amd/host/zerodiff.hpp, the boundary engine for H1) so the CPU
-// test suite can check them
-// against the reference with inputs the oracle computes (full matrix, sorted
-// edges, apparent flags) — no GPU involved. Not part of the product.
+// engines (turbotops_amd/host/cohomology.hpp, the cohomology engine's column
+// reduction, and turbotops_amd/host/zerodiff.hpp, the boundary engine for H1)
+// so the CPU test suite can check them against the reference with inputs the
+// oracle computes (full matrix, sorted edges, apparent flags) — no GPU
+// involved. The cohomology engine's columns and emergent states, which the
+// product takes from the GPU (csrc/columns.hip), come here from the oracle's
+// restatement (oracle/ttoracle.c, linked in). Not part of the product.
 #include <cstdlib>
 #include <cstring>
 #include <string>
 
-#include "reducer.hpp"
+#include "cohomology.hpp"
+#include "ttoracle.h"
 #include "zerodiff.hpp"
 
 using namespace tops::cob;
@@ -33,24 +36,87 @@ void flatten(const Diagram& dg, uint64_t* nbars, uint32_t** dims, float** b, flo
         }
 }
 
+// The oracle's restatements of the GPU column stages, as a ColumnSource.
+class OracleColumns final : public ColumnSource {
+  public:
+    OracleColumns(const DenseView& cx, u32 words) : sparse_(0), n_(cx.n), mat_(cx.mat), thr_(cx.threshold),
+                                                    words_(words) {}
+    OracleColumns(const SparseView& cx, u32 words)
+        : sparse_(1), n_(cx.n), rp_(cx.row_ptr), col_(cx.col), w_(cx.w), thr_(cx.threshold), words_(words) {}
+    void emergent(u32 nv, const u64* idx, const f32* d, u64 count, u64* cof, u8* state) override {
+        tto_emergent_cx(sparse_, n_, mat_, rp_, col_, w_, thr_, nv, words_, idx, d, count, cof, state);
+    }
+    void seed(u32 nv, const u64* idx, const f32* d, u64 count) override {
+        nv_ = nv;
+        sidx_.assign(idx, idx + count * words_);
+        sd_.assign(d, d + count);
+    }
+    void assemble(const u64* pivots, u64 npivots, bool keep_next, std::vector<u64>& idx, std::vector<f32>& d,
+                  std::vector<u64>& cof, std::vector<u8>& state, u64& skipped) override {
+        std::vector<u64> piv(pivots, pivots + npivots * words_);
+        // the oracle takes the pivots sorted
+        std::vector<u128> keys(npivots);
+        for (u64 t = 0; t < npivots; ++t)
+            keys[t] = words_ == 2 ? (u128(piv[2 * t + 1]) << 64) | piv[2 * t] : u128(piv[t]);
+        std::sort(keys.begin(), keys.end());
+        for (u64 t = 0; t < npivots; ++t) {
+            if (words_ == 2) {
+                piv[2 * t] = u64(keys[t]);
+                piv[2 * t + 1] = u64(keys[t] >> 64);
+            } else {
+                piv[t] = u64(keys[t]);
+            }
+        }
+        u64 nc = 0, nn = 0;
+        u64 *ci = nullptr, *cc = nullptr, *ni = nullptr;
+        f32 *cd = nullptr, *nd = nullptr;
+        u8* cs = nullptr;
+        tto_assemble_cx(sparse_, n_, mat_, rp_, col_, w_, thr_, nv_, words_, sidx_.data(), sd_.data(), sd_.size(),
+                        piv.empty() ? nullptr : piv.data(), npivots, keep_next, &nc, &ci, &cd, &cc, &cs, &nn, &ni,
+                        &nd, &skipped);
+        idx.assign(ci, ci + nc * words_);
+        cof.assign(cc, cc + nc * words_);
+        d.assign(cd, cd + nc);
+        state.assign(cs, cs + nc);
+        if (keep_next) {
+            sidx_.assign(ni, ni + nn * words_);
+            sd_.assign(nd, nd + nn);
+            ++nv_;
+        }
+        for (void* p : {(void*)ci, (void*)cc, (void*)ni, (void*)cd, (void*)nd, (void*)cs}) tto_free(p);
+    }
+
+  private:
+    int sparse_;
+    u32 n_;
+    const f32* mat_ = nullptr;
+    const u64* rp_ = nullptr;
+    const u32* col_ = nullptr;
+    const f32* w_ = nullptr;
+    f32 thr_;
+    u32 words_;
+    u32 nv_ = 0;
+    std::vector<u64> sidx_;
+    std::vector<f32> sd_;
+};
+
 template <class Cx>
 int run(const Cx& cx, uint32_t maxdim, const uint64_t* idx, const float* d, const uint8_t* ap, uint64_t m,
-        uint64_t* nbars, uint32_t** dims, float** b, float** e) {
+        int force_wide, uint64_t* nbars, uint32_t** dims, float** b, float** e) {
     try {
         Diagram out;
         out.ensure(maxdim);
         Telemetry tel;
-        // several threads, so the CPU tests also cover the parallel assembly
-        if (index_fits<u64>(cx.n, maxdim)) {
+        if (!force_wide && index_fits<u64>(cx.n, maxdim)) {
             Cohomology<u64, Cx> r(cx, maxdim, out, tel);
-            r.set_threads(4);
             r.dim0(idx, d, ap, m);
-            r.run_from_dim1();
+            OracleColumns src(cx, 1);
+            r.run(src);
         } else {
             Cohomology<u128, Cx> r(cx, maxdim, out, tel);
-            r.set_threads(4);
             r.dim0(idx, d, ap, m);
-            r.run_from_dim1();
+            OracleColumns src(cx, 2);
+            r.run(src);
         }
         flatten(out, nbars, dims, b, e);
         return 0;
@@ -84,44 +150,25 @@ int hh_boundary(const float* mat, uint32_t n, const uint32_t* verts, const float
 void hh_free(void* p) { std::free(p); }
 
 int hh_dense(const float* mat, uint32_t n, float threshold, uint32_t maxdim, const uint64_t* idx, const float* d,
-             const uint8_t* ap, uint64_t m, uint64_t* nbars, uint32_t** dims, float** b, float** e) {
+             const uint8_t* ap, uint64_t m, int force_wide, uint64_t* nbars, uint32_t** dims, float** b, float** e) {
     DenseView cx;
     cx.n = n;
     cx.threshold = threshold;
     cx.full = true;
     cx.mat = mat;
-    return run(cx, maxdim, idx, d, ap, m, nbars, dims, b, e);
+    return run(cx, maxdim, idx, d, ap, m, force_wide, nbars, dims, b, e);
 }
 
 int hh_sparse(uint32_t n, const uint64_t* rp, const uint32_t* col, const float* w, float threshold, uint32_t maxdim,
-              const uint64_t* idx, const float* d, const uint8_t* ap, uint64_t m, uint64_t* nbars, uint32_t** dims,
-              float** b, float** e) {
+              const uint64_t* idx, const float* d, const uint8_t* ap, uint64_t m, int force_wide, uint64_t* nbars,
+              uint32_t** dims, float** b, float** e) {
     SparseView cx;
     cx.n = n;
     cx.threshold = threshold;
     cx.row_ptr = rp;
     cx.col = col;
     cx.w = w;
-    return run(cx, maxdim, idx, d, ap, m, nbars, dims, b, e);
+    return run(cx, maxdim, idx, d, ap, m, force_wide, nbars, dims, b, e);
 }
 
-// apparent flags of sorted edges on a sparse complex (the host dim-0 test)
-int hh_sparse_apparent(uint32_t n, const uint64_t* rp, const uint32_t* col, const float* w, uint32_t maxdim,
-                       const uint64_t* idx, const float* d, uint64_t m, uint8_t* flags) {
-    try {
-        SparseView cx;
-        cx.n = n;
-        cx.row_ptr = rp;
-        cx.col = col;
-        cx.w = w;
-        Diagram out;
-        Telemetry tel;
-        Cohomology<u64, SparseView> r(cx, maxdim, out, tel);
-        for (uint64_t t = 0; t < m; ++t) flags[t] = r.edge_apparent(idx[t], d[t]);
-        return 0;
-    } catch (const std::exception& ex) {
-        g_err = ex.what();
-        return 1;
-    }
-}
 }

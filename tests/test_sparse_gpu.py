@@ -6,20 +6,16 @@ Reducer::dim0's edge order and apparent test over SparseComplex::cofacets
 * Apparent flags: the definition, brute force (the first common neighbour x,
   descending, with max(w(a,x), w(b,x)) <= d is the zero-pivot cofacet; the
   edge is apparent iff it is that triangle's first facet of diameter d in
-  ascending index order), on small graphs with many ties; and the host
-  reducer's own test (tests/host_harness.cpp -> Cohomology::edge_apparent,
-  whose diagrams match the reference bar for bar) on larger ones.
+  ascending index order), on small graphs with many ties; and the oracle's
+  restatement (tto_apparent_cx, pinned against the reference's
+  zero_apparent_cofacet) on larger ones.
 * End to end, the kNN and radius branches of persistence() already match the
   reference bar for bar (test_persistence_gpu.py); they run through this path.
 """
-import ctypes as C
 import math
-import os
 
 import numpy as np
 import pytest
-
-from conftest import ROOT
 
 pytestmark = pytest.mark.gpu
 
@@ -106,31 +102,19 @@ def test_sparse_edges_brute_force(gpu, seed, n, m, ties, thr):
         assert ap.any() and not ap.all()
 
 
-@pytest.fixture(scope="module")
-def harness():
-    path = os.path.join(ROOT, "tests", "libhost_harness.so")
-    if not os.path.exists(path):
-        pytest.skip("tests/libhost_harness.so not built (make -C tests)")
-    L = C.CDLL(path)
-    L.hh_sparse_apparent.argtypes = [C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p,
-                                     C.c_void_p, C.c_uint64, C.c_void_p]
-    return L
-
-
 @pytest.mark.parametrize("kind", ["knn", "radius"])
-def test_sparse_edges_vs_host_reducer(gpu, harness, kind):
+def test_sparse_edges_vs_oracle(gpu, restatement, kind):
+    from oracle import Complex
     from turbotops_amd import knn_graph_exact, radius_graph, sparse_edges
     rng = np.random.default_rng(31 if kind == "knn" else 32)
     x = rng.uniform(size=(3000, 4)).astype(np.float32)
     g = knn_graph_exact(x, "l2", 8) if kind == "knn" else radius_graph(x, "l2", 0.22)
-    print(f"case sparse vs host {kind} n=3000 nnz={g.row_ptr[-1]}", flush=True)
+    print(f"case sparse vs oracle {kind} n=3000 nnz={g.row_ptr[-1]}", flush=True)
     idx, d, ap = sparse_edges(g)
     want_idx, want_d = _kept_sorted(g.n, g.row_ptr, g.col_idx, g.weights, math.inf)
     assert np.array_equal(idx, want_idx) and np.array_equal(d, want_d)
-    flags = np.zeros(idx.size, np.uint8)
-    rc = harness.hh_sparse_apparent(g.n, g.row_ptr.ctypes.data, g.col_idx.ctypes.data, g.weights.ctypes.data, 1,
-                                    idx.ctypes.data, d.ctypes.data, idx.size, flags.ctypes.data)
-    assert rc == 0
+    cx = Complex(n=g.n, row_ptr=g.row_ptr, col=g.col_idx, w=g.weights)
+    flags = restatement.apparent_cx(cx, 2, [int(v) for v in idx], d) & 1
     assert np.array_equal(ap, flags)
     assert 0 < ap.sum() < ap.size
 

@@ -10,6 +10,7 @@ from . import capi  # noqa: F401
 from .api import BatchPersistence, persistence, persistence_batch, persistence_pairs  # noqa: F401
 from .frontend import (  # noqa: F401
     CsrGraph,
+    DeviceComplex,
     build_edges,
     build_edges_csr,
     dense_edges,

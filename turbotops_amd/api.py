@@ -22,7 +22,8 @@ def _ext():
 
 
 def persistence(x, maxdim=1, threshold=math.inf, metric="l2", graph="full", k=15,
-                approximate=False, recall_samples=0, threads=0, memory_budget=0):
+                approximate=False, recall_samples=0, threads=0, memory_budget=0, *, _mode="auto",
+                _force_wide=False):
     """Persistence diagrams of the Vietoris-Rips filtration.
 
     x: array (n, d) or scipy.sparse matrix (n, n); maxdim: highest homology
@@ -32,7 +33,10 @@ def persistence(x, maxdim=1, threshold=math.inf, metric="l2", graph="full", k=15
     graph: "full" or "knn"; k: neighbours for graph="knn"; approximate:
     HNSW kNN (not part of this build: raises); threads: host worker
     threads; memory_budget: bytes for the host copy of the dense matrix
-    (0 = half of RAM)."""
+    (0 = half of RAM). Not in the reference's signature, for tests: _mode
+    ("auto", "boundary", "coboundary": the engine, as the reference CLI's
+    --mode) and _force_wide (128-bit simplex indices in the cohomology
+    engine, as rips::compute's force_wide)."""
     ext = _ext()
     if hasattr(x, "tocsr"):
         g = x.tocsr()
@@ -41,13 +45,14 @@ def persistence(x, maxdim=1, threshold=math.inf, metric="l2", graph="full", k=15
             np.ascontiguousarray(g.indptr, dtype=np.int64),
             np.ascontiguousarray(g.indices, dtype=np.int64),
             np.ascontiguousarray(g.data, dtype=np.float32),
-            int(g.shape[0]), int(maxdim), float(threshold), int(threads))
+            int(g.shape[0]), int(maxdim), float(threshold), int(threads), bool(_force_wide))
     x = np.ascontiguousarray(x, dtype=np.float32)
     if x.ndim != 2:
         raise ValueError("x must be a 2-d array (n, d)")
     return ext.persistence_points(
         x, int(maxdim), float(threshold), str(metric), str(graph), int(k),
-        bool(approximate), int(recall_samples), int(threads), int(memory_budget))
+        bool(approximate), int(recall_samples), int(threads), int(memory_budget), str(_mode),
+        bool(_force_wide))
 
 
 def persistence_pairs(x, maxdim=1, threshold=math.inf, metric="l2", graph="full", k=15, threads=0,

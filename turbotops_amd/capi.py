@@ -57,6 +57,12 @@ _SIGS = {
     "tt_build_edges_witness": ([_vp, C.c_uint32, C.c_uint32, C.c_int, C.c_float, C.c_float, C.c_int, _vp, _u64p,
                                 _vp, _vp, _f32p], None),
     "tt_build_edges_csr": ([C.c_uint32, _vp, _vp, _vp, C.c_float, _u64p, _vp, _vp], None),
+    "tt_complex_dense": ([_vp, C.c_uint32, C.c_int, C.c_int, C.c_float, C.c_uint32, C.c_int, _vp], None),
+    "tt_complex_sparse": ([C.c_uint32, _vp, _vp, _vp, C.c_float, C.c_uint32, C.c_int, _vp], None),
+    "tt_complex_emergent": ([_vp, C.c_uint32, _vp, _vp, C.c_uint64, _vp, _vp], None),
+    "tt_complex_seed": ([_vp, C.c_uint32, _vp, _vp, C.c_uint64], None),
+    "tt_complex_assemble": ([_vp, _vp, C.c_uint64, C.c_int, _u64p, _vp, _vp, _vp, _vp, _u64p], None),
+    "tt_complex_destroy": ([_vp], C.c_int),
     "tt_free": ([_vp], C.c_int),
 }
 

@@ -340,3 +340,90 @@ def mst_h0(x, metric="l2", threshold=math.inf):
     fin = deaths[(deaths <= thr) & (deaths > 0)]
     d = np.concatenate([fin, np.full(essential, np.inf, np.float32)]).astype(np.float32)
     return np.zeros_like(d), d
+
+
+def _words(values, words):
+    vals = [int(v) for v in values]
+    if words == 1:
+        return np.array(vals, dtype=np.uint64)
+    return np.array([[v & (2**64 - 1), v >> 64] for v in vals], dtype=np.uint64).reshape(-1, 2)
+
+
+def _values(arr, words):
+    a = np.asarray(arr, dtype=np.uint64)
+    if words == 1:
+        return [int(v) for v in a.ravel()]
+    return [int(lo) | (int(hi) << 64) for lo, hi in a.reshape(-1, 2)]
+
+
+class DeviceComplex:
+    """The coboundary reducer's complex in HBM (include/ttgpu.h tt_complex_*):
+    the per-simplex work above dimension 0 — emergent candidates of columns
+    (rips.hpp:903-928) and the next dimension's columns (Reducer::assemble,
+    :981-1023). dense: an (n, n) float32 matrix (full layout) or the flat
+    lower triangle; sparse: a CsrGraph. Indices are Python ints; words is
+    the index width in 64-bit words (1 or 2)."""
+
+    def __init__(self, dense=None, graph=None, threshold=math.inf, maxdim=2, words=1):
+        lib = capi.lib()
+        self.words = int(words)
+        h = C.c_void_p()
+        if dense is not None:
+            m = _f32(dense)
+            full = m.ndim == 2
+            n = m.shape[0] if full else int((1 + math.isqrt(1 + 8 * m.size)) // 2)
+            self._keep = m
+            lib.call("tt_complex_dense", _addr(m), n, capi.LAYOUT_FULL if full else capi.LAYOUT_LOWER, 0,
+                     float(threshold), int(maxdim), self.words, C.byref(h))
+        else:
+            g = graph
+            self._keep = g
+            lib.call("tt_complex_sparse", g.n, _addr(g.row_ptr), _addr(g.col_idx), _addr(g.weights),
+                     float(threshold), int(maxdim), self.words, C.byref(h))
+        self._h = h
+
+    def close(self):
+        if self._h:
+            capi.lib().dll.tt_complex_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def emergent(self, nv, idx, d):
+        """(cofacets, states) per column: state 1 a candidate emergent pair,
+        2 its cofacet has a zero-apparent facet, 0 none."""
+        w = self.words
+        iw = _words(idx, w)
+        d = _f32(d)
+        cof = np.zeros((max(len(d), 1), w), np.uint64)
+        st = np.zeros(max(len(d), 1), np.uint8)
+        capi.lib().call("tt_complex_emergent", self._h, int(nv), _addr(iw), _addr(d), len(d), _addr(cof),
+                        _addr(st))
+        return _values(cof[: len(d)], w), st[: len(d)]
+
+    def seed(self, nv, idx, d):
+        iw = _words(idx, self.words)
+        d = _f32(d)
+        capi.lib().call("tt_complex_seed", self._h, int(nv), _addr(iw), _addr(d), len(d))
+
+    def assemble(self, pivots=(), keep_next=False):
+        """(columns [(idx, d)], (cofacets, states), skipped)."""
+        lib = capi.lib()
+        w = self.words
+        pv = _words(pivots, w)
+        nc, sk = C.c_uint64(), C.c_uint64()
+        pi, pc = C.POINTER(C.c_uint64)(), C.POINTER(C.c_uint64)()
+        pd = _f32p()
+        ps = C.POINTER(C.c_uint8)()
+        lib.call("tt_complex_assemble", self._h, _addr(pv), len(pv), int(bool(keep_next)), C.byref(nc),
+                 C.byref(pi), C.byref(pd), C.byref(pc), C.byref(ps), C.byref(sk))
+        k = nc.value
+        idx = _values(_take(lib, pi, k * w, C.c_uint64, np.uint64), w)
+        d = _take(lib, pd, k, C.c_float, np.float32)
+        cof = _values(_take(lib, pc, k * w, C.c_uint64, np.uint64), w)
+        st = _take(lib, ps, k, C.c_uint8, np.uint8)
+        return list(zip(idx, d.tolist())), (cof, st), sk.value

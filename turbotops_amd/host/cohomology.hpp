@@ -1,24 +1,31 @@
 // SPDX-License-Identifier: MIT
-// reducer.hpp — host-side persistent cohomology of a Rips filtration from
-// front-end outputs computed on the GPU.
+// cohomology.hpp — the host half of the cohomology (coboundary) engine: the
+// serial column reduction that remains once the GPU has done the
+// per-simplex work.
 //
-// The algorithm is Ripser's (U. Bauer, "Ripser: efficient computation of
-// Vietoris-Rips persistence barcodes", J. Appl. Comput. Topol. 2021), the one
-// the reference's coboundary engine implements (include/turbotops/rips.hpp:
-// 489-1032): simplices are addressed by the combinatorial number system;
-// columns are coboundaries enumerated on demand and reduced in a binary heap
-// with mod-2 cancellation; apparent pairs and emergent pairs skip reduction;
-// clearing drops every column that is already a pivot of the previous
-// dimension. Filtration order is (diameter ascending, index descending)
-// (rips.hpp:116-118). The diagram does not depend on the reduction strategy,
-// only on the filtration.
+// The filtration and the pairs are those of Ripser's persistent cohomology
+// (U. Bauer, J. Appl. Comput. Topol. 2021), the algorithm behind the
+// reference's coboundary engine (include/turbotops/rips.hpp:489-1032):
+// simplices are named by the combinatorial number system and ordered by
+// (diameter ascending, index descending) (rips.hpp:116-118); a column's
+// coboundary is reduced mod 2 against earlier pivots; pairs that the
+// filtration makes apparent are skipped; pivots of one dimension clear the
+// columns of the next.
 //
-// Dimension 0 does not run here: the GPU hands over the kept edges in
-// filtration order with their apparent flags (edges.hip), and the union-find
-// below only splits them into H0 deaths and dimension-1 columns.
+// What runs where:
+//   GPU (csrc/edges.hip, sparse.hip)  dimension-0 edges in filtration order
+//                                     with their apparent flags
+//   host (dim0 below)                 union-find: H0 deaths, dimension-1
+//                                     columns
+//   GPU (csrc/columns.hip, through a ColumnSource)
+//                                     each next dimension's columns (cofacets
+//                                     neither cleared nor apparent, sorted),
+//                                     and per column its emergent candidate
+//   host (reduce below)               the column reduction itself; only
+//                                     pivot-dependent decisions are made here
 //
-// Besides the bars, every finite pair records its birth and death simplices
-// (for the gradient of a diagram with respect to the points).
+// Every finite pair also records its birth and death simplices (for the
+// gradient of a diagram with respect to the points).
 #pragma once
 
 #include <algorithm>
@@ -29,7 +36,6 @@
 #include <cstring>
 #include <stdexcept>
 #include <string>
-#include <thread>
 #include <type_traits>
 #include <vector>
 
@@ -72,44 +78,72 @@ struct Telemetry {
     std::vector<u64> columns, apparent, emergent;
 };
 
-// C(i, k) for i <= n, k <= K with overflow detection.
+// Pascal's triangle up to C(n, K): row k holds C(0..n, k). Construction fails
+// with overflow_error when a value does not fit I.
 template <class I>
-struct Binomials {
-    u32 n = 0, K = 0;
-    std::vector<I> t;   // t[k * (n + 1) + i]
-    Binomials(u32 nn, u32 KK) : n(nn), K(KK), t(u64(nn + 1) * (KK + 1), I(0)) {
-        for (u32 i = 0; i <= n; ++i) {
-            t[i] = 1;
-            for (u32 k = 1; k <= std::min(i, K); ++k) {
-                I s;
-                if (__builtin_add_overflow(at(i - 1, k - 1), at(i - 1, k), &s))
+class Binomials {
+  public:
+    Binomials(u32 n, u32 K) : rows_(K + 1, std::vector<I>(u64(n) + 1, I(0))) {
+        for (u32 v = 0; v <= n; ++v) rows_[0][v] = 1;
+        for (u32 k = 1; k <= K; ++k)
+            for (u32 v = k; v <= n; ++v)
+                if (__builtin_add_overflow(rows_[k][v - 1], rows_[k - 1][v - 1], &rows_[k][v]))
                     throw std::overflow_error("rips: simplex index overflow");
-                t[u64(k) * (n + 1) + i] = s;
-            }
-        }
     }
-    I at(u32 i, u32 k) const { return k > K ? I(0) : t[u64(k) * (n + 1) + i]; }
-    I operator()(u32 i, u32 k) const { return at(i, k); }
+    I operator()(u32 v, u32 k) const { return k < rows_.size() ? rows_[k][v] : I(0); }
+    const I* row(u32 k) const { return rows_[k].data(); }
+
+  private:
+    std::vector<std::vector<I>> rows_;
 };
 
+// Simplex indices with up to maxdim + 2 vertices fit I, the all-ones value
+// left free as the "none" sentinel.
 template <class I>
 bool index_fits(u32 n, u32 maxdim) {
     try {
-        Binomials<I> b(n, maxdim + 2);
-        return b(n, maxdim + 2) < kNone<I>;
+        const Binomials<I> b(n, maxdim + 2);
+        return b(n, maxdim + 2) != kNone<I>;
     } catch (const std::overflow_error&) {
         return false;
     }
 }
 
 // ---- complexes ----------------------------------------------------------------
-// Both expose n, threshold, dist(i, j) and
-//   cofacets(vd, k, diam, all, cap, emit): vertices w not in the simplex (vd:
-//   k vertices, descending) with max(diam, d(v, w) for v in vd) <= cap, in
-//   descending w (all == false: only w > vd[0]); emit(w, t, cd) with t the
-//   number of simplex vertices above w; emit returns false to stop.
+// Both views answer:
+//   dist(i, j)
+//   coboundary(U, k, diam, cap, f): every cofacet U + {w} (w outside U, U: k
+//     vertices descending) with cd = max(diam, d(u, w) for u in U) <= cap, in
+//     ascending w, as f(index, cd) — indices from the table B
+//   zpc_above(U, k, d, above): the largest w > above outside U with every
+//     d(u, w) <= d (the vertex that makes U's zero-pivot cofacet), or ~0u
+//
+// The cofacet index of U + {w} with t vertices of U above w is
+//   sum_{i<t} C(u_i, k+1-i) + C(w, k+1-t) + sum_{i>=t} C(u_i, k-i);
+// as w ascends past u_{t-1} that vertex moves from the first sum (its term
+// C(u, k+1-i)) to the last (C(u, k-i)), so both sums are kept running.
+template <class I>
+struct IndexWalk {
+    const Binomials<I>& B;
+    const u32* U;
+    u32 k, t;
+    I above_sum = 0, below_sum = 0;
+    IndexWalk(const Binomials<I>& b, const u32* u, u32 kk) : B(b), U(u), k(kk), t(kk) {
+        for (u32 i = 0; i < k; ++i) above_sum += B(U[i], k + 1 - i);
+    }
+    // index of U + {w}; w must not decrease between calls
+    I at(u32 w) {
+        while (t > 0 && U[t - 1] < w) {
+            --t;
+            above_sum -= B(U[t], k + 1 - t);
+            below_sum += B(U[t], k - t);
+        }
+        return above_sum + B(w, k + 1 - t) + below_sum;
+    }
+};
 
-// Dense distances, full (n x n) or lower (i(i-1)/2 + j) layout.
+constexpr u32 kNoVertex = ~0u;
+
 struct DenseView {
     u32 n = 0;
     f32 threshold = INFINITY;
@@ -123,58 +157,57 @@ struct DenseView {
         return mat[u64(i) * (i - 1) / 2 + j];
     }
 
-    template <class Emit>
-    void cofacets(const u32* vd, u32 k, f32 diam, bool all, f32 cap, Emit&& emit) const {
-        const u32 stop = all ? 0 : vd[0] + 1;
-        u32 t = 0;
-        if (full) {
-            constexpr u32 B = 64;
-            f32 mx[B];
-            for (u32 hi = n; hi > stop;) {
-                const u32 lo = hi - std::min<u32>(B, hi - stop);
-                const u32 len = hi - lo;
-                const f32* r0 = mat + u64(vd[0]) * n + lo;
-                for (u32 j = 0; j < len; ++j) mx[j] = r0[j];
-                for (u32 i = 1; i < k; ++i) {
-                    const f32* ri = mat + u64(vd[i]) * n + lo;
-                    for (u32 j = 0; j < len; ++j) mx[j] = mx[j] < ri[j] ? ri[j] : mx[j];
+    template <class I, class F>
+    void coboundary(const Binomials<I>& B, const u32* U, u32 k, f32 diam, f32 cap, F&& f) const {
+        if (!(diam <= cap)) return;
+        IndexWalk<I> walk(B, U, k);
+        if (!full) {
+            for (u32 w = 0; w < n; ++w) {
+                f32 cd = diam;
+                bool ok = true;
+                for (u32 i = 0; i < k && ok; ++i) {
+                    if (U[i] == w) ok = false;
+                    else cd = std::max(cd, dist(U[i], w));
                 }
-                for (u32 j = len; j-- > 0;) {
-                    const f32 m = mx[j] < diam ? diam : mx[j];
-                    if (!(m <= cap)) continue;
-                    const u32 w = lo + j;
-                    while (t < k && vd[t] > w) ++t;
-                    if (t < k && vd[t] == w) continue;
-                    if (!emit(w, t, m)) return;
-                }
-                hi = lo;
+                if (ok && cd <= cap) f(walk.at(w), cd);
             }
             return;
         }
-        for (u32 wi = n; wi-- > stop;) {
-            const u32 w = wi;
-            while (t < k && vd[t] > w) ++t;
-            if (t < k && vd[t] == w) continue;
-            f32 cd = diam;
-            bool ok = true;
-            for (u32 i = 0; i < k; ++i) {
-                const f32 dv = dist(vd[i], w);
-                if (dv > cd) {
-                    cd = dv;
-                    if (!(cd <= cap)) {
-                        ok = false;
-                        break;
-                    }
-                }
+        // 64 candidates at a time: a bit mask of the w whose distance to every
+        // vertex is within the cap, then the set bits in ascending order
+        for (u32 base = 0; base < n; base += 64) {
+            const u32 len = std::min<u32>(64, n - base);
+            u64 pass = len == 64 ? ~0ull : ((u64(1) << len) - 1);
+            for (u32 i = 0; i < k && pass; ++i) {
+                const f32* r = mat + u64(U[i]) * n + base;
+                u64 bad = 0;
+                for (u32 j = 0; j < len; ++j) bad |= u64(!(r[j] <= cap)) << j;
+                pass &= ~bad;
+                if (U[i] >= base && U[i] < base + len) pass &= ~(u64(1) << (U[i] - base));
             }
-            if (ok && !(cd <= cap)) ok = false;
-            if (ok && !emit(w, t, cd)) return;
+            while (pass) {
+                const u32 w = base + u32(std::countr_zero(pass));
+                pass &= pass - 1;
+                f32 cd = diam;
+                for (u32 i = 0; i < k; ++i) cd = std::max(cd, mat[u64(U[i]) * n + w]);
+                f(walk.at(w), cd);
+            }
         }
+    }
+
+    u32 zpc_above(const u32* U, u32 k, f32 d, int64_t above) const {
+        for (int64_t w = int64_t(n) - 1; w > above; --w) {
+            bool ok = true;
+            for (u32 i = 0; i < k && ok; ++i) ok = U[i] != u32(w) && dist(U[i], u32(w)) <= d;
+            if (ok) return u32(w);
+        }
+        return kNoVertex;
     }
 };
 
 // A weighted graph in CSR form (rows sorted by neighbour id, both
-// directions, no loops): cofacets are common neighbours.
+// directions, no loops): a simplex is a clique, its cofacets the common
+// neighbours of its vertices.
 struct SparseView {
     u32 n = 0;
     f32 threshold = INFINITY;
@@ -182,53 +215,96 @@ struct SparseView {
     const u32* col = nullptr;
     const f32* w = nullptr;
 
-    f32 dist(u32 i, u32 j) const {
-        if (i == j) return 0.0f;
+    // position of j in row i, or ~0
+    u64 find(u32 i, u32 j) const {
         const u32* b = col + row_ptr[i];
         const u32* e = col + row_ptr[i + 1];
         const u32* it = std::lower_bound(b, e, j);
-        return (it != e && *it == j) ? w[it - col] : INFINITY;
+        return (it != e && *it == j) ? u64(it - col) : ~0ull;
+    }
+    f32 dist(u32 i, u32 j) const {
+        if (i == j) return 0.0f;
+        const u64 p = find(i, j);
+        return p == ~0ull ? INFINITY : w[p];
+    }
+    u32 shortest_row(const u32* U, u32 k) const {
+        u32 best = 0;
+        for (u32 i = 1; i < k; ++i)
+            if (row_ptr[U[i] + 1] - row_ptr[U[i]] < row_ptr[U[best] + 1] - row_ptr[U[best]]) best = i;
+        return best;
     }
 
-    template <class Emit>
-    void cofacets(const u32* vd, u32 k, f32 diam, bool all, f32 cap, Emit&& emit) const {
-        constexpr u32 KM = 33;
-        u64 p[KM], lo[KM];
+    template <class I, class F>
+    void coboundary(const Binomials<I>& B, const u32* U, u32 k, f32 diam, f32 cap, F&& f) const {
+        // the shortest row drives in ascending order; every other row keeps a
+        // cursor that only moves forward
+        constexpr u32 KC = 34;
+        u64 cur[KC], end[KC];
         for (u32 i = 0; i < k; ++i) {
-            lo[i] = row_ptr[vd[i]];
-            const u64 e = row_ptr[vd[i] + 1];
-            if (e == lo[i]) return;
-            p[i] = e - 1;
+            cur[i] = row_ptr[U[i]];
+            end[i] = row_ptr[U[i] + 1];
         }
-        const u32 stop = all ? 0 : vd[0] + 1;
-        for (;;) {
-            // the largest id present in every row at or below the cursors
-            u32 x = col[p[0]];
-            bool agree = false;
-            while (!agree) {
-                agree = true;
-                for (u32 i = 0; i < k; ++i) {
-                    while (col[p[i]] > x) {
-                        if (p[i] == lo[i]) return;
-                        --p[i];
-                    }
-                    if (col[p[i]] < x) {
-                        x = col[p[i]];
-                        agree = false;
-                    }
+        const u32 drv = shortest_row(U, k);
+        IndexWalk<I> walk(B, U, k);
+        for (u64 p = cur[drv]; p < end[drv]; ++p) {
+            const u32 x = col[p];
+            f32 cd = std::max(diam, w[p]);
+            bool ok = cd <= cap;
+            for (u32 i = 0; i < k && ok; ++i) {
+                if (i == drv) continue;
+                while (cur[i] < end[i] && col[cur[i]] < x) ++cur[i];
+                ok = cur[i] < end[i] && col[cur[i]] == x;
+                if (ok) {
+                    cd = std::max(cd, w[cur[i]]);
+                    ok = cd <= cap;
                 }
             }
-            if (x < stop) return;
-            u32 t = 0;
-            while (t < k && vd[t] > x) ++t;
-            f32 cd = diam;
-            for (u32 i = 0; i < k; ++i) cd = cd < w[p[i]] ? w[p[i]] : cd;
-            if (cd <= cap && !emit(x, t, cd)) return;
-            if (p[0] == lo[0]) return;
-            --p[0];
+            // a vertex of U is never its own neighbour, so x outside U
+            // already follows from x being in every row
+            if (ok) f(walk.at(x), cd);
         }
     }
+
+    u32 zpc_above(const u32* U, u32 k, f32 d, int64_t above) const {
+        const u32 drv = shortest_row(U, k);
+        const u64 lo = row_ptr[U[drv]];
+        for (u64 p = row_ptr[U[drv] + 1]; p > lo; --p) {
+            const u32 x = col[p - 1];
+            if (int64_t(x) <= above) break;
+            if (!(w[p - 1] <= d)) continue;
+            bool ok = true;
+            for (u32 i = 0; i < k && ok; ++i) {
+                if (i == drv) continue;
+                if (U[i] == x) ok = false;
+                else {
+                    const u64 q = find(U[i], x);
+                    ok = q != ~0ull && w[q] <= d;
+                }
+            }
+            if (ok) return x;
+        }
+        return kNoVertex;
+    }
 };
+
+// Position (in U, descending) of the vertex whose removal gives the
+// zero-pivot facet — the first facet in ascending index order keeping an
+// edge of length d — or k if there is none: the first position outside the
+// intersection of the end points of the edges of length d.
+template <class Cx>
+u32 zpf_position(const Cx& cx, const u32* U, u32 k, f32 d) {
+    u64 common = ~0ull;
+    bool any = false;
+    for (u32 a = 0; a < k; ++a)
+        for (u32 b = a + 1; b < k; ++b)
+            if (cx.dist(U[a], U[b]) == d) {
+                common &= (u64(1) << a) | (u64(1) << b);
+                any = true;
+            }
+    if (!any) return k;
+    const u32 j = u32(std::countr_zero(~common));
+    return j < k ? j : k;
+}
 
 // ---- pivot map (open addressing) -------------------------------------------
 template <class I>
@@ -275,6 +351,11 @@ struct PivotTable {
         reset(ok.size());
         for (u64 i = 0; i < ok.size(); ++i)
             if (ok[i] != kNone<I>) put(ok[i], ov[i]);
+    }
+    template <class F>
+    void each_key(F&& f) const {
+        for (const I& k : key)
+            if (k != kNone<I>) f(k);
     }
 };
 
@@ -356,10 +437,108 @@ class RadixHeap {
     }
 };
 
+// ---- a set under symmetric difference ---------------------------------------------
+// The simplices added to a column, modulo 2: toggling a member removes it.
+// Open addressing over slots that point into a dense item array, so the
+// members are read out as one array; removal moves the last item into the
+// hole, and deleted slots are tombstones until the next rebuild.
+template <class S>
+class ParitySet {
+  public:
+    void clear() {
+        if (!items_.empty() || tombs_) std::fill(slot_.begin(), slot_.end(), kEmpty);
+        items_.clear();
+        where_.clear();
+        tombs_ = 0;
+    }
+    void toggle(const S& s) {
+        if (slot_.empty() || 2 * (items_.size() + tombs_ + 1) > slot_.size()) rebuild();
+        const u64 mask = slot_.size() - 1;
+        u64 h = mix(s.idx) & mask;
+        u64 hole = ~0ull;
+        for (;; h = (h + 1) & mask) {
+            const u32 e = slot_[h];
+            if (e == kEmpty) break;
+            if (e == kTomb) {
+                if (hole == ~0ull) hole = h;
+                continue;
+            }
+            if (items_[e].idx == s.idx) {   // present: remove
+                slot_[h] = kTomb;
+                ++tombs_;
+                const u32 last = u32(items_.size() - 1);
+                if (e != last) {
+                    items_[e] = items_[last];
+                    where_[e] = where_[last];
+                    slot_[where_[e]] = e;
+                }
+                items_.pop_back();
+                where_.pop_back();
+                return;
+            }
+        }
+        if (hole != ~0ull) {
+            h = hole;
+            --tombs_;
+        }
+        slot_[h] = u32(items_.size());
+        items_.push_back(s);
+        where_.push_back(h);
+    }
+    const std::vector<S>& items() const { return items_; }
+
+  private:
+    static constexpr u32 kEmpty = ~0u, kTomb = ~0u - 1;
+    std::vector<u32> slot_;
+    std::vector<S> items_;
+    std::vector<u64> where_;
+    u64 tombs_ = 0;
+    template <class I>
+    static u64 mix(I k) {
+        u64 x = u64(k) + u64(k >> 32 >> 32) * 0xD6E8FEB86659FD93ull;
+        x ^= x >> 32;
+        x *= 0xD6E8FEB86659FD93ull;
+        x ^= x >> 32;
+        return x;
+    }
+    void rebuild() {
+        u64 cap = 64;
+        while (cap < 4 * (items_.size() + 1)) cap <<= 1;
+        slot_.assign(cap, kEmpty);
+        tombs_ = 0;
+        for (u32 e = 0; e < items_.size(); ++e) {
+            u64 h = mix(items_[e].idx) & (cap - 1);
+            while (slot_[h] != kEmpty) h = (h + 1) & (cap - 1);
+            slot_[h] = e;
+            where_[e] = h;
+        }
+    }
+};
+
+// ---- where the columns come from -------------------------------------------
+// Simplex indices cross this interface as 64-bit words, one or two per
+// index (low word first), as the C ABI (include/ttgpu.h) passes them.
+struct ColumnSource {
+    virtual ~ColumnSource() = default;
+    // per column (nv vertices, filtration values d): the zero-pivot cofacet
+    // (cof) and its state: 0 none, 1 a candidate emergent pair (the column
+    // pairs with it unless it is already a pivot), 2 not a candidate (the
+    // cofacet has a zero-apparent facet)
+    virtual void emergent(u32 nv, const u64* idx, const f32* d, u64 count, u64* cof, u8* state) = 0;
+    // the simplices the next assemble starts from
+    virtual void seed(u32 nv, const u64* idx, const f32* d, u64 count) = 0;
+    // the columns of the next dimension from the seeded simplices, the
+    // given pivots cleared, in reduction order, with their emergent states;
+    // keep_next: the cofacets become the seeded simplices
+    virtual void assemble(const u64* pivots, u64 npivots, bool keep_next, std::vector<u64>& idx,
+                          std::vector<f32>& d, std::vector<u64>& cof, std::vector<u8>& state, u64& skipped) = 0;
+};
+
 // ---- the reducer ---------------------------------------------------------------
 template <class I, class Cx>
 class Cohomology {
   public:
+    static constexpr u32 kWords = sizeof(I) / 8;
     // A simplex in the filtration. Packed: 12 bytes with 64-bit indices, 20
     // with 128-bit ones (32 unpacked), which the column heaps move around.
     struct __attribute__((packed)) S {
@@ -372,10 +551,9 @@ class Cohomology {
     Cohomology(const Cx& cx, u32 maxdim, Diagram& out, Telemetry& tel)
         : cx_(cx), n_(cx.n), maxdim_(maxdim), B_(cx.n, maxdim + 2), out_(out), tel_(tel) {}
 
-    // host threads for assemble() (the columns of the next dimension)
-    void set_threads(u32 t) { threads_ = std::max<u32>(1, t); }
-
-    // Dimension 0 from the GPU's edges (filtration order) and apparent flags.
+    // Dimension 0 from the GPU's edges (filtration order) and apparent flags:
+    // a spanning forest by union-find gives the H0 deaths; the other edges
+    // that are not apparent are the dimension-1 columns, latest first.
     void dim0(const u64* idx, const f32* d, const u8* apparent, u64 m) {
         std::vector<u32> parent(n_), rank(n_, 0);
         for (u32 v = 0; v < n_; ++v) parent[v] = v;
@@ -386,7 +564,7 @@ class Cohomology {
             }
             return x;
         };
-        cols_.clear();
+        cur_.clear();
         u64 skipped = 0;
         for (u64 e = 0; e < m; ++e) {
             u32 a, b;
@@ -399,56 +577,70 @@ class Cohomology {
                 if (d[e] > 0.0f) out_.add(0, 0.0f, d[e], u128(a), u128(idx[e]));
             } else if (maxdim_ >= 1) {
                 if (apparent && apparent[e]) ++skipped;
-                else cols_.push_back(S{I(idx[e]), d[e]});
+                else cur_.push_back(S{I(idx[e]), d[e]});
             }
         }
         for (u32 v = 0; v < n_; ++v)
             if (find(v) == v) out_.add(0, 0.0f, INFINITY, u128(v), u128(kNone<I>));
-        std::reverse(cols_.begin(), cols_.end());
-        if (maxdim_ >= 2) {
-            simplices_.resize(m);
-            for (u64 e = 0; e < m; ++e) simplices_[e] = S{I(idx[e]), d[e]};
-        }
+        std::reverse(cur_.begin(), cur_.end());
+        edges_ = idx;
+        edge_d_ = d;
+        edge_count_ = m;
         tel_.edges = m;
         tel_.apparent.push_back(skipped);
     }
 
-    // dim-0 apparent test of one edge (rips.hpp:844-849): the edge is the
-    // zero-pivot facet of its zero-pivot cofacet.
-    bool edge_apparent(u64 idx, f32 d) const {
-        u32 vd[2];
-        decode_edge(idx, vd[0], vd[1]);
-        return apparent_cofacet_of(S{I(idx), d}, vd, 2).idx != kNone<I>;
-    }
-
-    void run_from_dim1() {
+    // Dimensions 1..maxdim: columns and emergent states from `src`.
+    void run(ColumnSource& src) {
+        if (maxdim_ >= 2) {
+            std::vector<u64> w(edge_count_ * kWords, 0);
+            for (u64 e = 0; e < edge_count_; ++e) w[e * kWords] = edges_[e];
+            src.seed(2, w.data(), edge_d_, edge_count_);
+        }
+        std::vector<u64> wi, wc, piv;
+        std::vector<f32> wd;
         for (u32 dim = 1; dim <= maxdim_; ++dim) {
+            if (dim == 1) {
+                to_words(cur_, wi);
+                wd.resize(cur_.size());
+                for (size_t c = 0; c < cur_.size(); ++c) wd[c] = cur_[c].d;
+                wc.assign(cur_.size() * kWords, 0);
+                state_.assign(cur_.size(), 0);
+                src.emergent(2, wi.data(), wd.data(), cur_.size(), wc.data(), state_.data());
+            } else {
+                u64 skipped = 0;
+                src.assemble(piv.data(), piv.size() / kWords, dim < maxdim_, wi, wd, wc, state_, skipped);
+                tel_.apparent.push_back(skipped);
+                cur_.resize(wd.size());
+                for (size_t c = 0; c < wd.size(); ++c) cur_[c] = S{word_index(wi.data(), c), wd[c]};
+            }
+            cof_.resize(cur_.size());
+            for (size_t c = 0; c < cur_.size(); ++c) cof_[c] = word_index(wc.data(), c);
             if (!(dim == 1 && reduce_dim1_lazy())) reduce(dim);
-            if (dim < maxdim_) assemble(dim);
+            if (dim < maxdim_) {
+                piv.clear();
+                pivots_.each_key([&](I k) {
+                    piv.push_back(u64(k));
+                    if (kWords == 2) piv.push_back(u64(k >> 32 >> 32));
+                });
+            }
         }
     }
 
     // ---- index arithmetic -------------------------------------------------
-    // largest v < top with C(v, k) <= idx
-    u32 max_vertex(I idx, u32 k, u32 top) const {
-        u32 lo = k - 1, hi = top;   // C(lo, k) = 0 <= idx
-        while (hi - lo > 1) {
-            const u32 mid = lo + (hi - lo) / 2;
-            if (B_(mid, k) <= idx) lo = mid;
-            else hi = mid;
-        }
-        return lo;
-    }
-    void decode(I idx, u32 nv, u32* vd) const {
+    // vertices of simplex `idx` with k vertices, descending: for each
+    // position the largest v below the previous vertex with C(v, pos) <= idx
+    void decode(I idx, u32 k, u32* U) const {
         u32 top = n_;
-        for (u32 i = 0; i < nv; ++i) {
-            const u32 k = nv - i;
-            const u32 v = max_vertex(idx, k, top);
-            vd[i] = v;
-            idx -= B_(v, k);
+        for (u32 pos = k; pos >= 1; --pos) {
+            const I* row = B_.row(pos);
+            const u32 v = u32(std::upper_bound(row + pos - 1, row + top, idx) - row) - 1;
+            U[k - pos] = v;
+            idx -= row[v];
             top = v;
         }
     }
+    // edge (a, b), a > b, of index a(a-1)/2 + b
     static void decode_edge(u64 idx, u32& a, u32& b) {
         u64 v = u64((1.0 + std::sqrt(1.0 + 8.0 * double(idx))) * 0.5);
         while (v * (v - 1) / 2 > idx) --v;
@@ -456,83 +648,39 @@ class Cohomology {
         a = u32(v);
         b = u32(idx - v * (v - 1) / 2);
     }
-    I index_of(const u32* vd, u32 nv) const {
+    I index_of(const u32* U, u32 k) const {
         I s = 0;
-        for (u32 i = 0; i < nv; ++i) s += B_(vd[i], nv - i);
+        for (u32 i = 0; i < k; ++i) s += B_(U[i], k - i);
         return s;
     }
 
   private:
-    static constexpr u32 KM = 33;
+    static constexpr u32 KM = 34;
 
-    // cofacets of (vd, nv, diam) as simplices, in descending index order
-    template <class F>
-    void cofacet_simplices(const u32* vd, u32 nv, f32 diam, bool all, f32 cap, F&& f) const {
-        I pre[KM + 1], suf[KM + 1];   // pre[t] = sum_{i<t} C(vd[i], nv+1-i); suf[t] = sum_{i>=t} C(vd[i], nv-i)
-        pre[0] = 0;
-        for (u32 i = 0; i < nv; ++i) pre[i + 1] = pre[i] + B_(vd[i], nv + 1 - i);
-        suf[nv] = 0;
-        for (u32 i = nv; i-- > 0;) suf[i] = suf[i + 1] + B_(vd[i], nv - i);
-        cx_.cofacets(vd, nv, diam, all, cap,
-                     [&](u32 w, u32 t, f32 cd) { return f(S{pre[t] + B_(w, nv + 1 - t) + suf[t], cd}, w, t); });
+    static I word_index(const u64* w, size_t t) {
+        if constexpr (kWords == 2) return (I(w[2 * t + 1]) << 32 << 32) | I(w[2 * t]);
+        else return I(w[t]);
     }
-
-    // zero pivot cofacet: the first cofacet (descending index) of diameter d
-    S zero_pivot_cofacet(const u32* vd, u32 nv, f32 d, u32* cv) const {
-        S found{kNone<I>, 0.0f};
-        u32 fw = 0, ft = 0;
-        cofacet_simplices(vd, nv, d, true, d, [&](const S& c, u32 w, u32 t) {
-            found = c;
-            fw = w;
-            ft = t;
-            return false;
-        });
-        if (found.idx != kNone<I>) {
-            for (u32 i = 0; i < ft; ++i) cv[i] = vd[i];
-            cv[ft] = fw;
-            for (u32 i = ft; i < nv; ++i) cv[i + 1] = vd[i];
+    static void to_words(const std::vector<S>& v, std::vector<u64>& out) {
+        out.assign(v.size() * kWords, 0);
+        for (size_t t = 0; t < v.size(); ++t) {
+            out[t * kWords] = u64(v[t].idx);
+            if constexpr (kWords == 2) out[t * 2 + 1] = u64(v[t].idx >> 32 >> 32);
         }
-        return found;
     }
 
-    // zero pivot facet: the first facet (ascending index) of diameter d
-    S zero_pivot_facet(const u32* vd, u32 nv, f32 d, u32* fv) const {
-        if (nv < 2) return S{kNone<I>, 0.0f};
-        for (u32 j = 0; j < nv; ++j) {
-            bool hit = false;
-            for (u32 a = 0; a < nv && !hit; ++a) {
-                if (a == j) continue;
-                for (u32 b = a + 1; b < nv && !hit; ++b)
-                    if (b != j && cx_.dist(vd[a], vd[b]) == d) hit = true;
-            }
-            if (hit) {
-                u32 o = 0;
-                for (u32 i = 0; i < nv; ++i)
-                    if (i != j) fv[o++] = vd[i];
-                return S{index_of(fv, nv - 1), d};
-            }
-        }
-        return S{kNone<I>, 0.0f};
-    }
-
-    // (f, s) is an apparent pair with s the cofacet: returns f, else none
-    S apparent_facet_of(const S& s, const u32* vd, u32 nv) const {
-        u32 fv[KM], cv[KM + 1];
-        const S f = zero_pivot_facet(vd, nv, s.d, fv);
-        if (f.idx == kNone<I>) return f;
-        return zero_pivot_cofacet(fv, nv - 1, f.d, cv).idx == s.idx ? f : S{kNone<I>, 0.0f};
-    }
-    // (s, c) is an apparent pair with s the facet: returns c, else none
-    S apparent_cofacet_of(const S& s, const u32* vd, u32 nv) const {
-        u32 fv[KM + 1], cv[KM + 1];
-        const S c = zero_pivot_cofacet(vd, nv, s.d, cv);
-        if (c.idx == kNone<I>) return c;
-        return zero_pivot_facet(cv, nv + 1, c.d, fv).idx == s.idx ? c : S{kNone<I>, 0.0f};
-    }
-    S apparent_facet_of(const S& s, u32 nv) const {
-        u32 vd[KM];
-        decode(s.idx, nv, vd);
-        return apparent_facet_of(s, vd, nv);
+    // (F, P) is a zero-apparent pair with the simplex P (k vertices) the
+    // cofacet: F = zpf(P) removes U[j], and no vertex above U[j] extends F
+    // within P's diameter. Returns F, or none.
+    S apparent_facet_of(const S& p, u32 k) const {
+        u32 U[KM], F[KM];
+        decode(p.idx, k, U);
+        const u32 j = zpf_position(cx_, U, k, p.d);
+        if (j >= k) return S{kNone<I>, 0.0f};
+        for (u32 i = 0, o = 0; i < k; ++i)
+            if (i != j) F[o++] = U[i];
+        if (cx_.zpc_above(F, k - 1, p.d, int64_t(U[j])) != kNoVertex) return S{kNone<I>, 0.0f};
+        return S{index_of(F, k - 1), p.d};
     }
 
     // ---- working column: binary min-heap in filtration order -------------
@@ -543,10 +691,6 @@ class Cohomology {
         void push(const S& e) {
             h.push_back(e);
             std::push_heap(h.begin(), h.end(), later);
-        }
-        void load(std::vector<S>& v) {
-            h.swap(v);
-            std::make_heap(h.begin(), h.end(), later);
         }
         S pop_top() {
             std::pop_heap(h.begin(), h.end(), later);
@@ -576,7 +720,17 @@ class Cohomology {
     // before the pivot cancel among themselves and are not pushed.
     bool radix_ = false;
     RadixHeap col_;
+    Heap heap_;
 
+    // the filtration order as one integer: diameter bits (non-negative
+    // floats order as their bits), then the index descending
+    static u128 order_key(const S& e) { return (u128(std::bit_cast<u32>(e.d)) << 64) | u128(~u64(e.idx)); }
+    static S key_simplex(u128 k) { return S{I(~u64(k)), std::bit_cast<f32>(u32(k >> 64))}; }
+
+    void col_clear() {
+        if (radix_) col_.clear();
+        else heap_.clear();
+    }
     void col_push(const S& c) {
         if (!radix_) {
             heap_.push(c);
@@ -584,14 +738,6 @@ class Cohomology {
         }
         const u128 k = order_key(c);
         if (k >= col_.last()) col_.push(k, 0);
-    }
-    void col_load(std::vector<S>& v) {
-        if (!radix_) {
-            heap_.load(v);
-            return;
-        }
-        col_.clear();
-        for (const S& c : v) col_.push(order_key(c), 0);
     }
     // the whole working column in filtration order after mod-2 cancellation,
     // taken out of it
@@ -632,15 +778,11 @@ class Cohomology {
         return S{kNone<I>, 0.0f};
     }
 
-    void push_coboundary(const S& s, u32 nv) {
-        u32 vd[KM];
-        decode(s.idx, nv, vd);
-        cofacet_simplices(vd, nv, s.d, true, cx_.threshold, [&](const S& c, u32, u32) {
-            col_push(c);
-            return true;
-        });
+    void push_coboundary(const S& s, u32 k) {
+        u32 U[KM];
+        decode(s.idx, k, U);
+        cx_.coboundary(B_, U, k, s.d, cx_.threshold, [&](I idx, f32 cd) { col_push(S{idx, cd}); });
     }
-
 
     // ---- dimension 1 on a dense full matrix: lazy cofacet streams --------
     // The same column reduction as reduce(1) — same columns, same pivots,
@@ -796,11 +938,6 @@ class Cohomology {
         return false;
     }
 
-    // the filtration order as one integer: diameter bits (non-negative
-    // floats order as their bits), then the index descending
-    static u128 order_key(const S& e) { return (u128(std::bit_cast<u32>(e.d)) << 64) | u128(~u64(e.idx)); }
-    static S key_simplex(u128 k) { return S{I(~u64(k)), std::bit_cast<f32>(u32(k >> 64))}; }
-
     void lazy_push_stream(const S& edge) {
         u32 id;
         if (free_.empty()) {
@@ -856,7 +993,7 @@ class Cohomology {
         return S{kNone<I>, 0.0f};
     }
 
-    // zero_pivot_cofacet of an edge, cached by edge index
+    // zero-pivot cofacet of an edge, cached by edge index
     I edge_zero_cofacet(u64 eidx, f32 d) {
         if (zc_.key.empty()) zc_.reset(1024);
         const u64 hit = zc_.find(I(eidx));
@@ -893,33 +1030,32 @@ class Cohomology {
         return edge_zero_cofacet(e, t.d) == t.idx ? S{I(e), t.d} : S{kNone<I>, 0.0f};
     }
 
+    void keep_reduction() {
+        for (const S& s : parity_.items()) red_.push_back(s);
+    }
+
     bool reduce_dim1_lazy() {
         if constexpr (!std::is_same_v<Cx, DenseView> || !std::is_same_v<I, u64>) {
             return false;
         } else {
             if (!cx_.full) return false;
-            pivots_.reset(cols_.size());
+            pivots_.reset(cur_.size());
             red_off_.assign(1, 0);
             red_.clear();
             u64 emergent = 0;
-            std::vector<S> added;
-            for (u64 c = 0; c < cols_.size(); ++c) {
-                const S col = cols_[c];
+            for (u64 c = 0; c < cur_.size(); ++c) {
+                const S col = cur_[c];
+                if (state_[c] == 1 && !pivots_.has(cof_[c])) {   // emergent pair
+                    pivots_.put(cof_[c], c);
+                    ++emergent;
+                    red_off_.push_back(red_.size());
+                    continue;
+                }
                 lheap_.clear();
                 free_.clear();
                 for (u32 id = 0; id < pool_.size(); ++id) free_.push_back(id);
-                added.clear();
-                // emergent pair: the first cofacet of equal diameter (phase A head)
+                parity_.clear();
                 lazy_push_stream(col);
-                if (!lheap_.empty() && key_simplex(lheap_.top_key()).d == col.d) {
-                    const S cf = key_simplex(lheap_.top_key());
-                    if (!pivots_.has(cf.idx) && apparent_edge_of_triangle(cf).idx == kNone<I>) {
-                        pivots_.put(cf.idx, c);
-                        ++emergent;
-                        red_off_.push_back(red_.size());
-                        continue;
-                    }
-                }
                 S piv = lazy_pivot();
                 for (;;) {
                     if (piv.idx == kNone<I>) {
@@ -928,10 +1064,10 @@ class Cohomology {
                     }
                     const u64 jc = pivots_.find(piv.idx);
                     if (jc != ~0ull) {
-                        added.push_back(cols_[jc]);
-                        lazy_push_stream(cols_[jc]);
+                        parity_.toggle(cur_[jc]);
+                        lazy_push_stream(cur_[jc]);
                         for (u64 t = red_off_[jc]; t < red_off_[jc + 1]; ++t) {
-                            added.push_back(red_[t]);
+                            parity_.toggle(red_[t]);
                             lazy_push_stream(red_[t]);
                         }
                         piv = lazy_pivot();
@@ -939,23 +1075,19 @@ class Cohomology {
                     }
                     const S fa = apparent_edge_of_triangle(piv);
                     if (fa.idx != kNone<I>) {
-                        added.push_back(fa);
+                        parity_.toggle(fa);
                         lazy_push_stream(fa);
                         piv = lazy_pivot();
                         continue;
                     }
                     if (piv.d > col.d) out_.add(1, col.d, piv.d, u128(col.idx), u128(piv.idx));
                     pivots_.put(piv.idx, c);
-                    std::sort(added.begin(), added.end(), [](const S& a, const S& b) { return a.idx < b.idx; });
-                    for (size_t x = 0; x < added.size();) {
-                        if (x + 1 < added.size() && added[x].idx == added[x + 1].idx) x += 2;
-                        else red_.push_back(added[x++]);
-                    }
+                    keep_reduction();
                     break;
                 }
                 red_off_.push_back(red_.size());
             }
-            tel_.columns.push_back(cols_.size());
+            tel_.columns.push_back(cur_.size());
             tel_.emergent.push_back(emergent);
             return true;
         }
@@ -976,52 +1108,36 @@ class Cohomology {
     enum : u8 { KEEP_NONE, KEEP_REDUCTION, KEEP_COBOUNDARY };
     std::vector<u8> keep_;
 
+    // Columns in reduction order. A column whose emergent candidate is not
+    // yet a pivot pairs with it at once (its coboundary's first entry of
+    // equal diameter, pivot-free and with no apparent facet, is its pivot).
+    // Otherwise the working coboundary is reduced: a pivot that belongs to
+    // an earlier column adds that column; a pivot with a zero-apparent facet
+    // adds the facet; any other pivot closes the column.
     void reduce(u32 dim) {
-        const u32 nv = dim + 1;
-        // cofacets have nv + 1 vertices: indices below C(n, nv + 1)
-        radix_ = std::is_same_v<I, u64> || B_(n_, nv + 1) <= I(~u64(0));
+        const u32 k = dim + 1;
+        radix_ = std::is_same_v<I, u64> || B_(n_, k + 1) <= I(~u64(0));
         constexpr bool store_cob = std::is_same_v<Cx, SparseView>;
-        pivots_.reset(cols_.size());
+        const u64 nc = cur_.size();
+        pivots_.reset(nc);
         red_off_.assign(1, 0);
         red_.clear();
-        keep_.assign(cols_.size(), KEEP_NONE);
+        keep_.assign(nc, KEEP_NONE);
         u64 emergent = 0;
-        std::vector<S> buf;
-        std::vector<S> added;
-        u32 vd[KM];
-        for (u64 c = 0; c < cols_.size(); ++c) {
-            const S col = cols_[c];
-            heap_.clear();
-            added.clear();
-            buf.clear();
-            decode(col.idx, nv, vd);
-            S piv{kNone<I>, 0.0f};
-            bool fresh = false, checking = true;
-            cofacet_simplices(vd, nv, col.d, true, cx_.threshold, [&](const S& cf, u32, u32) {
-                if (checking && cf.d == col.d) {
-                    if (!pivots_.has(cf.idx) && apparent_facet_of(cf, nv + 1).idx == kNone<I>) {
-                        piv = cf;           // emergent pair: the column is reduced already
-                        fresh = true;
-                        return false;
-                    }
-                    checking = false;
-                }
-                buf.push_back(cf);
-                return true;
-            });
-            if (fresh) {
+        for (u64 c = 0; c < nc; ++c) {
+            const S col = cur_[c];
+            if (state_[c] == 1 && !pivots_.has(cof_[c])) {
+                pivots_.put(cof_[c], c);
                 ++emergent;
-            } else {
-                col_load(buf);
-                piv = col_pivot();
+                red_off_.push_back(red_.size());
+                continue;
             }
-            for (;;) {
+            col_clear();
+            parity_.clear();
+            push_coboundary(col, k);
+            for (S piv = col_pivot();;) {
                 if (piv.idx == kNone<I>) {
                     out_.add(dim, col.d, INFINITY, u128(col.idx), u128(kNone<I>));
-                    break;
-                }
-                if (fresh) {
-                    pivots_.put(piv.idx, c);
                     break;
                 }
                 const u64 j = pivots_.find(piv.idx);
@@ -1029,110 +1145,54 @@ class Cohomology {
                     if (keep_[j] == KEEP_COBOUNDARY) {
                         for (u64 t = red_off_[j]; t < red_off_[j + 1]; ++t) col_push(red_[t]);
                     } else {
-                        added.push_back(cols_[j]);
-                        push_coboundary(cols_[j], nv);
+                        parity_.toggle(cur_[j]);
+                        push_coboundary(cur_[j], k);
                         for (u64 t = red_off_[j]; t < red_off_[j + 1]; ++t) {
-                            added.push_back(red_[t]);
-                            push_coboundary(red_[t], nv);
+                            parity_.toggle(red_[t]);
+                            push_coboundary(red_[t], k);
                         }
                     }
                     piv = col_pivot();
                     continue;
                 }
-                const S fa = apparent_facet_of(piv, nv + 1);
+                const S fa = apparent_facet_of(piv, k + 1);
                 if (fa.idx != kNone<I>) {
-                    added.push_back(fa);
-                    push_coboundary(fa, nv);
+                    parity_.toggle(fa);
+                    push_coboundary(fa, k);
                     piv = col_pivot();
                     continue;
                 }
                 if (piv.d > col.d) out_.add(dim, col.d, piv.d, u128(col.idx), u128(piv.idx));
                 pivots_.put(piv.idx, c);
                 if constexpr (store_cob) {
-                    // the reduced coboundary: the pivot and what follows it
                     keep_[c] = KEEP_COBOUNDARY;
                     col_drain([&](const S& e) { red_.push_back(e); });
                 } else {
-                    // the reduction column modulo 2
                     keep_[c] = KEEP_REDUCTION;
-                    std::sort(added.begin(), added.end(), [](const S& a, const S& b) { return a.idx < b.idx; });
-                    for (size_t x = 0; x < added.size();) {
-                        if (x + 1 < added.size() && added[x].idx == added[x + 1].idx) x += 2;
-                        else red_.push_back(added[x++]);
-                    }
+                    keep_reduction();
                 }
                 break;
             }
             red_off_.push_back(red_.size());
         }
-        tel_.columns.push_back(cols_.size());
+        tel_.columns.push_back(nc);
         tel_.emergent.push_back(emergent);
-    }
-
-    // columns of dimension dim + 1 from the simplices of dimension dim
-    void assemble(u32 dim) {
-        const u32 nv = dim + 1;
-        const bool keep = dim + 1 < maxdim_;
-        const u64 ns = simplices_.size();
-        // Contiguous ranges of the simplices, one per thread; the outputs are
-        // concatenated in range order, so columns and next simplices come out
-        // in the serial order.
-        const u32 nt = u32(std::max<u64>(1, std::min<u64>(threads_, ns / 4096)));
-        std::vector<std::vector<S>> cols(nt), nexts(nt);
-        std::vector<u64> skipped(nt, 0);
-        auto range = [&](u32 t) {
-            const u64 lo = ns * t / nt, hi = ns * (t + 1) / nt;
-            std::vector<S>& oc = cols[t];
-            std::vector<S>& on = nexts[t];
-            u32 vd[KM], cv[KM + 1];
-            for (u64 x = lo; x < hi; ++x) {
-                const S& s = simplices_[x];
-                decode(s.idx, nv, vd);
-                cofacet_simplices(vd, nv, s.d, false, cx_.threshold, [&](const S& cf, u32 w, u32 t2) {
-                    if (keep) on.push_back(cf);
-                    if (pivots_.has(cf.idx)) return true;          // cleared
-                    for (u32 i = 0; i < t2; ++i) cv[i] = vd[i];
-                    cv[t2] = w;
-                    for (u32 i = t2; i < nv; ++i) cv[i + 1] = vd[i];
-                    if (apparent_cofacet_of(cf, cv, nv + 1).idx != kNone<I> ||
-                        apparent_facet_of(cf, cv, nv + 1).idx != kNone<I>) {
-                        ++skipped[t];
-                        return true;
-                    }
-                    oc.push_back(cf);
-                    return true;
-                });
-            }
-        };
-        if (nt == 1) {
-            range(0);
-        } else {
-            std::vector<std::thread> pool;
-            for (u32 t = 1; t < nt; ++t) pool.emplace_back(range, t);
-            range(0);
-            for (auto& th : pool) th.join();
-        }
-        cols_.clear();
-        for (auto& v : cols) cols_.insert(cols_.end(), v.begin(), v.end());
-        std::vector<S> next;
-        for (auto& v : nexts) next.insert(next.end(), v.begin(), v.end());
-        simplices_.swap(next);
-        std::sort(cols_.begin(), cols_.end(), [](const S& a, const S& b) { return before(b, a); });
-        u64 sk = 0;
-        for (u64 v : skipped) sk += v;
-        tel_.apparent.push_back(sk);
     }
 
     const Cx& cx_;
     const u32 n_, maxdim_;
-    u32 threads_ = 1;
     const Binomials<I> B_;
     Diagram& out_;
     Telemetry& tel_;
-    std::vector<S> cols_, simplices_, red_;
+    std::vector<S> cur_, red_;        // columns of the current dimension; stored reductions
+    std::vector<I> cof_;              // per column: emergent candidate
+    std::vector<u8> state_;           // per column: emergent state
     std::vector<u64> red_off_;
     PivotTable<I> pivots_;
-    Heap heap_;
+    ParitySet<S> parity_;
+    const u64* edges_ = nullptr;      // dimension-0 edges (the first seed)
+    const f32* edge_d_ = nullptr;
+    u64 edge_count_ = 0;
 };
 
 }  // namespace tops::cob

@@ -1,7 +1,7 @@
 // SPDX-License-Identifier: MIT
 // persistence.cpp — tops::persistence and the front-end functions of the
 // reference's API, with every data-parallel stage on the GPU (include/ttgpu.h)
-// and the residual reduction on the host (reducer.hpp).
+// and the residual reduction on the host (cohomology.hpp, zerodiff.hpp).
 #include <sys/resource.h>
 #include <unistd.h>
 
@@ -101,27 +101,77 @@ void reduce_boundary(u32 n, const f32* mat, u64 m, const u32* verts, const f32* 
     st.seconds_reduce = since(t0);
 }
 
-// Runs the reducer from dimension 1 after dim0 on the given edges.
-// threads: host threads for the assembly of each next dimension's columns.
+// The GPU's column stages (include/ttgpu.h tt_complex_*) as the cohomology
+// engine's ColumnSource: the complex is copied to the device once, the
+// simplices between dimensions stay there.
+class GpuColumns final : public cob::ColumnSource {
+  public:
+    GpuColumns(const cob::DenseView& cx, u32 maxdim, int words) {
+        check(tt_complex_dense(cx.mat, cx.n, cx.full ? TT_LAYOUT_FULL : TT_LAYOUT_LOWER, 0, cx.threshold, maxdim,
+                               words, &h_));
+    }
+    GpuColumns(const cob::SparseView& cx, u32 maxdim, int words) {
+        check(tt_complex_sparse(cx.n, cx.row_ptr, cx.col, cx.w, cx.threshold, maxdim, words, &h_));
+    }
+    ~GpuColumns() override { tt_complex_destroy(h_); }
+    GpuColumns(const GpuColumns&) = delete;
+    GpuColumns& operator=(const GpuColumns&) = delete;
+
+    void emergent(u32 nv, const u64* idx, const f32* d, u64 count, u64* cof, u8* state) override {
+        check(tt_complex_emergent(h_, nv, idx, d, count, cof, state));
+    }
+    void seed(u32 nv, const u64* idx, const f32* d, u64 count) override {
+        check(tt_complex_seed(h_, nv, idx, d, count));
+    }
+    void assemble(const u64* pivots, u64 npivots, bool keep_next, std::vector<u64>& idx, std::vector<f32>& d,
+                  std::vector<u64>& cof, std::vector<u8>& state, u64& skipped) override {
+        u64 nc = 0;
+        u64 *pi = nullptr, *pc = nullptr;
+        f32* pd = nullptr;
+        u8* ps = nullptr;
+        const tt_status rc = tt_complex_assemble(h_, pivots, npivots, keep_next ? 1 : 0, &nc, &pi, &pd, &pc, &ps,
+                                                 &skipped);
+        std::unique_ptr<u64, void (*)(void*)> a(pi, tt_free), c(pc, tt_free);
+        std::unique_ptr<f32, void (*)(void*)> b(pd, tt_free);
+        std::unique_ptr<u8, void (*)(void*)> e(ps, tt_free);
+        check(rc);
+        const u64 w = words_;
+        idx.assign(pi, pi + nc * w);
+        cof.assign(pc, pc + nc * w);
+        d.assign(pd, pd + nc);
+        state.assign(ps, ps + nc);
+    }
+    void set_words(u64 w) { words_ = w; }
+
+  private:
+    tt_complex* h_ = nullptr;
+    u64 words_ = 1;
+};
+
+// Runs the cohomology engine on the given dimension-0 edges: dim0 on the
+// host, every higher dimension's columns from the GPU.
 template <class Cx>
 void reduce_all(const Cx& cx, u32 maxdim, const u64* idx, const f32* d, const u8* apparent, u64 m,
-                Barcode& out, Stats& st, u32 threads) {
+                Barcode& out, Stats& st, bool force_wide = false) {
     const auto t0 = std::chrono::steady_clock::now();
     if (maxdim + 2 > 32) throw std::invalid_argument("rips: maxdim must be <= 30");
     out.ensure(maxdim);
     if (cx.n == 0) return;
     cob::Telemetry tel;
-    if (cob::index_fits<u64>(cx.n, maxdim)) {
-        cob::Cohomology<u64, Cx> r(cx, maxdim, out, tel);
-        r.set_threads(threads);
+    auto run = [&]<class I>() {
+        cob::Cohomology<I, Cx> r(cx, maxdim, out, tel);
         r.dim0(idx, d, apparent, m);
-        r.run_from_dim1();
+        if (maxdim >= 1) {
+            GpuColumns src(cx, maxdim, int(sizeof(I) / 8));
+            src.set_words(sizeof(I) / 8);
+            r.run(src);
+        }
+    };
+    if (!force_wide && cob::index_fits<u64>(cx.n, maxdim)) {
+        run.template operator()<u64>();
     } else if (cob::index_fits<cob::u128>(cx.n, maxdim)) {
         tel.wide_index = true;
-        cob::Cohomology<cob::u128, Cx> r(cx, maxdim, out, tel);
-        r.set_threads(threads);
-        r.dim0(idx, d, apparent, m);
-        r.run_from_dim1();
+        run.template operator()<cob::u128>();
     } else {
         throw std::overflow_error("rips: C(n, maxdim + 2) exceeds 128-bit simplex indices; lower maxdim");
     }
@@ -293,7 +343,7 @@ DenseComplex make_dense(const PointCloud& c, u64 budget) {
 }  // namespace rips
 
 // ---- persistence ----------------------------------------------------------------
-Result persistence(const CsrGraph& g, u32 maxdim, f32 threshold, u32 threads, Mode mode) {
+Result persistence(const CsrGraph& g, u32 maxdim, f32 threshold, u32 threads, Mode mode, bool force_wide) {
     const auto t0 = std::chrono::steady_clock::now();
     validate_options(threshold, 1);
     if (!g.valid())
@@ -309,7 +359,7 @@ Result persistence(const CsrGraph& g, u32 maxdim, f32 threshold, u32 threads, Mo
     cx.col = g.col_idx.data();
     cx.w = g.weights.data();
     const SparseEdgeList e(g, threshold, maxdim >= 1);
-    reduce_all(cx, maxdim, e.idx, e.d, e.apparent, e.m, r.barcode, r.stats, num_threads());
+    reduce_all(cx, maxdim, e.idx, e.d, e.apparent, e.m, r.barcode, r.stats, force_wide);
     r.barcode.ensure(maxdim);
     r.stats.method = "sparse";
     r.stats.threshold = threshold;
@@ -388,7 +438,7 @@ Result persistence(const f32* points, u32 n, u32 dim, const Options& o) {
         cx.full = layout == TT_LAYOUT_FULL;
         cx.mat = mat.data();
         reduce_all(cx, o.maxdim, idx.data(), d.data(), ap.empty() ? nullptr : ap.data(), m, r.barcode, st,
-                   num_threads());
+                   o.force_wide);
         st.method = "dense";
         st.threshold = thr;
     } else {
@@ -407,7 +457,7 @@ Result persistence(const f32* points, u32 n, u32 dim, const Options& o) {
                                      &col, &w));
         g = graph_from_abi(n, rp, nnz, col, w);
         st.seconds_graph = since(tg);
-        const Result rg = persistence(g, o.maxdim, o.threshold, 0, o.mode);
+        const Result rg = persistence(g, o.maxdim, o.threshold, 0, o.mode, o.force_wide);
         r.barcode = rg.barcode;
         const double sg = st.seconds_graph;
         st = rg.stats;
@@ -558,7 +608,7 @@ std::vector<Result> BatchRunner::run(const f32* pts, bool use_graph, u32 threads
                     cx.threshold = thr;
                     cx.full = true;
                     cx.mat = mat;
-                    reduce_all(cx, maxdim_, idx, d, nullptr, m, r.barcode, r.stats, 1);   // one cloud per thread
+                    reduce_all(cx, maxdim_, idx, d, nullptr, m, r.barcode, r.stats);
                     r.stats.engine = "coboundary";
                 }
                 r.barcode.ensure(maxdim_);

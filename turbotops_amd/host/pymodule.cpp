@@ -52,8 +52,11 @@ py::dict stats_dict(const Stats& s) {
 }
 
 Options make_options(u32 maxdim, float threshold, const std::string& metric, const std::string& graph, u32 k,
-                     bool approximate, u64 recall_samples, u32 threads, u64 memory_budget) {
+                     bool approximate, u64 recall_samples, u32 threads, u64 memory_budget,
+                     const std::string& mode = "auto", bool force_wide = false) {
     Options o;
+    o.mode = parse_mode(mode);
+    o.force_wide = force_wide;
     o.maxdim = maxdim;
     o.threshold = threshold;
     o.metric = metric;
@@ -79,18 +82,20 @@ Result run_points(F32Array x, const Options& o) {
     return r;
 }
 
+// mode and force_wide are not in the reference's Python API: the
+// engine choice (--mode of its CLI) and rips::compute's force_wide, for tests.
 py::tuple persistence_points(F32Array x, u32 maxdim, float threshold, const std::string& metric,
                              const std::string& graph, u32 k, bool approximate, u64 recall_samples, u32 threads,
-                             u64 memory_budget) {
+                             u64 memory_budget, const std::string& mode, bool force_wide) {
     const Result r = run_points(x, make_options(maxdim, threshold, metric, graph, k, approximate, recall_samples,
-                                                threads, memory_budget));
+                                                threads, memory_budget, mode, force_wide));
     return py::make_tuple(to_python(r.barcode), stats_dict(r.stats));
 }
 
 // persistence_csr (turbotops_ext.cpp:76-106): both directions of every
 // entry, the smallest weight of repeated entries, then persistence(CsrGraph).
 py::tuple persistence_csr(I64Array indptr, I64Array indices, F32Array data, u32 n, u32 maxdim, float threshold,
-                          u32 threads) {
+                          u32 threads, bool force_wide) {
     if (size_t(indptr.shape(0)) != size_t(n) + 1 || indices.shape(0) != data.shape(0))
         throw py::value_error("inconsistent CSR arrays");
     std::vector<std::vector<std::pair<u32, f32>>> rows(n);
@@ -116,7 +121,7 @@ py::tuple persistence_csr(I64Array indptr, I64Array indices, F32Array data, u32 
     Result r;
     {
         py::gil_scoped_release release;
-        r = persistence(g, maxdim, threshold, threads);
+        r = persistence(g, maxdim, threshold, threads, Mode::Auto, force_wide);
     }
     return py::make_tuple(to_python(r.barcode), stats_dict(r.stats));
 }
@@ -211,9 +216,9 @@ PYBIND11_MODULE(_turbotops, m) {
     m.attr("__version__") = "0.1.0";
     m.def("persistence_points", &persistence_points, py::arg("x"), py::arg("maxdim"), py::arg("threshold"),
           py::arg("metric"), py::arg("graph"), py::arg("k"), py::arg("approximate"), py::arg("recall_samples"),
-          py::arg("threads"), py::arg("memory_budget"));
+          py::arg("threads"), py::arg("memory_budget"), py::arg("mode") = "auto", py::arg("force_wide") = false);
     m.def("persistence_csr", &persistence_csr, py::arg("indptr"), py::arg("indices"), py::arg("data"),
-          py::arg("n"), py::arg("maxdim"), py::arg("threshold"), py::arg("threads"));
+          py::arg("n"), py::arg("maxdim"), py::arg("threshold"), py::arg("threads"), py::arg("force_wide") = false);
     m.def("persistence_pairs", &persistence_pairs, py::arg("x"), py::arg("maxdim"), py::arg("threshold"),
           py::arg("metric"), py::arg("graph"), py::arg("k"), py::arg("threads"), py::arg("memory_budget"));
     py::class_<BatchRunner>(m, "BatchRunner")

@@ -9,7 +9,9 @@
 //   tops::radius_graph / knn_lists_exact / knn_graph (graph.hpp:170-304)
 //   tops::mst::h0                                    (mst.hpp:25-99)
 //   tops::rips::make_dense / DenseComplex            (rips.hpp:255-416)
-// Persistence pairs come from the host cohomology reducer (reducer.hpp).
+// Persistence pairs come from the cohomology engine (cohomology.hpp: the
+// column reduction on the host, its per-simplex work on the GPU) or, for
+// dense H1, the boundary engine (zerodiff.hpp).
 #pragma once
 
 #include <cmath>
@@ -20,7 +22,7 @@
 #include <utility>
 #include <vector>
 
-#include "reducer.hpp"
+#include "cohomology.hpp"
 #include "ttgpu.h"
 
 namespace tops {
@@ -133,6 +135,10 @@ struct Options {
     u64 recall_samples = 0;
     u64 memory_budget = 0;   // bytes; 0 = half of physical memory
     u32 threads = 0;         // 0 = hardware concurrency
+    // 128-bit simplex indices in the cohomology engine even where 64 bits
+    // suffice, as the force_wide argument of rips::compute
+    // (rips.hpp:1038-1061); for tests
+    bool force_wide = false;
 };
 
 struct Stats {
@@ -154,7 +160,7 @@ struct Result {
 
 Result persistence(const f32* points, u32 n, u32 dim, const Options& o);
 Result persistence(const CsrGraph& g, u32 maxdim, f32 threshold = INFINITY, u32 threads = 0,
-                   Mode mode = Mode::Auto);
+                   Mode mode = Mode::Auto, bool force_wide = false);
 
 // Critical edges of a diagram: per dimension and bar, the longest edge
 // (a, b) of its birth simplex and of its death simplex, read with dist(a, b)

@@ -58,7 +58,7 @@
 #include <immintrin.h>
 #endif
 
-#include "reducer.hpp"
+#include "cohomology.hpp"
 
 namespace tops::zd {
 
