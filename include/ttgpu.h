@@ -92,7 +92,9 @@ tt_status tt_radius_graph(const float* pts, uint32_t n, uint32_t dim, int kind, 
 
 /* knn_lists_exact(PointCloud, k) (graph.hpp:211-225): per point the k
  * nearest j != i by (distance, id), ascending; k is clamped to n - 1 and
- * written to *k_out. ids: n * k_out uint32 (caller's). k <= 64. */
+ * written to *k_out. ids: n * k_out uint32 (caller's). Any k: up to 64 the
+ * lists are kept per lane in registers, beyond by a per-query selection;
+ * n * k_out < 2^32 (TT_ELENGTH beyond). */
 tt_status tt_knn_exact(const float* pts, uint32_t n, uint32_t dim, int kind, float p, uint32_t k,
                        uint32_t* k_out, uint32_t* ids);
 

@@ -37,12 +37,61 @@ def test_reference_selftest_case(gpu, restatement, metric):
     ids = knn_lists_exact(x, metric, 7)
     if metric in ("l1", "l2", "linf"):
         assert np.array_equal(ids, restatement.knn(x, metric, 7))
+    else:
+        _lp_lists_agree(restatement, x, metric, ids)
     g = radius_graph(x, metric, 1.2)
     rp, col, w = restatement.radius_graph(x, metric, 1.2)
-    assert g.valid() and np.array_equal(g.row_ptr, rp) and np.array_equal(g.col_idx, col)
-    assert _same_w(g.weights, w, metric)
+    if metric in ("l1", "l2", "linf"):
+        assert g.valid() and np.array_equal(g.row_ptr, rp) and np.array_equal(g.col_idx, col)
+        assert _same_w(g.weights, w, metric)
+    else:
+        assert g.valid()
+        _lp_graphs_agree(g, rp, col, w, 1.2)
     mat = restatement.make_dense(x, metric)
     assert g.m == int((mat[np.triu_indices(300, 1)] <= np.float32(1.2)).sum())
+
+
+def _lp_lists_agree(restatement, x, metric, ids):
+    """Lp distances may differ from glibc powf by an ulp (2 at most): each
+    list must hold the oracle's neighbours wherever its k-th and (k+1)-th
+    distances are more than 4 ulp apart, and then in the same order wherever
+    consecutive distances are."""
+    n, k = ids.shape
+    mat = restatement.make_dense(x, metric)
+    want = restatement.knn(x, metric, k)
+    checked = 0
+    for i in range(n):
+        row = np.delete(mat[i], i)
+        srt = np.sort(row)
+        if k < n - 1 and ulp_diff(srt[k - 1], srt[k]) <= 4:
+            continue
+        assert set(ids[i].tolist()) == set(want[i].tolist()), i
+        dw = mat[i, want[i]]
+        sep = np.r_[True, ulp_diff(dw[1:], dw[:-1]) > 4] & np.r_[ulp_diff(dw[1:], dw[:-1]) > 4, True]
+        assert np.array_equal(ids[i][sep], want[i][sep]), i
+        checked += 1
+    assert checked > n // 2
+
+
+def _lp_graphs_agree(g, rp, col, w, tau):
+    """Every edge whose oracle weight is more than 2 ulp from tau is in both
+    graphs with weights within 2 ulp; edges within 2 ulp of tau may differ."""
+    def edges(rp_, col_, w_):
+        out = {}
+        for v in range(len(rp_) - 1):
+            for t in range(int(rp_[v]), int(rp_[v + 1])):
+                out[(v, int(col_[t]))] = np.float32(w_[t])
+        return out
+    got = edges(g.row_ptr, g.col_idx, g.weights)
+    want = edges(rp, col, w)
+    t32 = np.float32(tau)
+    for e, wv in want.items():
+        if ulp_diff(wv, t32) > 2:
+            assert e in got and ulp_diff(got[e], wv) <= 2, e
+    for e, wv in got.items():
+        if ulp_diff(wv, t32) > 2:
+            assert e in want, e
+    assert len(want) > 0
 
 
 @pytest.mark.parametrize("metric", ["l1", "l2", "linf"])
@@ -79,6 +128,45 @@ def test_knn_candidate_chunks(gpu, restatement, metric, n, d, kind, k):
     assert ids.shape == want.shape and np.array_equal(ids, want)
 
 
+@pytest.mark.parametrize("metric", ["l1", "l2", "linf"])
+@pytest.mark.parametrize("n,d,kind,k", [(300, 3, "grid", 65), (700, 4, "gauss", 100), (1500, 8, "unif", 500),
+                                        (120, 2, "grid", 119), (2000, 3, "grid", 300)])
+def test_knn_large_k(gpu, restatement, metric, n, d, kind, k):
+    """k > 64 (knn_lists_exact clamps only to n - 1, graph.hpp:214): the
+    selection path, ties at the k-th distance included (integer grids), the
+    whole row at k = n - 1; ids bit-exact, and the kNN graph."""
+    from turbotops_amd import knn_graph, knn_graph_exact, knn_lists_exact
+    x = _cloud(n, d, n + 5 * d, kind)
+    print(f"case knn large k metric={metric} n={n} d={d} k={k}", flush=True)
+    ids = knn_lists_exact(x, metric, k)
+    want = restatement.knn(x, metric, k)
+    assert ids.shape == want.shape == (n, min(k, n - 1)) and np.array_equal(ids, want)
+    g = knn_graph(x, metric, ids)
+    rp, col, w = restatement.knn_graph(x, metric, want)
+    assert np.array_equal(g.row_ptr, rp) and np.array_equal(g.col_idx, col)
+    assert np.array_equal(bits(g.weights), bits(w))
+    g2 = knn_graph_exact(x, metric, k)
+    assert np.array_equal(g2.row_ptr, rp) and np.array_equal(g2.col_idx, col)
+
+
+def test_knn_large_k_lp(gpu, restatement):
+    from turbotops_amd import knn_lists_exact
+    x = _cloud(400, 5, 77)
+    _lp_lists_agree(restatement, x, "lp:1.5", knn_lists_exact(x, "lp:1.5", 90))
+
+
+@pytest.mark.parametrize("metric", ["l2", "l1"])
+def test_persistence_knn_k100_vs_reference(gpu, reference, metric):
+    """persistence(x, graph="knn", k=100), which the reference accepts for any
+    k (graph.hpp:211-225), bar for bar against the compiled reference."""
+    from turbotops_amd import persistence
+    x = _cloud(300, 3, 5)
+    dg, _ = persistence(x, maxdim=1, metric=metric, graph="knn", k=100)
+    (dims, b, e), _ = reference.persistence(x, maxdim=1, metric=metric, graph="knn", k=100)
+    got = sorted((kk, float(bb), float(ee)) for kk, a in enumerate(dg) for bb, ee in np.asarray(a).tolist())
+    assert got == sorted(zip(dims.tolist(), b.tolist(), e.tolist()))
+
+
 @pytest.mark.parametrize("metric", ["lp:1.5", "lp:3"])
 def test_knn_lp(gpu, restatement, metric):
     """Lp distances may differ from glibc powf by an ulp; lists must agree
@@ -107,8 +195,10 @@ def test_radius_graph(gpu, restatement, metric, n, d, kind, tau):
     if metric in ("l1", "l2", "linf"):
         assert np.array_equal(g.row_ptr, rp) and np.array_equal(g.col_idx, col)
         assert np.array_equal(bits(g.weights), bits(w))
-    else:   # an ulp at the threshold may move an edge; away from it the sets agree
+    else:   # an ulp at the threshold may move an edge; away from it the graphs agree
         assert abs(int(g.row_ptr[-1]) - int(rp[-1])) <= 2
+        if rp[-1]:
+            _lp_graphs_agree(g, rp, col, w, tau)
     assert g.valid()
 
 

@@ -17,6 +17,12 @@
 //                 a sorted insertion network. Candidates arrive in ascending id,
 //                 so "strictly closer than the current k-th" is the reference's
 //                 (distance, id) lexicographic rule (graph.hpp:112, :145).
+//   kNN, k > 64   one workgroup per query: a radix select over the 32-bit
+//                 distance bits (four 8-bit passes, recomputing the distances)
+//                 finds the k-th distance D*, a second select over the ids
+//                 breaks the ties at D* (only when they do not all fit); the k
+//                 chosen (distance, id) pairs are then ordered by two stable
+//                 radix sorts over every query at once.
 //   kNN graph     both directions of every list entry as (row, col) keys, radix
 //                 sort, adjacent-duplicate removal, weights recomputed as
 //                 c(row, col) (graph.hpp:295-302).
@@ -310,6 +316,147 @@ __global__ __launch_bounds__(GB) void knn_merge(const float* __restrict__ pd, co
     }
 }
 
+// ---- kNN for k > 64: selection instead of a register list ---------------------
+constexpr u32 KS = 256;   // threads per query
+
+// 8-bit digit of the radix select at `shift` where the running rank `want`
+// (1-based) falls, from a 256-bin histogram in LDS; updates want to the rank
+// within that digit and returns the digit. Whole block.
+__device__ __forceinline__ u32 select_digit(u32* hist, u32& want, u32* s_digit, u32* s_want) {
+    __syncthreads();
+    if (threadIdx.x < 64) {
+        const u32 lane = threadIdx.x;
+        u32 c[4], sum = 0;
+#pragma unroll
+        for (u32 t = 0; t < 4; ++t) {
+            c[t] = hist[lane * 4 + t];
+            sum += c[t];
+        }
+        u32 inc = sum;   // inclusive scan of the lane sums
+#pragma unroll
+        for (u32 off = 1; off < 64; off <<= 1) {
+            const u32 o = __shfl_up(inc, off);
+            if (lane >= off) inc += o;
+        }
+        u32 run = inc - sum;
+#pragma unroll
+        for (u32 t = 0; t < 4; ++t) {
+            if (run < want && run + c[t] >= want) {
+                *s_digit = lane * 4 + t;
+                *s_want = want - run;
+            }
+            run += c[t];
+        }
+    }
+    __syncthreads();
+    want = *s_want;
+    return *s_digit;
+}
+
+template <int V, int DIM>
+__device__ __forceinline__ float knn_pair(const float* __restrict__ x, const float* q, const float* qp, u32 j,
+                                          u32 dim, MetricArgs ma) {
+    if constexpr (DIM > 0) {
+        float c[DIM];
+#pragma unroll
+        for (int t = 0; t < DIM; ++t) c[t] = x[u64(j) * DIM + t];
+        return pdist<V, DIM>(q, c, DIM, ma);
+    } else {
+        return pdist<V, 0>(qp, x + u64(j) * dim, dim, ma);
+    }
+}
+
+// Per query (blockIdx.x): the k nearest j != q by (distance, id), written in
+// any order as (distance bits << jb | j) keys with the query as payload.
+template <int V, int DIM>
+__global__ __launch_bounds__(KS) void knn_select(const float* __restrict__ x, u32 n, u32 dim_rt, MetricArgs ma,
+                                                 u32 k, int jb, u64* __restrict__ keys, u32* __restrict__ owner) {
+    constexpr u32 DS = DIM > 0 ? DIM : 1;
+    __shared__ u32 hist[256];
+    __shared__ u32 s_digit, s_want, s_count;
+    const u32 dim = DIM > 0 ? u32(DIM) : dim_rt;
+    const u32 qi = blockIdx.x;
+    const float* qp = x + u64(qi) * dim;
+    float q[DS];
+    if constexpr (DIM > 0) {
+#pragma unroll
+        for (int t = 0; t < DIM; ++t) q[t] = qp[t];
+    }
+    // the k-th distance: four passes over the distance bits, most significant first
+    u32 want = k, dpre = 0, dmask = 0;
+    for (int shift = 24; shift >= 0; shift -= 8) {
+        for (u32 t = threadIdx.x; t < 256; t += KS) hist[t] = 0;
+        __syncthreads();
+        for (u32 j = threadIdx.x; j < n; j += KS) {
+            if (j == qi) continue;
+            const u32 b = __float_as_uint(knn_pair<V, DIM>(x, q, qp, j, dim, ma));
+            if ((b & dmask) == dpre) atomicAdd(&hist[(b >> shift) & 255u], 1u);
+        }
+        const u32 before = want;
+        const u32 g = select_digit(hist, want, &s_digit, &s_want);
+        (void)before;
+        dpre |= g << shift;
+        dmask |= 255u << shift;
+    }
+    // `want` of the candidates at distance D* = dpre are taken, the smallest ids
+    // first (the reference's (distance, id) order); hist[g] of the last pass
+    // holds how many there are
+    const u32 ties = hist[dpre & 255u];
+    u32 jmax = 0xffffffffu;
+    if (want < ties) {
+        u32 jpre = 0, jmask = 0;
+        for (int shift = 24; shift >= 0; shift -= 8) {
+            __syncthreads();
+            for (u32 t = threadIdx.x; t < 256; t += KS) hist[t] = 0;
+            __syncthreads();
+            for (u32 j = threadIdx.x; j < n; j += KS) {
+                if (j == qi || (j & jmask) != jpre) continue;
+                if (__float_as_uint(knn_pair<V, DIM>(x, q, qp, j, dim, ma)) == dpre)
+                    atomicAdd(&hist[(j >> shift) & 255u], 1u);
+            }
+            const u32 g = select_digit(hist, want, &s_digit, &s_want);
+            jpre |= g << shift;
+            jmask |= 255u << shift;
+        }
+        jmax = jpre;
+    }
+    if (threadIdx.x == 0) s_count = 0;
+    __syncthreads();
+    const u64 base = u64(qi) * k;
+    for (u32 j = threadIdx.x; j < n; j += KS) {
+        if (j == qi) continue;
+        const u32 b = __float_as_uint(knn_pair<V, DIM>(x, q, qp, j, dim, ma));
+        if (b < dpre || (b == dpre && j <= jmax)) {
+            const u32 slot = atomicAdd(&s_count, 1u);
+            if (slot < k) {
+                keys[base + slot] = (u64(b) << jb) | j;
+                owner[base + slot] = qi;
+            }
+        }
+    }
+}
+
+template <int V, int DIM>
+struct KnnSelectLaunch {
+    static void run(u32 n, hipStream_t s, const float* x, u32 dim, MetricArgs ma, u32 k, int jb, u64* keys,
+                    u32* owner) {
+        knn_select<V, DIM><<<n, KS, 0, s>>>(x, n, dim, ma, k, jb, keys, owner);
+    }
+};
+
+// (query << 32 | position) keys: the second sort groups the (distance, id)
+// order by query
+__global__ void knn_owner_keys(const u32* __restrict__ owner, u64 total, u64* __restrict__ keys) {
+    for (u64 t = blockIdx.x * u64(blockDim.x) + threadIdx.x; t < total; t += u64(gridDim.x) * blockDim.x)
+        keys[t] = (u64(owner[t]) << 32) | u32(t);
+}
+
+__global__ void knn_final_ids(const u64* __restrict__ by_query, const u64* __restrict__ by_dist, u64 total,
+                              u64 jmask, u32* __restrict__ ids) {
+    for (u64 t = blockIdx.x * u64(blockDim.x) + threadIdx.x; t < total; t += u64(gridDim.x) * blockDim.x)
+        ids[t] = u32(by_dist[u32(by_query[t])] & jmask);
+}
+
 struct KnnChunks {
     u32 chunks = 1;          // candidate chunks (compile-time dimensions only)
     float* pd = nullptr;     // chunks x n x k partial lists
@@ -473,11 +620,38 @@ DevCsr radius_graph_dev(const float* d_x, u32 n, u32 dim, int kind, float p, flo
 }
 
 // ---- kNN --------------------------------------------------------------------
+// k > 64: per-query selection, then the lists ordered by (query, distance, id)
+static void knn_large_k(const float* d_x, u32 n, u32 dim, int kind, float p, u32 kk, u32* d_ids, Workspace& ws,
+                        hipStream_t s) {
+    const u64 total = u64(n) * kk;
+    if (total >= (u64(1) << 32)) fail(TT_ELENGTH, "knn: n * k must be below 2^32");
+    const int jb = std::max(1, bit_length(n - 1));
+    u64* keys = ws.get<u64>(total);
+    u64* kalt = ws.get<u64>(total);
+    u32* owner = ws.get<u32>(total);
+    u32* oalt = ws.get<u32>(total);
+    const MetricVariant v = variant_of(kind, p);
+    const MetricArgs ma = metric_args(kind, p);
+    dispatch_metric_dim<KnnSelectLaunch>(v, dim, n, s, d_x, dim, ma, kk, jb, keys, owner);
+    TTG_LAUNCH_CHECK();
+    // (distance, id) ascending, the query carried along; then stably by query
+    radix_sort_u64(keys, owner, total, 0, 31 + jb, kalt, oalt, ws, s);
+    u64* qkeys = ws.get<u64>(total);
+    knn_owner_keys<<<blocks_for(total), GB, 0, s>>>(owner, total, qkeys);
+    TTG_LAUNCH_CHECK();
+    radix_sort_u64(qkeys, nullptr, total, 32, 32 + std::max(1, bit_length(n - 1)), kalt, nullptr, ws, s);
+    knn_final_ids<<<blocks_for(total), GB, 0, s>>>(qkeys, keys, total, (u64(1) << jb) - 1, d_ids);
+    TTG_LAUNCH_CHECK();
+}
+
 u32 knn_dev(const float* d_x, u32 n, u32 dim, int kind, float p, u32 k, u32* d_ids, Workspace& ws,
             hipStream_t s) {
     const u32 kk = n ? std::min<u32>(k, n - 1) : 0;
     if (!kk) return 0;
-    if (kk > 64) fail(TT_EINVAL, "knn: k > 64 is not supported by the device kernel");
+    if (kk > 64) {
+        knn_large_k(d_x, n, dim, kind, p, kk, d_ids, ws, s);
+        return kk;
+    }
     const u32 kmax = kk <= 8 ? 8 : kk <= 16 ? 16 : kk <= 32 ? 32 : 64;
     const MetricVariant v = variant_of(kind, p);
     const MetricArgs ma = metric_args(kind, p);
