@@ -12,6 +12,7 @@
 #include <string>
 
 #include "cohomology.hpp"
+#include "powf.hpp"
 #include "ttoracle.h"
 #include "zerodiff.hpp"
 
@@ -171,4 +172,13 @@ int hh_sparse(uint32_t n, const uint64_t* rp, const uint32_t* col, const float* 
     return run(cx, maxdim, idx, d, ap, m, force_wide, nbars, dims, b, e);
 }
 
+
+// The Lp kernels' power (turbotops_amd/csrc/powf.hpp), host build of the same
+// source: out[t] = x[t]^p.
+void hh_pow_f32(const float* x, uint64_t count, float p, float* out) {
+    for (uint64_t t = 0; t < count; ++t) out[t] = ttg::pow_f32(x[t], p);
+}
+void hh_pow_half_f32(const float* x, uint64_t count, uint32_t n, float* out) {
+    for (uint64_t t = 0; t < count; ++t) out[t] = ttg::pow_half_f32(x[t], n);
+}
 }

@@ -11,6 +11,7 @@
 #include <stdexcept>
 #include <string>
 
+#include "powf.hpp"
 #include "ttgpu.h"
 
 namespace ttg {
@@ -125,11 +126,7 @@ __device__ __forceinline__ float ipow_ref(float x, u32 p) {
     return r;
 }
 
-// powf for x in [0, 1] (terms) and s in [1, dim] (root): the double-precision
-// power rounded once to float. Within 1 ulp of glibc's powf, which is not
-// correctly rounded itself.
-__device__ __forceinline__ float pow_f32(float x, float p) {
-    return static_cast<float>(pow(static_cast<double>(x), static_cast<double>(p)));
-}
+// powf for the Lp terms and root: pow_f32 in powf.hpp (table-driven, double
+// precision inside, within one ulp of glibc's powf).
 
 }  // namespace ttg

@@ -124,6 +124,11 @@ MetricArgs metric_args(int kind, float p) {
     ma.p = p;
     ma.inv_p = 1.0f / p;
     ma.ip = (kind == TT_METRIC_LP && p == std::floor(p) && p <= 8.0f) ? u32(p) : 0u;
+    ma.half = 0;
+    if (kind == TT_METRIC_LP && !ma.ip && p - std::floor(p) == 0.5f && p <= 16.5f) {
+        ma.half = 1;
+        ma.ip = u32(std::floor(p));
+    }
     return ma;
 }
 

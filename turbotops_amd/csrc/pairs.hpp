@@ -18,8 +18,16 @@ constexpr u32 KC = 16;        // coordinates per register chunk (runtime dim)
 struct MetricArgs {
     float p;       // Minkowski exponent
     float inv_p;   // 1.0f / p in float, as the reference computes it (metric.hpp:206)
-    u32 ip;        // integral exponent for square-and-multiply
+    u32 ip;        // integral exponent for square-and-multiply; n for p = n + 1/2
+    u32 half;      // p = n + 1/2, 1 <= n <= 16: terms as x^n sqrt(x) (lp_term)
 };
+
+// One Lp term x^p, x in [0, 1] (the reference's powf(x, p), metric.hpp:204):
+// for half-integral p (ma.half, uniform across the grid) pow_half_f32, as
+// accurate as pow_f32 in a third of its instructions.
+__device__ __forceinline__ float lp_term(float x, const MetricArgs& ma) {
+    return ma.half ? pow_half_f32(x, ma.ip) : pow_f32(x, ma.p);
+}
 
 // ---- per-pair kernels, compile-time dimension -----------------------------
 // a: row point (wave-uniform, scalar loads); b: column point in VGPRs.
@@ -57,7 +65,7 @@ __device__ __forceinline__ float pair_dist(const float* __restrict__ a, const fl
                 const float v = __builtin_fabsf(a[k] - b[k]) * inv;
                 const float x = (v < 1.0f) ? v : 1.0f;
                 if constexpr (V == MV_LP_INT) s += ipow_ref(x, ma.ip);
-                else s += pow_f32(x, ma.p);
+                else s += lp_term(x, ma);
             }
             return m * pow_f32(s, ma.inv_p);
         }
@@ -127,7 +135,7 @@ __device__ __forceinline__ void rows_runtime(const float* __restrict__ x, u32 n,
                         const float v = __builtin_fabsf(a[t] - b[t]) * inv[r];
                         const float xx = (v < 1.0f) ? v : 1.0f;
                         if constexpr (V == MV_LP_INT) s += ipow_ref(xx, ma.ip);
-                        else s += pow_f32(xx, ma.p);
+                        else s += lp_term(xx, ma);
                     }
                 }
                 s2[r] = s;
@@ -192,7 +200,7 @@ __device__ __forceinline__ float pdist(const float* __restrict__ a, const float*
                 const float v = __builtin_fabsf(a[k] - b[k]) * inv;
                 const float x = (v < 1.0f) ? v : 1.0f;
                 if constexpr (V == MV_LP_INT) s += ipow_ref(x, ma.ip);
-                else s += pow_f32(x, ma.p);
+                else s += lp_term(x, ma);
             }
             return m * pow_f32(s, ma.inv_p);
         }
