@@ -40,6 +40,11 @@ class Restatement:
         L.tto_radius_graph.restype = C.c_uint64
         L.tto_knn.argtypes = [_f32p, C.c_uint32, C.c_uint32, C.c_int, C.c_float, C.c_uint32, _u32p]
         L.tto_knn.restype = C.c_uint32
+        L.tto_knn_rows.argtypes = [_f32p, C.c_uint32, C.c_uint32, C.c_int, C.c_float, C.c_uint32, _u32p,
+                                   C.c_uint32, _u32p]
+        L.tto_radius_row.argtypes = [_f32p, C.c_uint32, C.c_uint32, C.c_int, C.c_float, C.c_float, C.c_uint32,
+                                     _u32p, _f32p]
+        L.tto_radius_row.restype = C.c_uint64
         L.tto_knn_graph.argtypes = [_f32p, C.c_uint32, C.c_uint32, C.c_int, C.c_float, C.c_uint32,
                                     _u32p, _u64p, _u32p, _f32p]
         L.tto_knn_graph.restype = C.c_uint64
@@ -105,6 +110,27 @@ class Restatement:
         ids = np.zeros(max(n * kk, 1), np.uint32)
         kk = self.lib.tto_knn(_ptr(x, _f32p), n, d, kind, p, k, _ptr(ids, _u32p))
         return ids[: n * kk].reshape(n, kk)
+
+    def knn_rows(self, x, metric, k, rows):
+        """knn for the given query rows only: (len(rows), k) ids, k <= n - 1."""
+        x = _f32(x)
+        n, d = x.shape
+        kind, p = self.parse(metric)
+        rows = np.ascontiguousarray(rows, np.uint32)
+        ids = np.zeros(max(rows.size * k, 1), np.uint32)
+        self.lib.tto_knn_rows(_ptr(x, _f32p), n, d, kind, p, k, _ptr(rows, _u32p), rows.size, _ptr(ids, _u32p))
+        return ids[: rows.size * k].reshape(rows.size, k)
+
+    def radius_row(self, x, metric, tau, i):
+        """(neighbour ids, weights) of row i of the radius graph."""
+        x = _f32(x)
+        n, d = x.shape
+        kind, p = self.parse(metric)
+        m = self.lib.tto_radius_row(_ptr(x, _f32p), n, d, kind, p, tau, int(i), None, None)
+        col = np.zeros(max(m, 1), np.uint32)
+        w = np.zeros(max(m, 1), np.float32)
+        self.lib.tto_radius_row(_ptr(x, _f32p), n, d, kind, p, tau, int(i), _ptr(col, _u32p), _ptr(w, _f32p))
+        return col[:m], w[:m]
 
     def knn_graph(self, x, metric, ids):
         x = _f32(x)

@@ -229,6 +229,46 @@ uint32_t tto_knn(const float* pts, uint32_t n, uint32_t dim, int kind, float p, 
     return kk;
 }
 
+/* tto_knn for the given query rows only (knn_query per row, graph.hpp:
+ * 123-166): ids holds nrows * k (k already clamped to n - 1). */
+void tto_knn_rows(const float* pts, uint32_t n, uint32_t dim, int kind, float p, uint32_t k,
+                  const uint32_t* rows, uint32_t nrows, uint32_t* ids) {
+    cand_t* c = (cand_t*)malloc(sizeof(cand_t) * (n ? n : 1));
+    for (uint32_t r = 0; r < nrows; ++r) {
+        const uint64_t i = rows[r];
+        uint32_t m = 0;
+        for (uint32_t j = 0; j < n; ++j) {
+            if (j == i) continue;
+            c[m].d = tto_dist(kind, p, pts + i * dim, pts + (uint64_t)j * dim, dim);
+            c[m].id = j;
+            ++m;
+        }
+        qsort(c, m, sizeof(cand_t), cand_cmp);
+        for (uint32_t t = 0; t < k; ++t) ids[(uint64_t)r * k + t] = c[t].id;
+    }
+    free(c);
+}
+
+/* One row of radius_graph (graph.hpp:170-201): the j != i with
+ * c(i, j) <= tau, ascending, and their weights; col/w may be NULL to count.
+ * Returns the row's length. */
+uint64_t tto_radius_row(const float* pts, uint32_t n, uint32_t dim, int kind, float p, float tau,
+                        uint32_t i, uint32_t* col, float* w) {
+    uint64_t o = 0;
+    for (uint32_t j = 0; j < n; ++j) {
+        if (j == i) continue;
+        const float d = tto_dist(kind, p, pts + (uint64_t)i * dim, pts + (uint64_t)j * dim, dim);
+        if (d <= tau) {
+            if (col) {
+                col[o] = j;
+                w[o] = d;
+            }
+            ++o;
+        }
+    }
+    return o;
+}
+
 static int u32_cmp(const void* x, const void* y) {
     const uint32_t a = *(const uint32_t*)x, b = *(const uint32_t*)y;
     return (a > b) - (a < b);

@@ -27,6 +27,10 @@ uint64_t tto_radius_graph(const float* pts, uint32_t n, uint32_t dim, int kind, 
                           float tau, uint64_t* row_ptr, uint32_t* col, float* w);
 uint32_t tto_knn(const float* pts, uint32_t n, uint32_t dim, int kind, float p, uint32_t k,
                  uint32_t* ids);
+void tto_knn_rows(const float* pts, uint32_t n, uint32_t dim, int kind, float p, uint32_t k,
+                  const uint32_t* rows, uint32_t nrows, uint32_t* ids);
+uint64_t tto_radius_row(const float* pts, uint32_t n, uint32_t dim, int kind, float p, float tau,
+                        uint32_t i, uint32_t* col, float* w);
 uint64_t tto_knn_graph(const float* pts, uint32_t n, uint32_t dim, int kind, float p,
                        uint32_t k, const uint32_t* ids, uint64_t* row_ptr, uint32_t* col,
                        float* w);
