@@ -181,4 +181,25 @@ void hh_pow_f32(const float* x, uint64_t count, float p, float* out) {
 void hh_pow_half_f32(const float* x, uint64_t count, uint32_t n, float* out) {
     for (uint64_t t = 0; t < count; ++t) out[t] = ttg::pow_half_f32(x[t], n);
 }
+
+// The boundary engine's set and map primitives (turbotops_amd/host/
+// zerodiff.hpp), for direct checks against plain references.
+// xor_into: out = x ^ v for sorted sets; returns |out|.
+uint64_t hh_xor_into(const uint32_t* x, uint64_t nx, const uint32_t* v, uint64_t nv, uint32_t* out) {
+    std::vector<uint32_t> a(x, x + nx), b(v, v + nv), tmp;
+    tops::zd::xor_into(a, b, tmp);
+    std::copy(a.begin(), a.end(), out);
+    return a.size();
+}
+// RankMap: ops[3t] = 0 put(key, val), 1 get(key) -> res[t], 2 erase(key).
+void hh_rankmap(const uint32_t* ops, uint64_t count, uint32_t* res) {
+    tops::zd::RankMap m;
+    for (uint64_t t = 0; t < count; ++t) {
+        const uint32_t op = ops[3 * t], k = ops[3 * t + 1], v = ops[3 * t + 2];
+        res[t] = 0;
+        if (op == 0) m.put(k, v);
+        else if (op == 1) res[t] = m.get(k);
+        else m.erase(k);
+    }
+}
 }

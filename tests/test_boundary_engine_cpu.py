@@ -282,3 +282,73 @@ def test_brute_force_catches_a_wrong_tie_break(harness, restatement):
     bad_v = np.stack([lo[o], hi[o]], 1).astype(np.uint32)
     bad = canon(*engine_bars(harness, mat, bad_v, np.ascontiguousarray(dd[o], np.float32)))
     assert bad != want
+
+
+# ---- the engine's set and map primitives, directly ----------------------------
+@pytest.fixture(scope="module")
+def prims():
+    path = os.path.join(ROOT, "tests", "libhost_harness.so")
+    if not os.path.exists(path):
+        pytest.skip("tests/libhost_harness.so not built (make -C tests)")
+    L = C.CDLL(path)
+    L.hh_xor_into.argtypes = [_u32p, C.c_uint64, _u32p, C.c_uint64, _u32p]
+    L.hh_xor_into.restype = C.c_uint64
+    L.hh_rankmap.argtypes = [_u32p, C.c_uint64, _u32p]
+    return L
+
+
+@pytest.mark.parametrize("nx,nv", [(0, 1), (5, 1), (3000, 1), (4000, 3), (4000, 250), (100, 7), (50, 40),
+                                   (0, 9), (17, 0), (1600, 100), (1601, 100)])
+def test_xor_into_matches_set_symmetric_difference(prims, nx, nv):
+    """All three branches of xor_into (one toggle; a few elements into a
+    large set by binary search and run copies, v.size * 16 <= x.size; the
+    merge), with members present and absent, against Python's set ^."""
+    rng = np.random.default_rng(nx * 7 + nv)
+    for trial in range(20):
+        span = 3 * max(nx, 1) + 2 * nv
+        x = np.sort(rng.choice(span, size=nx, replace=False)).astype(np.uint32)
+        absent = np.setdiff1d(np.arange(span), x)
+        take_in = min(nv // 2, x.size)
+        v = np.unique(np.concatenate([rng.choice(x, size=take_in, replace=False) if take_in else x[:0],
+                                      rng.choice(absent, size=nv - take_in, replace=False)])).astype(np.uint32)
+        out = np.zeros(x.size + v.size + 1, np.uint32)
+        k = prims.hh_xor_into(x.ctypes.data_as(_u32p), x.size, v.ctypes.data_as(_u32p), v.size,
+                              out.ctypes.data_as(_u32p))
+        assert out[:k].tolist() == sorted(set(x.tolist()) ^ set(v.tolist()))
+
+
+def test_rankmap_matches_dict(prims):
+    """2M random puts, gets and erases (erase by backward shift, growth by
+    rehash) against a dict, keys > 0 in a narrow range so probe runs collide."""
+    rng = np.random.default_rng(8)
+    count = 2_000_000
+    op = rng.choice([0, 1, 2], size=count, p=[0.45, 0.35, 0.2]).astype(np.uint32)
+    key = rng.integers(1, 40000, size=count).astype(np.uint32)
+    val = rng.integers(1, 2**32 - 1, size=count, dtype=np.uint64).astype(np.uint32)
+    ops = np.ascontiguousarray(np.stack([op, key, val], 1))
+    res = np.zeros(count, np.uint32)
+    prims.hh_rankmap(ops.ctypes.data_as(_u32p), count, res.ctypes.data_as(_u32p))
+    ref = {}
+    want = np.zeros(count, np.uint32)
+    for t in range(count):
+        o, k = int(op[t]), int(key[t])
+        if o == 0:
+            ref[k] = int(val[t])
+        elif o == 1:
+            want[t] = ref.get(k, 0)
+        else:
+            ref.pop(k, None)
+    assert np.array_equal(res, want)
+
+
+@pytest.mark.parametrize("metric", ["l2", "l1", "linf"])
+def test_sixteen_dimensional_clouds_vs_reference(harness, restatement, reference, metric):
+    """16-D clouds, where the engine's resolves sum classes with many
+    generators (the binary-search branch of xor_into, generator-free slots,
+    G-set compaction): bar for bar against the reference, witness flags on
+    and off."""
+    x = np.random.default_rng({"l2": 71, "l1": 72, "linf": 73}[metric]).uniform(size=(900, 16)).astype(np.float32)
+    want, _ = reference.persistence(x, maxdim=1, metric=metric)
+    for witness in (False, True):
+        got, _, _ = engine_diagram(harness, restatement, x, metric, witness)
+        assert canon(*got) == canon(*want)
