@@ -2,15 +2,19 @@
 // sort.hip — exclusive prefix sums and a stable LSD radix sort (64-bit keys,
 // optional 32-bit payload) for gfx950.
 //
-// Radix sort, one pass per 8-bit digit:
-//   upsweep    per-block digit histogram (LDS atomics, one counter row per
-//              wave64) -> hist[digit][block]
+// Radix sort, one pass per 8-bit digit, on a persistent grid (at most 2048
+// workgroups, each owning a contiguous range of 4096-key tiles; the ranges
+// are taken from the key count on the device, so a sort sized for a large
+// capacity costs little when few keys are present):
+//   upsweep    per-block digit histogram over its range (LDS atomics, one
+//              counter row per wave64) -> hist[digit][block]
 //   scan       exclusive scan of hist in digit-major order: the global start
 //              of every (digit, block) run
-//   downsweep  each block ranks its 4096 keys stably — wave64 ballots on the
-//              digit bits give the lanes sharing a digit, mbcnt the rank among
-//              them, per-wave running counters keep (t, lane) order — reorders
-//              them in LDS, then writes each digit run contiguously.
+//   downsweep  each block walks its tiles in order, ranks a tile's keys
+//              stably — wave64 ballots on the digit bits give the lanes sharing
+//              a digit, mbcnt the rank among them, per-wave running counters
+//              keep (t, lane) order — reorders them in LDS, and appends each
+//              digit's run at the block's running offset for that digit.
 #include <algorithm>
 
 #include "sort.hpp"
@@ -150,128 +154,163 @@ void exclusive_scan(const T* in, T* out, u64 count, T* d_total, Workspace& ws, h
 
 // ---- radix sort ---------------------------------------------------------
 constexpr u32 RB = 256;             // threads (4 wave64s)
-constexpr u32 RI = 16;              // keys per thread
-constexpr u32 RTILE = RB * RI;      // keys per block
+constexpr u32 RI = 16;              // keys per thread and tile
+constexpr u32 RTILE = RB * RI;      // keys per tile
 constexpr u32 RADIX = 256;
-constexpr u32 WSEG = RTILE / (RB / 64);   // keys per wave segment
+constexpr u32 WSEG = RTILE / (RB / 64);   // keys per wave segment of a tile
+constexpr u32 MAX_SORT_BLOCKS = 2048;     // persistent workgroups (8 per CU)
 
 __device__ __forceinline__ u32 lane_rank(u64 mask) {
     return __builtin_amdgcn_mbcnt_hi(u32(mask >> 32), __builtin_amdgcn_mbcnt_lo(u32(mask), 0u));
 }
 
+// The contiguous range of tiles block b of nb owns once `count` keys are
+// known on the device: ranges follow block order, so every digit's keys of
+// block b land in one contiguous run after those of blocks below b.
+__device__ __forceinline__ void block_tiles(u64 count, u32 nb, u32 b, u64& t0, u64& t1) {
+    const u64 tiles = (count + RTILE - 1) / RTILE;
+    t0 = tiles * b / nb;
+    t1 = tiles * (b + 1) / nb;
+}
+
 __global__ __launch_bounds__(RB) void rs_upsweep(const u64* __restrict__ keys, u64 count,
                                                  const u32* __restrict__ d_count, u32 shift,
-                                                 u32 dmask, u32* __restrict__ hist, u32 nblocks) {
+                                                 u32 dmask, u32* __restrict__ hist) {
     __shared__ u32 h[RB / 64][RADIX];
     if (d_count) count = min<u64>(count, *d_count);
+    u64 t0, t1;
+    block_tiles(count, gridDim.x, blockIdx.x, t0, t1);
     for (u32 t = threadIdx.x; t < (RB / 64) * RADIX; t += RB) (&h[0][0])[t] = 0;
     __syncthreads();
     const u32 lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const u64 base = u64(blockIdx.x) * RTILE + u64(w) * WSEG;
+    for (u64 tile = t0; tile < t1; ++tile) {
+        const u64 base = tile * RTILE + u64(w) * WSEG;
 #pragma unroll 4
-    for (u32 t = 0; t < RI; ++t) {
-        const u64 i = base + t * 64 + lane;
-        if (i < count) atomicAdd(&h[w][u32(keys[i] >> shift) & dmask], 1u);
+        for (u32 t = 0; t < RI; ++t) {
+            const u64 i = base + t * 64 + lane;
+            if (i < count) atomicAdd(&h[w][u32(keys[i] >> shift) & dmask], 1u);
+        }
     }
     __syncthreads();
     for (u32 d = threadIdx.x; d < RADIX; d += RB) {
         u32 c = 0;
 #pragma unroll
         for (u32 v = 0; v < RB / 64; ++v) c += h[v][d];
-        hist[u64(d) * nblocks + blockIdx.x] = c;
+        hist[u64(d) * gridDim.x + blockIdx.x] = c;
     }
 }
 
+// Each block walks its tiles in order: ranks a tile's keys stably (wave64
+// ballots on the digit bits give the lanes sharing a digit, mbcnt the rank
+// among them, per-wave running counters keep (t, lane) order), reorders
+// them in LDS by digit, and writes every digit's run at the block's running
+// offset for that digit, which then advances. A block's writes to one digit
+// are therefore one contiguous stream in time as in memory, and partial
+// cache lines of one tile are completed by the next in the same L2.
 template <bool VALS>
 __global__ __launch_bounds__(RB) void rs_downsweep(const u64* __restrict__ keys,
                                                    const u32* __restrict__ vals, u64 count,
                                                    const u32* __restrict__ d_count,
                                                    u32 shift, u32 dmask, u32 dbits,
-                                                   const u32* __restrict__ offs, u32 nblocks,
+                                                   const u32* __restrict__ offs,
                                                    u64* __restrict__ keys_out,
                                                    u32* __restrict__ vals_out) {
     __shared__ u32 wrun[RB / 64][RADIX];
     __shared__ u32 bstart[RADIX];
+    __shared__ u32 run[RADIX];      // this block's next output position per digit
     __shared__ u64 skeys[RTILE];
     __shared__ u32 svals[VALS ? RTILE : 1];
 
     const u32 lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     if (d_count) count = min<u64>(count, *d_count);
-    const u64 tile0 = u64(blockIdx.x) * RTILE;
-    if (tile0 >= count) return;   // whole block idle (uniform)
-    for (u32 t = threadIdx.x; t < (RB / 64) * RADIX; t += RB) (&wrun[0][0])[t] = 0;
-    __syncthreads();
+    u64 t0, t1;
+    block_tiles(count, gridDim.x, blockIdx.x, t0, t1);
+    if (t0 >= t1) return;   // uniform: the block owns no tile
+    run[threadIdx.x] = offs[u64(threadIdx.x) * gridDim.x + blockIdx.x];   // RB == RADIX
 
-    const u64 base = tile0 + u64(w) * WSEG;
-    u64 k[RI];
-    u32 v[RI];
-#pragma unroll
-    for (u32 t = 0; t < RI; ++t) {
-        const u64 i = base + t * 64 + lane;
-        const bool ok = i < count;
-        k[t] = ok ? keys[i] : 0;
-        if constexpr (VALS) v[t] = ok ? vals[i] : 0;
-        if (ok) atomicAdd(&wrun[w][u32(k[t] >> shift) & dmask], 1u);
-    }
-    __syncthreads();
+    for (u64 tile = t0; tile < t1; ++tile) {
+        const u64 tile0 = tile * RTILE;
+        for (u32 t = threadIdx.x; t < (RB / 64) * RADIX; t += RB) (&wrun[0][0])[t] = 0;
+        __syncthreads();
 
-    // block-local digit starts and each wave's running start per digit
-    {
-        const u32 d = threadIdx.x;   // RB == RADIX
-        u32 c[RB / 64], tot = 0;
+        const u64 base = tile0 + u64(w) * WSEG;
+        u64 k[RI];
+        u32 v[RI];
 #pragma unroll
-        for (u32 q = 0; q < RB / 64; ++q) {
-            c[q] = wrun[q][d];
-            tot += c[q];
+        for (u32 t = 0; t < RI; ++t) {
+            const u64 i = base + t * 64 + lane;
+            const bool ok = i < count;
+            k[t] = ok ? keys[i] : 0;
+            if constexpr (VALS) v[t] = ok ? vals[i] : 0;
+            if (ok) atomicAdd(&wrun[w][u32(k[t] >> shift) & dmask], 1u);
         }
-        u32 all;
-        const u32 start = block_exclusive_scan(tot, &all);
-        bstart[d] = start;
-        u32 run = start;
-#pragma unroll
-        for (u32 q = 0; q < RB / 64; ++q) {
-            wrun[q][d] = run;
-            run += c[q];
-        }
-    }
-    __syncthreads();
+        __syncthreads();
 
-    // stable ranking, (t, lane) order within each wave segment
+        // tile-local digit starts, and each wave's running start per digit
+        u32 tot = 0;
+        {
+            const u32 d = threadIdx.x;
+            u32 c[RB / 64];
 #pragma unroll
-    for (u32 t = 0; t < RI; ++t) {
-        const u64 i = base + t * 64 + lane;
-        const bool ok = i < count;
-        const u32 d = u32(k[t] >> shift) & dmask;
-        u64 peers = __ballot(ok);
-        for (u32 b = 0; b < dbits; ++b) {
-            const u64 m = __ballot((d >> b) & 1u);
-            peers &= ((d >> b) & 1u) ? m : ~m;
+            for (u32 q = 0; q < RB / 64; ++q) {
+                c[q] = wrun[q][d];
+                tot += c[q];
+            }
+            u32 all;
+            const u32 start = block_exclusive_scan(tot, &all);
+            bstart[d] = start;
+            u32 r = start;
+#pragma unroll
+            for (u32 q = 0; q < RB / 64; ++q) {
+                wrun[q][d] = r;
+                r += c[q];
+            }
         }
-        const u32 rank = lane_rank(peers);
-        u32 pos = 0;
-        if (ok) pos = wrun[w][d] + rank;
-        __builtin_amdgcn_wave_barrier();
-        if (ok && rank == 0) wrun[w][d] = pos + u32(__popcll(peers));
-        __builtin_amdgcn_wave_barrier();
-        if (ok) {
-            skeys[pos] = k[t];
-            if constexpr (VALS) svals[pos] = v[t];
-        }
-    }
-    __syncthreads();
+        __syncthreads();
 
-    const u32 tile_count = u32(min<u64>(RTILE, count - tile0));
+#pragma unroll
+        for (u32 t = 0; t < RI; ++t) {
+            const u64 i = base + t * 64 + lane;
+            const bool ok = i < count;
+            const u32 d = u32(k[t] >> shift) & dmask;
+            u64 peers = __ballot(ok);
+            for (u32 bb = 0; bb < dbits; ++bb) {
+                const u64 m = __ballot((d >> bb) & 1u);
+                peers &= ((d >> bb) & 1u) ? m : ~m;
+            }
+            const u32 rank = lane_rank(peers);
+            u32 pos = 0;
+            if (ok) pos = wrun[w][d] + rank;
+            __builtin_amdgcn_wave_barrier();
+            if (ok && rank == 0) wrun[w][d] = pos + u32(__popcll(peers));
+            __builtin_amdgcn_wave_barrier();
+            if (ok) {
+                skeys[pos] = k[t];
+                if constexpr (VALS) svals[pos] = v[t];
+            }
+        }
+        __syncthreads();
+
+        const u32 tile_count = u32(min<u64>(RTILE, count - tile0));
 #pragma unroll 4
-    for (u32 t = 0; t < RI; ++t) {
-        const u32 q = t * RB + threadIdx.x;
-        if (q < tile_count) {
-            const u64 key = skeys[q];
-            const u32 d = u32(key >> shift) & dmask;
-            const u64 g = u64(offs[u64(d) * nblocks + blockIdx.x]) + (q - bstart[d]);
-            keys_out[g] = key;
-            if constexpr (VALS) vals_out[g] = svals[q];
+        for (u32 t = 0; t < RI; ++t) {
+            const u32 q = t * RB + threadIdx.x;
+            if (q < tile_count) {
+                const u64 key = skeys[q];
+                const u32 d = u32(key >> shift) & dmask;
+                const u64 g = u64(run[d]) + (q - bstart[d]);
+                keys_out[g] = key;
+                if constexpr (VALS) vals_out[g] = svals[q];
+            }
         }
+        __syncthreads();
+        run[threadIdx.x] += tot;   // this tile's keys of digit threadIdx.x
+        __syncthreads();
     }
+}
+
+u32 sort_blocks(u64 capacity) {
+    return u32(std::max<u64>(1, std::min<u64>(MAX_SORT_BLOCKS, (capacity + RTILE - 1) / RTILE)));
 }
 
 }  // namespace
@@ -306,16 +345,16 @@ void radix_sort_u64_counted(u64* keys, u64 capacity, const u32* d_count, int lo_
                             u64* keys_alt, u32* hist, u32* scan_tmp, hipStream_t s) {
     if (capacity <= 1 || hi_bit <= lo_bit) return;
     if (capacity >= (u64(1) << 32)) fail(TT_ELENGTH, "radix_sort: too many keys for one sort");
-    const u32 nblocks = u32((capacity + RTILE - 1) / RTILE);
+    const u32 nb = sort_blocks(capacity);
     u64 *ks = keys, *kd = keys_alt;
     for (int shift = lo_bit; shift < hi_bit; shift += 8) {
         const u32 dbits = u32(std::min(8, hi_bit - shift));
         const u32 dmask = (1u << dbits) - 1u;
-        rs_upsweep<<<nblocks, RB, 0, s>>>(ks, capacity, d_count, u32(shift), dmask, hist, nblocks);
+        rs_upsweep<<<nb, RB, 0, s>>>(ks, capacity, d_count, u32(shift), dmask, hist);
         TTG_LAUNCH_CHECK();
-        exclusive_scan_buf<u32>(hist, hist, u64(RADIX) * nblocks, nullptr, scan_tmp, s);
-        rs_downsweep<false><<<nblocks, RB, 0, s>>>(ks, nullptr, capacity, d_count, u32(shift), dmask, dbits,
-                                                   hist, nblocks, kd, nullptr);
+        exclusive_scan_buf<u32>(hist, hist, u64(RADIX) * nb, nullptr, scan_tmp, s);
+        rs_downsweep<false><<<nb, RB, 0, s>>>(ks, nullptr, capacity, d_count, u32(shift), dmask, dbits, hist,
+                                              kd, nullptr);
         TTG_LAUNCH_CHECK();
         std::swap(ks, kd);
     }
@@ -325,29 +364,28 @@ void radix_sort_u64_counted(u64* keys, u64 capacity, const u32* d_count, int lo_
     }
 }
 
-u64 radix_hist_entries(u64 capacity) { return u64(RADIX) * ((capacity + RTILE - 1) / RTILE); }
+u64 radix_hist_entries(u64 capacity) { return u64(RADIX) * sort_blocks(capacity); }
 
 void radix_sort_u64(u64* keys, u32* vals, u64 count, int lo_bit, int hi_bit, u64* keys_alt,
                     u32* vals_alt, Workspace& ws, hipStream_t s) {
     if (count <= 1 || hi_bit <= lo_bit) return;
     if (count >= (u64(1) << 32)) fail(TT_ELENGTH, "radix_sort: too many keys for one sort");
-    const u32 nblocks = u32((count + RTILE - 1) / RTILE);
-    u32* hist = ws.get<u32>(u64(RADIX) * nblocks);
+    const u32 nb = sort_blocks(count);
+    u32* hist = ws.get<u32>(u64(RADIX) * nb);
+    u32* stmp = ws.get<u32>(std::max<u64>(1, scan_tmp_entries(u64(RADIX) * nb)));
     u64 *ks = keys, *kd = keys_alt;
     u32 *vs = vals, *vd = vals_alt;
-    int passes = 0;
-    for (int shift = lo_bit; shift < hi_bit; shift += 8, ++passes) {
+    for (int shift = lo_bit; shift < hi_bit; shift += 8) {
         const u32 dbits = u32(std::min(8, hi_bit - shift));
         const u32 dmask = (1u << dbits) - 1u;
-        rs_upsweep<<<nblocks, RB, 0, s>>>(ks, count, nullptr, u32(shift), dmask, hist, nblocks);
+        rs_upsweep<<<nb, RB, 0, s>>>(ks, count, nullptr, u32(shift), dmask, hist);
         TTG_LAUNCH_CHECK();
-        exclusive_scan_u32(hist, hist, u64(RADIX) * nblocks, nullptr, ws, s);
+        exclusive_scan_buf<u32>(hist, hist, u64(RADIX) * nb, nullptr, stmp, s);
         if (vals)
-            rs_downsweep<true><<<nblocks, RB, 0, s>>>(ks, vs, count, nullptr, u32(shift), dmask, dbits,
-                                                      hist, nblocks, kd, vd);
+            rs_downsweep<true><<<nb, RB, 0, s>>>(ks, vs, count, nullptr, u32(shift), dmask, dbits, hist, kd, vd);
         else
-            rs_downsweep<false><<<nblocks, RB, 0, s>>>(ks, nullptr, count, nullptr, u32(shift), dmask,
-                                                       dbits, hist, nblocks, kd, nullptr);
+            rs_downsweep<false><<<nb, RB, 0, s>>>(ks, nullptr, count, nullptr, u32(shift), dmask, dbits, hist, kd,
+                                                  nullptr);
         TTG_LAUNCH_CHECK();
         std::swap(ks, kd);
         std::swap(vs, vd);
