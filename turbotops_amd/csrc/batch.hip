@@ -13,10 +13,11 @@
 // with per-cloud offsets. Kernels take fixed grids and read every
 // data-dependent count from device memory, so the whole sequence can be
 // replayed from a graph:
-//   batch_dist     persistent: blocks stride over the global tile list; a
+//   batch_dist     persistent, one launch per dimension group (2, 3, 8,
+//                  other): blocks stride over the group's tile range; a
 //                  per-tile cloud table names the cloud, and the tile runs the
-//                  single-cloud tile body (tiles.hpp) for that cloud's
-//                  dimension
+//                  single-cloud tile body (tiles.hpp) specialised for the
+//                  group's dimension
 //   batch_radius   one workgroup per cloud: min of the row maxima
 //   batch_count    one wave per global row: kept edges j < i
 //   scan           global row offsets; cloud segments follow
@@ -73,30 +74,37 @@ struct BatchShape {
 // host copy is unpadded.
 __host__ __device__ __forceinline__ u32 row_ld(u32 n) { return (n + 31u) & ~31u; }
 
-template <int V, int RPW>
+// Dimension groups of the tile list: tiles are numbered group by group
+// (clouds of dimension 2, then 3, then 8, then every other dimension), cloud
+// by cloud within a group, so each group is one contiguous tile range and one
+// launch of a kernel specialised for its dimension (0: runtime dimension).
+// One kernel over every dimension holds the registers of its heaviest branch
+// (the d = 8 rows) for all of them and spills SGPRs.
+constexpr int kDimGroups = 4;
+constexpr u32 kGroupDim[kDimGroups] = {2, 3, 8, 0};
+inline int dim_group(u32 d) { return d == 2 ? 0 : d == 3 ? 1 : d == 8 ? 2 : 3; }
+
+template <int V, int RPW, int DIM>
 __global__ __launch_bounds__(64 * (64 / RPW)) void batch_dist(BatchShape b, MetricArgs ma,
                                                               const float* __restrict__ pts,
                                                               float* __restrict__ mats,
-                                                              u32* __restrict__ rowmax) {
+                                                              u32* __restrict__ rowmax, u64 t_begin,
+                                                              u64 t_end) {
     __shared__ TileSmem<RPW> sm;
     // Tiles in order: cloud by cloud, row-major lower triangle within one.
     // (A scattered order, t * A mod T, meant to spread the row-maximum
     // atomics of small clouds over more cache lines, measured 6% slower on
     // the 512-cloud set: 3.01 against 2.83 ms.)
-    for (u64 t = blockIdx.x; t < b.total_tiles; t += gridDim.x) {
+    for (u64 t = t_begin + blockIdx.x; t < t_end; t += gridDim.x) {
         const u32 c = __builtin_amdgcn_readfirstlane(b.tile_cloud[t]);
-        const u32 n = b.n[c], dim = b.dim[c];
+        const u32 n = b.n[c];
         const float* x = pts + b.pt_off[c];
         float* mat = mats + b.mat_off[c];
         u32* rm = rowmax + b.row_base[c];
         u32 bi, bj;
         lower_tile_of(t - b.tile_base[c], bi, bj);
-        switch (dim) {
-            case 2: dist_tile<V, 2, TT_LAYOUT_FULL, RPW>(x, n, dim, ma, bi, bj, mat, rm, sm, row_ld(n)); break;
-            case 3: dist_tile<V, 3, TT_LAYOUT_FULL, RPW>(x, n, dim, ma, bi, bj, mat, rm, sm, row_ld(n)); break;
-            case 8: dist_tile<V, 8, TT_LAYOUT_FULL, RPW>(x, n, dim, ma, bi, bj, mat, rm, sm, row_ld(n)); break;
-            default: dist_tile<V, 0, TT_LAYOUT_FULL, RPW>(x, n, dim, ma, bi, bj, mat, rm, sm, row_ld(n)); break;
-        }
+        dist_tile<V, DIM, TT_LAYOUT_FULL, RPW>(x, n, DIM ? u32(DIM) : b.dim[c], ma, bi, bj, mat, rm, sm,
+                                               row_ld(n));
         __syncthreads();
     }
 }
@@ -326,24 +334,42 @@ __global__ void batch_kept_segments(const u32* __restrict__ seg, const u32* __re
     }
 }
 
-template <int V>
-void launch_batch_dist(int rpw, u32 grid, hipStream_t s, const BatchShape& b, MetricArgs ma,
-                       const float* pts, float* mats, u32* rowmax) {
-    switch (rpw) {
-        case 8: batch_dist<V, 8><<<grid, 512, 0, s>>>(b, ma, pts, mats, rowmax); break;
-        case 32: batch_dist<V, 32><<<grid, 128, 0, s>>>(b, ma, pts, mats, rowmax); break;
-        default: batch_dist<V, 16><<<grid, 256, 0, s>>>(b, ma, pts, mats, rowmax); break;
+template <int V, int RPW>
+void launch_batch_dist_rpw(const u64* gt, hipStream_t s, const BatchShape& b, MetricArgs ma, const float* pts,
+                           float* mats, u32* rowmax) {
+    constexpr u32 threads = 64 * (64 / RPW);
+    for (int g = 0; g < kDimGroups; ++g) {
+        const u64 t0 = gt[g], t1 = gt[g + 1];
+        if (t1 <= t0) continue;
+        const u32 grid = u32(t1 - t0);   // one tile per workgroup
+        switch (kGroupDim[g]) {
+            case 2: batch_dist<V, RPW, 2><<<grid, threads, 0, s>>>(b, ma, pts, mats, rowmax, t0, t1); break;
+            case 3: batch_dist<V, RPW, 3><<<grid, threads, 0, s>>>(b, ma, pts, mats, rowmax, t0, t1); break;
+            case 8: batch_dist<V, RPW, 8><<<grid, threads, 0, s>>>(b, ma, pts, mats, rowmax, t0, t1); break;
+            default: batch_dist<V, RPW, 0><<<grid, threads, 0, s>>>(b, ma, pts, mats, rowmax, t0, t1); break;
+        }
     }
 }
 
-void launch_batch_dist_v(MetricVariant v, int rpw, u32 grid, hipStream_t s, const BatchShape& b,
+template <int V>
+void launch_batch_dist(int rpw, const u64* gt, hipStream_t s, const BatchShape& b, MetricArgs ma,
+                       const float* pts, float* mats, u32* rowmax) {
+    switch (rpw) {
+        case 8: launch_batch_dist_rpw<V, 8>(gt, s, b, ma, pts, mats, rowmax); break;
+        case 32: launch_batch_dist_rpw<V, 32>(gt, s, b, ma, pts, mats, rowmax); break;
+        default: launch_batch_dist_rpw<V, 16>(gt, s, b, ma, pts, mats, rowmax); break;
+    }
+}
+
+// gt: kDimGroups + 1 tile-range boundaries, one range per dimension group.
+void launch_batch_dist_v(MetricVariant v, int rpw, const u64* gt, hipStream_t s, const BatchShape& b,
                          MetricArgs ma, const float* pts, float* mats, u32* rowmax) {
     switch (v) {
-        case MV_L1: launch_batch_dist<MV_L1>(rpw, grid, s, b, ma, pts, mats, rowmax); break;
-        case MV_L2: launch_batch_dist<MV_L2>(rpw, grid, s, b, ma, pts, mats, rowmax); break;
-        case MV_LINF: launch_batch_dist<MV_LINF>(rpw, grid, s, b, ma, pts, mats, rowmax); break;
-        case MV_LP_INT: launch_batch_dist<MV_LP_INT>(rpw, grid, s, b, ma, pts, mats, rowmax); break;
-        default: launch_batch_dist<MV_LP_POW>(rpw, grid, s, b, ma, pts, mats, rowmax); break;
+        case MV_L1: launch_batch_dist<MV_L1>(rpw, gt, s, b, ma, pts, mats, rowmax); break;
+        case MV_L2: launch_batch_dist<MV_L2>(rpw, gt, s, b, ma, pts, mats, rowmax); break;
+        case MV_LINF: launch_batch_dist<MV_LINF>(rpw, gt, s, b, ma, pts, mats, rowmax); break;
+        case MV_LP_INT: launch_batch_dist<MV_LP_INT>(rpw, gt, s, b, ma, pts, mats, rowmax); break;
+        default: launch_batch_dist<MV_LP_POW>(rpw, gt, s, b, ma, pts, mats, rowmax); break;
     }
 }
 
@@ -368,7 +394,7 @@ struct BatchState {
     std::vector<u32> n, dim;
     std::vector<u64> pt_off, mat_off, dmat_off, row_base, tile_base, pair_base, bmin_off;
     u64 total_pts = 0, total_mat = 0, total_rows = 0, total_tiles = 0, capacity = 0, h_cap = 0;
-    u32 grid_dist = 0;
+    u64 group_tiles[kDimGroups + 1] = {};   // tile range of each dimension group
     // device
     Workspace ws;
     float *d_pts = nullptr, *d_mats = nullptr, *d_thr = nullptr, *d_out_d = nullptr, *d_bmin = nullptr;
@@ -409,7 +435,7 @@ struct BatchState {
     void enqueue(int rpw_use) {
         hipStream_t s = stream;
         fill_device_u32(d_rowmax, std::max<u64>(total_rows, 1), 0u, s);
-        if (total_tiles) launch_batch_dist_v(variant, rpw_use, grid_dist, s, shape, ma, d_pts, d_mats, d_rowmax);
+        if (total_tiles) launch_batch_dist_v(variant, rpw_use, group_tiles, s, shape, ma, d_pts, d_mats, d_rowmax);
         TTG_LAUNCH_CHECK();
         batch_radius<<<nclouds, 256, 0, s>>>(shape, d_rowmax, threshold, d_thr);
         fill_device_u32(d_cnt, total_rows + 1, 0u, s);
@@ -482,6 +508,7 @@ void batch_setup(BatchState& b, u32 nclouds, const uint32_t* n, const uint32_t* 
     b.pair_base.assign(nclouds + 1, 0);
     b.bmin_off.assign(nclouds + 1, 0);
     u64 maxM = 1;
+    std::vector<u64> ntiles(nclouds);
     for (u32 c = 0; c < nclouds; ++c) {
         if (b.n[c] && b.dim[c] == 0) fail(TT_EINVAL, "tt_batch_create: dim must be >= 1");
         if (b.n[c] > 92682) fail(TT_ELENGTH, "tt_batch_create: a cloud exceeds 92682 points");
@@ -492,21 +519,34 @@ void batch_setup(BatchState& b, u32 nclouds, const uint32_t* n, const uint32_t* 
         b.mat_off[c + 1] = b.mat_off[c] + u64(b.n[c]) * b.n[c];
         b.dmat_off[c + 1] = b.dmat_off[c] + u64(b.n[c]) * row_ld(b.n[c]);
         b.row_base[c + 1] = b.row_base[c] + b.n[c];
-        b.tile_base[c + 1] = b.tile_base[c] + nb * (nb + 1) / 2;
+        ntiles[c] = nb * (nb + 1) / 2;
         b.pair_base[c + 1] = b.pair_base[c] + M;
         b.bmin_off[c + 1] = b.bmin_off[c] + u64(b.n[c]) * ((u64(b.n[c]) + BLK - 1) / BLK);
     }
     b.total_pts = b.pt_off[nclouds];
     b.total_mat = b.mat_off[nclouds];
     b.total_rows = b.row_base[nclouds];
-    b.total_tiles = b.tile_base[nclouds];
+    // tile_base[c]: the first tile of cloud c, numbered group by group
+    {
+        u64 gcount[kDimGroups] = {};
+        for (u32 c = 0; c < nclouds; ++c) gcount[dim_group(b.dim[c])] += ntiles[c];
+        for (int g = 0; g < kDimGroups; ++g) b.group_tiles[g + 1] = b.group_tiles[g] + gcount[g];
+        u64 next[kDimGroups];
+        for (int g = 0; g < kDimGroups; ++g) next[g] = b.group_tiles[g];
+        for (u32 c = 0; c < nclouds; ++c) {
+            u64& at = next[dim_group(b.dim[c])];
+            b.tile_base[c] = at;
+            at += ntiles[c];
+        }
+        b.tile_base[nclouds] = b.group_tiles[kDimGroups];
+    }
+    b.total_tiles = b.group_tiles[kDimGroups];
     b.capacity = std::max<u64>(b.pair_base[nclouds], 1);
     if (b.capacity >= (u64(1) << 32)) fail(TT_ELENGTH, "tt_batch_create: more than 2^32 vertex pairs in one batch");
     b.ib = std::max(1, bit_length(maxM - 1));
     b.cb = bit_length(u64(nclouds - 1));
     if (b.cb + 31 + b.ib > 64) fail(TT_ELENGTH, "tt_batch_create: edge keys exceed 64 bits");
     b.witness_fits = b.cb + 31 + b.ib + 1 <= 64;
-    b.grid_dist = u32(std::max<u64>(1, std::min<u64>(b.total_tiles, 256 * 16)));
 
     Workspace& ws = b.ws;
     b.d_pts = ws.get<float>(std::max<u64>(b.total_pts, 1));
@@ -552,7 +592,7 @@ void batch_setup(BatchState& b, u32 nclouds, const uint32_t* n, const uint32_t* 
     {   // per tile and per row, the cloud it belongs to (one load instead of a search)
         std::vector<u32> tc(std::max<u64>(b.total_tiles, 1)), rc(std::max<u64>(b.total_rows, 1));
         for (u32 c = 0; c < nclouds; ++c) {
-            std::fill(tc.begin() + b.tile_base[c], tc.begin() + b.tile_base[c + 1], c);
+            std::fill(tc.begin() + b.tile_base[c], tc.begin() + b.tile_base[c] + ntiles[c], c);
             std::fill(rc.begin() + b.row_base[c], rc.begin() + b.row_base[c + 1], c);
         }
         TTG_HIP(hipMemcpyAsync(b.d_tile_cloud, tc.data(), tc.size() * 4, hipMemcpyHostToDevice, s));
@@ -596,7 +636,7 @@ void batch_autotune(BatchState& b) {
             fill_device_u32(b.d_rowmax, std::max<u64>(b.total_rows, 1), 0u, b.stream);
             TTG_HIP(hipEventRecord(b.t0, b.stream));
             if (b.total_tiles)
-                launch_batch_dist_v(b.variant, rpw, b.grid_dist, b.stream, b.shape, b.ma, b.d_pts, b.d_mats,
+                launch_batch_dist_v(b.variant, rpw, b.group_tiles, b.stream, b.shape, b.ma, b.d_pts, b.d_mats,
                                     b.d_rowmax);
             TTG_HIP(hipEventRecord(b.t1, b.stream));
             TTG_HIP(hipEventSynchronize(b.t1));
@@ -775,7 +815,7 @@ tt_status tt_batch_time_dist(tt_batch* b, float* ms) {
             fill_device_u32(st.d_rowmax, std::max<u64>(st.total_rows, 1), 0u, st.stream);
             TTG_HIP(hipEventRecord(st.t0, st.stream));
             if (st.total_tiles)
-                launch_batch_dist_v(st.variant, st.rpw, st.grid_dist, st.stream, st.shape, st.ma, st.d_pts,
+                launch_batch_dist_v(st.variant, st.rpw, st.group_tiles, st.stream, st.shape, st.ma, st.d_pts,
                                     st.d_mats, st.d_rowmax);
             TTG_HIP(hipEventRecord(st.t1, st.stream));
             TTG_HIP(hipEventSynchronize(st.t1));

@@ -147,10 +147,64 @@ __device__ __forceinline__ void rows_runtime(const float* __restrict__ x, u32 n,
     }
 }
 
+// L1, L2 and L-inf rows from DIM >= 8 (DIM a multiple of 4), four coordinates
+// at a time: the chunk loop is kept rolled, so only one chunk of the R row
+// points (4R scalars) is live in SGPRs. Fully unrolled, R x DIM row scalars
+// exceed the 106 SGPRs and spill through v_writelane / v_readlane, which
+// nearly doubles the VALU work per pair at DIM = 16. The column chunk is
+// re-read per step (64 lanes x 16 B, cache hits). Each pair still accumulates
+// strictly in k order.
+template <int V, int DIM, int R, bool KEY, bool CLAMP>
+__device__ __forceinline__ void rows_chunked_run(const float* __restrict__ x, u32 n, u32 ibase,
+                                                 u32 jc, float (&acc)[R]) {
+    constexpr u32 C = 4;
+    const float* __restrict__ xb = x + u64(jc) * DIM;
+#pragma unroll 1
+    for (u32 k0 = 0; k0 < u32(DIM); k0 += C) {
+        float b[C];
+#pragma unroll
+        for (u32 t = 0; t < C; ++t) b[t] = xb[k0 + t];
+#pragma unroll
+        for (u32 r = 0; r < R; ++r) {
+            const u32 i = CLAMP ? min(ibase + r, n - 1) : ibase + r;
+            const float* __restrict__ a = x + u64(i) * DIM + k0;
+            float s = acc[r];
+#pragma unroll
+            for (u32 t = 0; t < C; ++t) {
+                const float d = a[t] - b[t];
+                if constexpr (V == MV_L2) s = __builtin_fmaf(d, d, s);
+                else if constexpr (V == MV_L1) s += __builtin_fabsf(d);
+                else s = fmax_acc(s, __builtin_fabsf(d));
+            }
+            acc[r] = s;
+        }
+    }
+}
+
+template <int V, int DIM, int R, bool KEY>
+__device__ __forceinline__ void rows_chunked(const float* __restrict__ x, u32 n, u32 ibase, u32 jc,
+                                             float (&val)[R]) {
+    float acc[R];
+#pragma unroll
+    for (u32 r = 0; r < R; ++r) acc[r] = 0.0f;
+    // rows past n (the last row block) read row n - 1; their values are dropped
+    if (ibase + R <= n) rows_chunked_run<V, DIM, R, KEY, false>(x, n, ibase, jc, acc);
+    else rows_chunked_run<V, DIM, R, KEY, true>(x, n, ibase, jc, acc);
+#pragma unroll
+    for (u32 r = 0; r < R; ++r) {
+        if constexpr (V == MV_L2 && !KEY) val[r] = __builtin_sqrtf(acc[r]);
+        else val[r] = acc[r];
+    }
+}
+
 template <int V, int DIM, int R, bool KEY = false>
 __device__ __forceinline__ void rows_static(const float* __restrict__ x, u32 n,
                                             const MetricArgs& ma, u32 ibase, u32 jc,
                                             float (&val)[R]) {
+    if constexpr (DIM >= 8 && DIM % 4 == 0 && (V == MV_L1 || V == MV_L2 || V == MV_LINF)) {
+        rows_chunked<V, DIM, R, KEY>(x, n, ibase, jc, val);
+        return;
+    }
     float b[DIM];
 #pragma unroll
     for (int k = 0; k < DIM; ++k) b[k] = x[u64(jc) * DIM + k];

@@ -38,6 +38,62 @@ struct TileSmem {
     float part_col[NW][DTILE];
 };
 
+// One 64-float row segment at a wave-uniform base, stored through a buffer
+// resource whose range is `valid` floats: lanes at or past it store nothing
+// (the hardware's range check is the column guard). The scalar base and a
+// 32-bit lane offset replace a 64-bit address add per store; aux 2 is the
+// non-temporal policy (gfx94x/gfx950: bit 0 sc0, bit 1 nt, bit 4 sc1), as
+// __builtin_nontemporal_store.
+__device__ __forceinline__ void row_store(float* base, u32 valid, u32 lane, float v) {
+    const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(base, 0, int(valid * 4u), 0x00020000);
+    __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v), rs, int(lane * 4u), 0, 2);
+}
+
+// The tile's values to the matrix (and, full layout, to the mirrored block),
+// with each wave's column maxima in part_col and row maxima in part_row.
+template <int LAYOUT, int RPW, bool WHOLE>
+__device__ __forceinline__ void tile_out(const float (&val)[RPW], u32 n, u32 bi, u32 bj,
+                                         float* __restrict__ mat, TileSmem<RPW>& sm, u32 ld) {
+    const u32 lane = threadIdx.x & 63;
+    const u32 wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const u32 j = bj * DTILE + lane;
+    const u32 ibase = bi * DTILE + wave * RPW;
+    float cmax = 0.0f;
+#pragma unroll
+    for (u32 r = 0; r < RPW; ++r) {
+        const u32 i = ibase + r;
+        const bool in = WHOLE || (i < n && j < n);
+        const float v = in ? val[r] : 0.0f;
+        sm.tile[wave * RPW + r][lane] = v;
+        cmax = fmax_acc(cmax, v);
+        if constexpr (LAYOUT == TT_LAYOUT_FULL) {
+            const u32 valid = WHOLE ? DTILE : (i < n ? min(DTILE, n - bj * DTILE) : 0u);
+            row_store(mat + u64(i) * ld + bj * DTILE, valid, lane, v);
+        } else {   // columns j < i (bj <= bi: i - 64 bj >= 0)
+            const u32 valid = (WHOLE || i < n) ? min(DTILE, i - bj * DTILE) : 0u;
+            row_store(mat + u64(i) * (i - 1) / 2 + bj * DTILE, valid, lane, v);
+        }
+    }
+    sm.part_col[wave][lane] = cmax;
+    __syncthreads();
+
+    const u32 i = bi * DTILE + lane;
+    float rmax = 0.0f;
+#pragma unroll
+    for (u32 c = 0; c < RPW; ++c) {
+        const u32 cl = wave * RPW + c;
+        const float v = sm.tile[lane][cl];
+        rmax = fmax_acc(rmax, v);
+        if constexpr (LAYOUT == TT_LAYOUT_FULL) {
+            const u32 jj = bj * DTILE + cl;
+            const u32 valid = (bi != bj && (WHOLE || jj < n)) ? (WHOLE ? DTILE : min(DTILE, n - bi * DTILE)) : 0u;
+            row_store(mat + u64(jj) * ld + bi * DTILE, valid, lane, v);
+        }
+    }
+    sm.part_row[wave][lane] = rmax;
+    __syncthreads();
+}
+
 // ld: row stride of the full layout in floats (n, or n rounded up for
 // 128-byte aligned rows); unused for the lower layout.
 template <int V, int DIM, int LAYOUT, int RPW>
@@ -56,38 +112,15 @@ __device__ __forceinline__ void dist_tile(const float* __restrict__ x, u32 n, u3
     if constexpr (DIM > 0) rows_static<V, DIM, RPW>(x, n, ma, ibase, jc, val);
     else rows_runtime<V, RPW>(x, n, dim_rt, ma, ibase, jc, val);
 
-    float cmax = 0.0f;
-#pragma unroll
-    for (u32 r = 0; r < RPW; ++r) {
-        const u32 i = ibase + r;
-        const bool in = i < n && j < n;
-        const float v = in ? val[r] : 0.0f;
-        sm.tile[wave * RPW + r][lane] = v;
-        cmax = fmax_ref(cmax, v);
-        if constexpr (LAYOUT == TT_LAYOUT_FULL) {
-            if (in) __builtin_nontemporal_store(v, mat + u64(i) * ld + j);
-        } else {
-            if (in && j < i) __builtin_nontemporal_store(v, mat + u64(i) * (i - 1) / 2 + j);
-        }
-    }
-    sm.part_col[wave][lane] = cmax;
-    __syncthreads();
+    // A tile with every row and column below n (bj <= bi, so (bi + 1) * 64 <= n
+    // suffices) stores without per-element guards; each store addresses a
+    // wave-uniform row base plus the lane, one SGPR pair and a 32-bit offset.
+    // The running maxima take fmax_acc: the values are distances (>= +0 or
+    // NaN) and 0 for cells outside the matrix.
+    if ((bi + 1) * DTILE <= n) tile_out<LAYOUT, RPW, true>(val, n, bi, bj, mat, sm, ld);
+    else tile_out<LAYOUT, RPW, false>(val, n, bi, bj, mat, sm, ld);
 
     const u32 i = bi * DTILE + lane;
-    float rmax = 0.0f;
-#pragma unroll
-    for (u32 c = 0; c < RPW; ++c) {
-        const u32 cl = wave * RPW + c;
-        const float v = sm.tile[lane][cl];
-        rmax = fmax_ref(rmax, v);
-        if constexpr (LAYOUT == TT_LAYOUT_FULL) {
-            const u32 jj = bj * DTILE + cl;
-            if (bi != bj && i < n && jj < n) __builtin_nontemporal_store(v, mat + u64(jj) * ld + i);
-        }
-    }
-    sm.part_row[wave][lane] = rmax;
-    __syncthreads();
-
     if (rowmax) {
         if (wave == 0) {
             float m = sm.part_col[0][lane];
