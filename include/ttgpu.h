@@ -280,7 +280,7 @@ tt_status tt_batch_wait(tt_batch* b, uint32_t cloud);
 /* Cloud results (library-owned, valid until the next run): the full
  * matrix, its threshold, and the kept edges that are not in an apparent
  * pair (idx, d), in filtration order — the union-find and the H1 columns
- * of dimension 0 need no others. */
+ * of dimension 0 need no others (order 0); in order 3 every kept edge. */
 tt_status tt_batch_cloud(tt_batch* b, uint32_t cloud, const float** mat, float* threshold,
                          uint64_t* m, const uint64_t** idx, const float** d);
 /* Which edges a run hands over. order = 0 (the default): the kept edges that
@@ -290,9 +290,17 @@ tt_status tt_batch_cloud(tt_batch* b, uint32_t cloud, const float** mat, float* 
  * index ascending; redzed.hpp:544-613), read with tt_batch_cloud_edges.
  * order = 2: as 1, plus each edge's ready-witness flag (as
  * tt_build_edges_witness) where the batch's keys have a bit to spare.
+ * order = 3: every kept edge in the coboundary engine's order with its
+ * apparent flag (as tt_dense_edges hands them over), read with
+ * tt_batch_cloud and tt_batch_cloud_apparent: the input of maxdim >= 2,
+ * whose next dimension is assembled from all edges (rips.hpp:981-1023);
+ * TT_ELENGTH where the keys have no bit to spare.
  * Takes effect at the next run (the recorded sequence is rebuilt). No
  * reference counterpart: the reference has no batch API. */
 tt_status tt_batch_set_order(tt_batch* b, int order);
+/* Order 3: the apparent flags (1 = in a zero-apparent pair) of the cloud's m
+ * edges that tt_batch_cloud returns, in the same order. */
+tt_status tt_batch_cloud_apparent(tt_batch* b, uint32_t cloud, const uint8_t** apparent);
 /* Cloud results in orders 1 and 2: the full matrix, its threshold, and m
  * edges as 2 m vertex ids (lo < hi per edge) with their diameters. When
  * *witness_bit is 1, bit 31 of each hi id is the edge's ready-witness flag,

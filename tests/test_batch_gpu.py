@@ -85,11 +85,40 @@ def test_batch_vs_reference(gpu, reference):
         assert canon(dgms) == canon(want)
 
 
+@pytest.mark.parametrize("maxdim", [0, 2, 3])
+def test_batch_other_maxdim(gpu, maxdim):
+    """maxdim 0 (union-find on the non-apparent edges) and maxdim 2, 3 (every
+    kept edge with its GPU apparent flag into the cohomology engine, order 3):
+    each cloud's diagrams equal persistence() on that cloud alone, with and
+    without the graph."""
+    from turbotops_amd import BatchPersistence, persistence
+    clouds = _clouds(15, count=12, nmax=160 if maxdim == 3 else 300)
+    print(f"case batch maxdim={maxdim} clouds={len(clouds)}", flush=True)
+    bp = BatchPersistence([c.shape for c in clouds], maxdim=maxdim)
+    for use_graph in (True, False):
+        res = bp(clouds, use_graph=use_graph)
+        for c, (dgms, info) in zip(clouds, res):
+            want, _ = persistence(c, maxdim=maxdim)
+            assert len(dgms) == maxdim + 1
+            assert canon(dgms) == canon(want)
+            assert info["engine"] == "coboundary"
+
+
+def test_batch_maxdim2_vs_reference(gpu, reference):
+    from turbotops_amd import BatchPersistence
+    clouds = _clouds(16, count=6, nmax=200)
+    res = BatchPersistence([c.shape for c in clouds], maxdim=2)(clouds)
+    for c, (dgms, _) in zip(clouds, res):
+        dims, b, e = reference.persistence(c, maxdim=2)[0]
+        want = [np.stack([b[dims == d], e[dims == d]], 1) for d in range(3)]
+        assert canon(dgms) == canon(want)
+
+
 def test_batch_rejects(gpu):
     from turbotops_amd import BatchPersistence
     clouds = _clouds(14, count=4, nmax=50)
-    with pytest.raises(ValueError, match="maxdim must be 1"):
-        BatchPersistence([c.shape for c in clouds], maxdim=2)
+    with pytest.raises(ValueError, match="maxdim must be <= 30"):
+        BatchPersistence([c.shape for c in clouds], maxdim=31)
     bp = BatchPersistence([c.shape for c in clouds])
     with pytest.raises(ValueError, match="do not match"):
         bp(clouds[:3])

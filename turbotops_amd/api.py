@@ -71,12 +71,14 @@ def persistence_pairs(x, maxdim=1, threshold=math.inf, metric="l2", graph="full"
 
 class BatchPersistence:
     """Many clouds of uneven size and dimension through one batched GPU front
-    end (dense full Rips at each cloud's enclosing radius, H0 and H1), with
-    host reductions in parallel. Create once per set of shapes; call with the
+    end (dense full Rips at each cloud's enclosing radius), with host
+    reductions in parallel. Create once per set of shapes; call with the
     clouds (a list of (n_i, d_i) arrays) for a list of (dgms, info), each equal
-    to persistence(cloud, maxdim=1, metric=metric). mode "auto" and
-    "boundary" reduce with the boundary engine, "coboundary" with the
-    cohomology engine (same diagrams). witness_flags (the default) adds the
+    to persistence(cloud, maxdim=maxdim, metric=metric). At maxdim 1, mode
+    "auto" and "boundary" reduce with the boundary engine, "coboundary" with
+    the cohomology engine (same diagrams); maxdim 0 and >= 2 run the
+    cohomology engine (maxdim >= 2 on every kept edge with its GPU apparent
+    flag). witness_flags (the default) adds the
     GPU's ready-witness flags to the boundary engine's inputs: one more device
     scan (about 0.12 s on 512 clouds) that lets the host reduction skip most
     edges of clouds without long cycles; witness_flags=False leaves it out."""

@@ -265,6 +265,60 @@ __global__ void batch_unpack(const u64* __restrict__ keys, const u32* __restrict
     }
 }
 
+// The apparent flag of every compacted coboundary-order key (cloud,
+// diameter bits, M - 1 - idx) as its new lowest bit: every kept edge goes on
+// in the cohomology engine's order with its flag (maxdim 0 or >= 2, where
+// the next dimension is assembled from all edges).
+__global__ __launch_bounds__(256) void batch_apparent_tag(BatchShape b, const float* __restrict__ mats,
+                                                          const float* __restrict__ bmins, u64* __restrict__ keys,
+                                                          const u32* __restrict__ total, int ib) {
+    const u32 lane = threadIdx.x & 63;
+    const u64 gw = (blockIdx.x * 256ull + threadIdx.x) >> 6;
+    const u64 nw = (u64(gridDim.x) * 256) >> 6;
+    const u64 m = *total;
+    const u64 imask = (u64(1) << ib) - 1;
+    for (u64 e0 = gw * 64; e0 < m; e0 += nw * 64) {
+        const u64 e = e0 + lane;
+        const bool valid = e < m;
+        u32 i = 1, j = 0, n = 2, nb = 1;
+        float d = 0.0f;
+        u64 key = 0;
+        const float* mat = mats;
+        const float* bm = bmins;
+        if (valid) {
+            key = keys[e];
+            const u32 c = u32(key >> (31 + ib));
+            d = __uint_as_float(u32(key >> ib) & 0x7fffffffu);
+            n = b.n[c];
+            nb = (n + BLK - 1) / BLK;
+            const u64 M = u64(n) * (n - 1) / 2;
+            decode_edge_b(M - 1 - (key & imask), i, j);
+            mat = mats + b.mat_off[c];
+            bm = bmins + b.bmin_off[c];
+        }
+        const bool ap = edges_apparent_lanes<TOPK>(mat, bm, n, row_ld(n), nb, valid, i, j, d, lane);
+        if (valid) keys[e] = (key << 1) | (ap ? 1u : 0u);
+    }
+}
+
+// Sorted tagged keys -> (idx, d, apparent flag) per kept edge.
+__global__ void batch_unpack_tagged(const u64* __restrict__ keys, const u32* __restrict__ total, int ib,
+                                    const u32* __restrict__ n_of, u64* __restrict__ out_idx,
+                                    float* __restrict__ out_d, u8* __restrict__ out_ap) {
+    const u64 m = *total;
+    const u64 imask = (u64(1) << ib) - 1;
+    for (u64 e = blockIdx.x * 256ull + threadIdx.x; e < m; e += u64(gridDim.x) * 256) {
+        const u64 raw = keys[e];
+        const u64 key = raw >> 1;
+        const u32 c = u32(key >> (31 + ib));
+        const u64 n = n_of[c];
+        const u64 M = n * (n - 1) / 2;
+        out_idx[e] = M - 1 - (key & imask);
+        out_d[e] = __uint_as_float(u32(key >> ib) & 0x7fffffffu);
+        out_ap[e] = u8(raw & 1);
+    }
+}
+
 // The boundary engine's ready-witness flag of every compacted comb-order key
 // (cloud, diameter bits, comb index), 64 edges per wave (apparent.hpp); the
 // flag becomes the key's lowest bit.
@@ -391,6 +445,7 @@ struct BatchState {
     bool comb = false;   // hand over every kept edge in (diameter, comb index) order
     bool witness = false;   // ... with the ready-witness flags (asked for, and the keys have a bit to spare)
     bool witness_fits = false;
+    bool tagged = false;   // every kept edge in the coboundary order with its apparent flag
     std::vector<u32> n, dim;
     std::vector<u64> pt_off, mat_off, dmat_off, row_base, tile_base, pair_base, bmin_off;
     u64 total_pts = 0, total_mat = 0, total_rows = 0, total_tiles = 0, capacity = 0, h_cap = 0;
@@ -401,6 +456,7 @@ struct BatchState {
     u32 *d_rowmax = nullptr, *d_cnt = nullptr, *d_total = nullptr, *d_seg = nullptr, *d_keep = nullptr,
         *d_pos = nullptr, *d_kept = nullptr, *d_kseg = nullptr, *d_hist = nullptr, *d_stmp = nullptr,
         *d_n = nullptr, *d_dim = nullptr;
+    u8* d_out_ap = nullptr;
     u64 *d_keys = nullptr, *d_alt = nullptr, *d_out_idx = nullptr, *d_pt_off = nullptr, *d_mat_off = nullptr,
         *d_row_base = nullptr, *d_tile_base = nullptr, *d_pair_base = nullptr, *d_bmin_off = nullptr;
     u32 *d_tile_cloud = nullptr, *d_row_cloud = nullptr;
@@ -410,6 +466,7 @@ struct BatchState {
     float* h_mats = nullptr;
     u64* h_idx = nullptr;
     float* h_d = nullptr;
+    u8* h_ap = nullptr;
     std::vector<u32> seg, kseg;
     std::vector<float> thr;
     hipStream_t stream = nullptr;
@@ -429,6 +486,7 @@ struct BatchState {
         if (h_mats) (void)hipHostFree(h_mats);
         if (h_idx) (void)hipHostFree(h_idx);
         if (h_d) (void)hipHostFree(h_d);
+        if (h_ap) (void)hipHostFree(h_ap);
     }
 
     // the device part of one batch (recorded into the graph)
@@ -456,6 +514,15 @@ struct BatchState {
             }
             radix_sort_u64_counted(d_keys, capacity, d_total, 0, cb + 31 + ib + fb, d_alt, d_hist, d_stmp, s);
             batch_unpack_comb<<<ge, 256, 0, s>>>(d_keys, d_total, ib, fb, d_out_idx, d_out_d);
+            copy_u32<<<1, 256, 0, s>>>(d_seg, nclouds + 1, d_kseg);
+            TTG_LAUNCH_CHECK();
+            return;
+        }
+        if (tagged) {
+            batch_block_min<<<gw, 256, 0, s>>>(shape, d_mats, d_bmin);
+            batch_apparent_tag<<<ge, 256, 0, s>>>(shape, d_mats, d_bmin, d_keys, d_total, ib);
+            radix_sort_u64_counted(d_keys, capacity, d_total, 0, cb + 31 + ib + 1, d_alt, d_hist, d_stmp, s);
+            batch_unpack_tagged<<<ge, 256, 0, s>>>(d_keys, d_total, ib, d_n, d_out_idx, d_out_d, d_out_ap);
             copy_u32<<<1, 256, 0, s>>>(d_seg, nclouds + 1, d_kseg);
             TTG_LAUNCH_CHECK();
             return;
@@ -678,20 +745,24 @@ void batch_run(BatchState& b, const float* pts, int use_graph) {
     u32 kept = 0;
     TTG_HIP(hipMemcpyAsync(b.kseg.data(), b.d_kseg, (b.nclouds + 1) * 4, hipMemcpyDeviceToHost, s));
     TTG_HIP(hipMemcpyAsync(b.thr.data(), b.d_thr, b.nclouds * 4, hipMemcpyDeviceToHost, s));
-    TTG_HIP(hipMemcpyAsync(&kept, b.comb ? b.d_total : b.d_kept, 4, hipMemcpyDeviceToHost, s));
+    TTG_HIP(hipMemcpyAsync(&kept, (b.comb || b.tagged) ? b.d_total : b.d_kept, 4, hipMemcpyDeviceToHost, s));
     TTG_HIP(hipStreamSynchronize(s));
     TTG_HIP(hipEventElapsedTime(&b.last_ms, b.t0, b.t1));
     b.kseg[b.nclouds] = kept;
     if (!b.h_idx || b.h_cap < kept) {
         if (b.h_idx) (void)hipHostFree(b.h_idx);
         if (b.h_d) (void)hipHostFree(b.h_d);
+        if (b.h_ap) (void)hipHostFree(b.h_ap);
         b.h_idx = nullptr;
         b.h_d = nullptr;
+        b.h_ap = nullptr;
         const u64 cap = std::max<u64>(kept, 1) + (kept >> 3);
         TTG_HIP(hipHostMalloc(reinterpret_cast<void**>(&b.h_idx), cap * 8, hipHostMallocDefault));
         TTG_HIP(hipHostMalloc(reinterpret_cast<void**>(&b.h_d), cap * 4, hipHostMallocDefault));
         b.h_cap = cap;
     }
+    if (b.tagged && !b.h_ap)
+        TTG_HIP(hipHostMalloc(reinterpret_cast<void**>(&b.h_ap), b.h_cap, hipHostMallocDefault));
     // per cloud: matrix + its kept edges, then an event the host waits on
     for (u32 c = 0; c < b.nclouds; ++c) {
         const u64 nc = b.n[c];
@@ -702,6 +773,8 @@ void batch_run(BatchState& b, const float* pts, int use_graph) {
         if (e1 > e0) {
             TTG_HIP(hipMemcpyAsync(b.h_idx + e0, b.d_out_idx + e0, (e1 - e0) * 8, hipMemcpyDeviceToHost, s));
             TTG_HIP(hipMemcpyAsync(b.h_d + e0, b.d_out_d + e0, (e1 - e0) * 4, hipMemcpyDeviceToHost, s));
+            if (b.tagged)
+                TTG_HIP(hipMemcpyAsync(b.h_ap + e0, b.d_out_ap + e0, e1 - e0, hipMemcpyDeviceToHost, s));
         }
         TTG_HIP(hipEventRecord(b.ev[c], s));
     }
@@ -757,15 +830,29 @@ tt_status tt_batch_cloud(tt_batch* b, uint32_t cloud, const float** mat, float* 
     });
 }
 
+tt_status tt_batch_cloud_apparent(tt_batch* b, uint32_t cloud, const uint8_t** apparent) {
+    return guarded([&] {
+        if (!b || cloud >= b->st.nclouds) fail(TT_EINVAL, "tt_batch_cloud_apparent: bad cloud");
+        BatchState& s = b->st;
+        if (!s.tagged) fail(TT_EINVAL, "tt_batch_cloud_apparent: the batch hands over no apparent flags (order 3)");
+        if (apparent) *apparent = s.h_ap + s.kseg[cloud];
+    });
+}
+
 tt_status tt_batch_set_order(tt_batch* b, int order) {
     return guarded([&] {
         if (!b) fail(TT_EINVAL, "tt_batch_set_order: null handle");
-        if (order < 0 || order > 2) fail(TT_EINVAL, "tt_batch_set_order: order must be 0, 1 or 2");
+        if (order < 0 || order > 3) fail(TT_EINVAL, "tt_batch_set_order: order must be 0, 1, 2 or 3");
         BatchState& s = b->st;
-        const bool comb = order >= 1, witness = order == 2 && s.witness_fits;
-        if (s.comb == comb && s.witness == witness) return;
+        const bool comb = order == 1 || order == 2, witness = order == 2 && s.witness_fits;
+        const bool tagged = order == 3;
+        if (tagged && !s.witness_fits)
+            fail(TT_ELENGTH, "tt_batch_set_order: edge keys with an apparent flag exceed 64 bits");
+        if (s.comb == comb && s.witness == witness && s.tagged == tagged) return;
+        if (tagged && !s.d_out_ap) s.d_out_ap = s.ws.get<u8>(s.capacity);
         s.comb = comb;
         s.witness = witness;
+        s.tagged = tagged;
         // the recorded sequence belongs to the other order
         if (s.exec) TTG_HIP(hipGraphExecDestroy(s.exec));
         if (s.graph) TTG_HIP(hipGraphDestroy(s.graph));

@@ -476,13 +476,23 @@ BatchRunner::BatchRunner(const std::vector<u32>& n, const std::vector<u32>& dim,
                          u32 maxdim, Mode mode, bool witness_flags)
     : n_(n), dim_(dim), maxdim_(maxdim) {
     if (n.size() != dim.size()) throw std::invalid_argument("persistence_batch: n and dim differ in length");
-    if (maxdim != 1) throw std::invalid_argument("persistence_batch: maxdim must be 1 (H0 and H1)");
+    if (maxdim + 2 > 32)
+        throw std::invalid_argument(mode == Mode::Boundary ? "redzed: maxdim must be <= 30"
+                                                           : "rips: maxdim must be <= 30");
     const Metric m = Metric::parse(metric);
     u32 nmax = 0;
     for (u32 v : n) nmax = std::max(nmax, v);
+    (void)select_mode(mode, nmax, maxdim, INFINITY);   // the single-cloud checks and errors
+    // maxdim 1: the boundary engine (auto, boundary) or the cohomology engine
+    // on the non-apparent edges; maxdim 0: the union-find on the non-apparent
+    // edges (an apparent edge never joins two components); maxdim >= 2: the
+    // cohomology engine on every kept edge with its apparent flag, since the
+    // next dimension is assembled from all of them.
     boundary_ = boundary_engine(mode, nmax, maxdim, true);
+    tagged_ = !boundary_ && maxdim >= 2;
     check(tt_batch_create(u32(n.size()), n.data(), dim.data(), m.kind, m.p, INFINITY, &h_));
     if (boundary_) check(tt_batch_set_order(h_, witness_flags ? 2 : 1));
+    if (tagged_) check(tt_batch_set_order(h_, 3));
 }
 
 BatchRunner::~BatchRunner() {
@@ -602,13 +612,15 @@ std::vector<Result> BatchRunner::run(const f32* pts, bool use_graph, u32 threads
                     r.stats.engine = "boundary";
                 } else {
                     const u64* idx = nullptr;
+                    const u8* ap = nullptr;
                     check(tt_batch_cloud(h_, c, &mat, &thr, &m, &idx, &d));
+                    if (tagged_) check(tt_batch_cloud_apparent(h_, c, &ap));
                     cob::DenseView cx;
                     cx.n = n_[c];
                     cx.threshold = thr;
                     cx.full = true;
                     cx.mat = mat;
-                    reduce_all(cx, maxdim_, idx, d, nullptr, m, r.barcode, r.stats);
+                    reduce_all(cx, maxdim_, idx, d, ap, m, r.barcode, r.stats);
                     r.stats.engine = "coboundary";
                 }
                 r.barcode.ensure(maxdim_);

@@ -179,9 +179,11 @@ CriticalEdges critical_edges(const Barcode& bc, u32 n, const std::function<f32(u
 // cloud alone.
 class BatchRunner {
   public:
-    // mode: Auto and Boundary reduce with the boundary engine (every kept
-    // edge in (diameter, comb index) order), Coboundary with the cohomology
-    // engine (the non-apparent edges in its order). witness_flags: the
+    // mode: at maxdim 1, Auto and Boundary reduce with the boundary engine
+    // (every kept edge in (diameter, comb index) order), Coboundary with the
+    // cohomology engine (the non-apparent edges in its order); maxdim 0 and
+    // >= 2 run the cohomology engine (maxdim >= 2 on every kept edge with its
+    // apparent flag). witness_flags: the
     // boundary engine's inputs carry the GPU's ready-witness flags (faster
     // host reductions on clouds without long cycles, one more device scan;
     // used where the flag bit fits the batch's edge keys).
@@ -209,6 +211,7 @@ class BatchRunner {
     std::vector<u32> n_, dim_;
     u32 maxdim_ = 1;
     bool boundary_ = true;
+    bool tagged_ = false;   // maxdim >= 2: every kept edge with its apparent flag (order 3)
     float device_ms_ = 0;
     int rpw_ = 0;
     u64 edges_total_ = 0, edges_kept_ = 0;
