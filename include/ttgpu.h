@@ -320,6 +320,37 @@ tt_status tt_batch_info(tt_batch* b, float* device_ms, int* rows_per_wave, uint6
 tt_status tt_batch_time_dist(tt_batch* b, float* ms);
 void tt_batch_destroy(tt_batch* b);
 
+/* ---- batched kNN graphs ------------------------------------------------------ */
+
+/* The kNN stage of the batch: for every cloud, its exact k-nearest-neighbour
+ * lists (knn_lists_exact, include/turbotops/graph.hpp:211-225: k_c =
+ * min(k, n_c - 1), (distance, id) lexicographic, self excluded) and their
+ * symmetric union as a CSR graph, rows sorted by neighbour id, weights
+ * recomputed as c(row, col) (knn_graph, graph.hpp:286-304), in one launch
+ * sequence recorded into a HIP graph on the first run. Each cloud's graph
+ * equals tt_knn_graph_exact on that cloud alone, bit for bit. 1 <= k <= 64
+ * (tt_knn_graph_exact takes any k). No reference counterpart: the reference
+ * has no batch API; its per-cloud path is persistence(..., graph = knn)
+ * (turbotops.hpp:275-277). */
+typedef struct tt_batch_knn tt_batch_knn;
+tt_status tt_batch_knn_create(uint32_t nclouds, const uint32_t* n, const uint32_t* dim, int kind,
+                              float p, uint32_t k, tt_batch_knn** out);
+/* pts: every cloud's points, row-major, concatenated in order. Returns with
+ * every cloud's graph in host memory. */
+tt_status tt_batch_knn_run(tt_batch_knn* b, const float* pts, int use_graph);
+/* Cloud results (library-owned, valid until the next run): n_c + 1 row
+ * offsets into col / w, and col / w themselves; the offsets are batch-wide,
+ * so cloud c's rows run from col + row_off[0] to col + row_off[n_c], with
+ * cloud-local neighbour ids. */
+tt_status tt_batch_knn_cloud(tt_batch_knn* b, uint32_t cloud, const uint32_t** row_off,
+                             const uint32_t** col, const float** w);
+/* Cloud c's lists, n_c x k_c ids, each row by (distance, id); copied from
+ * the device per call (valid until the next call). */
+tt_status tt_batch_knn_lists(tt_batch_knn* b, uint32_t cloud, uint32_t* k, const uint32_t** ids);
+/* Device time of the last run and the batch's total CSR entries. */
+tt_status tt_batch_knn_info(tt_batch_knn* b, float* device_ms, uint64_t* nnz);
+void tt_batch_knn_destroy(tt_batch_knn* b);
+
 /* ---- differentiation ------------------------------------------------------- */
 
 /* Backward of a diagram with respect to the points. The reference computes

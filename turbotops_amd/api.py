@@ -81,13 +81,25 @@ class BatchPersistence:
     flag). witness_flags (the default) adds the
     GPU's ready-witness flags to the boundary engine's inputs: one more device
     scan (about 0.12 s on 512 clouds) that lets the host reduction skip most
-    edges of clouds without long cycles; witness_flags=False leaves it out."""
+    edges of clouds without long cycles; witness_flags=False leaves it out.
 
-    def __init__(self, shapes, metric="l2", maxdim=1, mode="auto", witness_flags=True):
+    graph="knn": each cloud's flag complex of its symmetric exact k-NN graph
+    instead (the kNN stage of the batch: every cloud's lists and graph in one
+    device sequence, 1 <= k <= 64), each result equal to
+    persistence(cloud, maxdim=maxdim, metric=metric, graph="knn", k=k);
+    graph(c) and lists(c) give cloud c's graph and lists after a call."""
+
+    def __init__(self, shapes, metric="l2", maxdim=1, mode="auto", witness_flags=True, graph="full", k=15):
         ext = _ext()
         self.shapes = [(int(n), int(d)) for n, d in shapes]
-        self._r = ext.BatchRunner([n for n, _ in self.shapes], [d for _, d in self.shapes], str(metric),
-                                  int(maxdim), str(mode), bool(witness_flags))
+        ns, ds = [n for n, _ in self.shapes], [d for _, d in self.shapes]
+        if graph not in ("full", "knn"):
+            raise ValueError(f"graph must be 'full' or 'knn', not {graph!r}")
+        self.graph_kind = graph
+        if graph == "knn":
+            self._r = ext.BatchKnnRunner(ns, ds, str(metric), int(k), int(maxdim), str(mode))
+        else:
+            self._r = ext.BatchRunner(ns, ds, str(metric), int(maxdim), str(mode), bool(witness_flags))
 
     def __call__(self, clouds, use_graph=True, threads=0, pairs=False):
         """With pairs=True each entry is (dgms, birth_edges, death_edges, info)
@@ -113,6 +125,22 @@ class BatchPersistence:
         """The distance stage alone, re-run on the last points (ms)."""
         return self._r.time_distance_stage()
 
+    def graph(self, cloud):
+        """graph="knn": cloud `cloud`'s symmetric kNN graph of the last call as
+        a CsrGraph."""
+        from .frontend import CsrGraph
+        rp, col, w = self._r.graph(int(cloud))
+        return CsrGraph(self.shapes[cloud][0], rp, col, w)
+
+    def lists(self, cloud):
+        """graph="knn": cloud `cloud`'s (n, min(k, n - 1)) neighbour lists."""
+        return self._r.lists(int(cloud))
+
+    @property
+    def nnz(self):
+        """graph="knn": CSR entries of every cloud's graph in the last call."""
+        return self._r.nnz
+
     @property
     def edges(self):
         """(kept edges, kept edges not in an apparent pair) of the last run."""
@@ -123,7 +151,8 @@ class BatchPersistence:
         return self._r.rows_per_wave
 
 
-def persistence_batch(clouds, maxdim=1, metric="l2", threads=0, use_graph=True):
+def persistence_batch(clouds, maxdim=1, metric="l2", threads=0, use_graph=True, graph="full", k=15):
     """persistence() of every cloud in `clouds` through one batched launch."""
     shapes = [np.shape(c) for c in clouds]
-    return BatchPersistence(shapes, metric=metric, maxdim=maxdim)(clouds, use_graph=use_graph, threads=threads)
+    bp = BatchPersistence(shapes, metric=metric, maxdim=maxdim, graph=graph, k=k)
+    return bp(clouds, use_graph=use_graph, threads=threads)

@@ -12,10 +12,9 @@
 //                 ballot reserves room in an edge list, mbcnt places each edge.
 //                 The CSR comes from a radix sort of (row, col) keys carrying the
 //                 weight bits: rows sorted by neighbour id, as from_rows.
-//   kNN           one query per lane; candidate points stream through LDS in
-//                 tiles of 256; the k best (distance, id) pairs sit in VGPRs as
-//                 a sorted insertion network. Candidates arrive in ascending id,
-//                 so "strictly closer than the current k-th" is the reference's
+//   kNN           one query per lane (knn.hpp); candidates stream through
+//                 scalar loads; the k best (distance, id) pairs sit in VGPRs as
+//                 a sorted insertion network, admitting by the reference's
 //                 (distance, id) lexicographic rule (graph.hpp:112, :145).
 //   kNN, k > 64   one workgroup per query: a radix select over the 32-bit
 //                 distance bits (four 8-bit passes, recomputing the distances)
@@ -28,8 +27,9 @@
 //                 c(row, col) (graph.hpp:295-302).
 #include <algorithm>
 
-#include "pairs.hpp"
 #include "frontend.hpp"
+#include "knn.hpp"
+#include "pairs.hpp"
 
 namespace ttg {
 
@@ -127,90 +127,28 @@ __global__ void widen_u32(const u32* __restrict__ in, u64* __restrict__ out, u64
 }
 
 // ---- kNN --------------------------------------------------------------------
-constexpr u32 KT = 256;   // candidate tile (points)
-
-template <int V, int DIM, int KMAX>
-__global__ __launch_bounds__(GB) void knn_scan(const float* __restrict__ x, u32 n, u32 dim_rt,
-                                               MetricArgs ma, u32 k, u32* __restrict__ ids) {
-    constexpr u32 DS = DIM > 0 ? DIM : 1;
-    __shared__ float tile[DIM > 0 ? KT * DS : 1];
-    const u32 dim = DIM > 0 ? u32(DIM) : dim_rt;
+// Runtime dimension: one query per lane over every candidate (knn.hpp).
+template <int V, int KMAX>
+__global__ __launch_bounds__(GB) void knn_scan(const float* __restrict__ x, u32 n, u32 dim, MetricArgs ma, u32 k,
+                                               u32* __restrict__ ids) {
     const u32 i = blockIdx.x * GB + threadIdx.x;
     const u32 qi = i < n ? i : n - 1;
-    float q[DS];
-    const float* __restrict__ qp = x + u64(qi) * dim;
-    if constexpr (DIM > 0) {
-#pragma unroll
-        for (int t = 0; t < DIM; ++t) q[t] = qp[t];
-    }
-    float bd[KMAX];
-    u32 bid[KMAX];
-#pragma unroll
-    for (int t = 0; t < KMAX; ++t) {
-        bd[t] = u32(t) < k ? INFINITY : -INFINITY;   // slots past k never change
-        bid[t] = 0xffffffffu;
-    }
-    float kth = INFINITY;       // current k-th (distance, id); id ~0 while slots are free
-    u32 kthj = 0xffffffffu;
-    for (u32 j0 = 0; j0 < n; j0 += KT) {
-        const u32 len = min(KT, n - j0);
-        if constexpr (DIM > 0) {
-            __syncthreads();
-            for (u32 t = threadIdx.x; t < len * DS; t += GB) tile[t] = x[u64(j0) * DS + t];
-            __syncthreads();
-        }
-        for (u32 jj = 0; jj < len; ++jj) {
-            const u32 j = j0 + jj;
-            float d;
-            if constexpr (DIM > 0) d = pdist<V, DIM>(q, tile + jj * DS, dim, ma);
-            else d = pdist<V, 0>(qp, x + u64(j) * dim, dim, ma);
-            if ((d < kth || (d == kth && j < kthj)) && j != qi) {
-                float cd = d;
-                u32 cj = j;
-#pragma unroll
-                for (int t = 0; t < KMAX; ++t) {
-                    const bool sw = cd < bd[t] || (cd == bd[t] && cj < bid[t]);
-                    const float td = bd[t];
-                    const u32 tj = bid[t];
-                    bd[t] = sw ? cd : td;
-                    bid[t] = sw ? cj : tj;
-                    cd = sw ? td : cd;
-                    cj = sw ? tj : cj;
-                }
-                float nk = bd[0];
-                u32 nj = bid[0];
-#pragma unroll
-                for (int t = 1; t < KMAX; ++t)
-                    if (u32(t) == k - 1) {
-                        nk = bd[t];
-                        nj = bid[t];
-                    }
-                kth = nk;
-                kthj = nj;
-            }
-        }
-    }
+    KnnList<KMAX> L;
+    L.init(k);
+    knn_scan_runtime<V, KMAX>(x, dim, 0, n, qi, ma, k, L);
     if (i < n) {
 #pragma unroll
         for (int t = 0; t < KMAX; ++t)
-            if (u32(t) < k) ids[u64(i) * k + t] = bid[t];
+            if (u32(t) < k) ids[u64(i) * k + t] = L.bid[t];
     }
 }
 
-// Compile-time dimension: the candidate is the same for every lane, so its
-// coordinates come through scalar loads (the points are read-only here) and
-// feed the VALU as SGPR operands; no LDS staging, no barriers. The next
-// candidate is loaded while the current one is measured. The ordering key
-// (pdist<KEY>: for L2 the sum before the square root) rejects most
-// candidates without the exact square root: only keys at or below
-// key_reject_above(k-th distance) are turned into distances and compared by
-// the reference's (distance, id) rule.
-//
-// The candidates are split into gridDim.y chunks (blockIdx.y = chunk), so
-// that a few thousand queries still fill the machine. With one chunk the ids
-// are written directly; otherwise each chunk's sorted (distance, id) list goes
-// to pd / pi and knn_merge combines them: the k best of the union are the k
-// best of the per-chunk k best.
+// Compile-time dimension (knn_scan_static: scalar candidate loads, key
+// rejection). The candidates are split into gridDim.y chunks (blockIdx.y =
+// chunk), so that a few thousand queries still fill the machine. With one
+// chunk the ids are written directly; otherwise each chunk's sorted
+// (distance, id) list goes to pd / pi and knn_merge combines them: the k best
+// of the union are the k best of the per-chunk k best.
 template <int V, int DIM, int KMAX>
 __global__ __launch_bounds__(GB) void knn_scan_s(const float* __restrict__ x, u32 n, MetricArgs ma, u32 k,
                                                  u32* __restrict__ ids, float* __restrict__ pd,
@@ -222,68 +160,21 @@ __global__ __launch_bounds__(GB) void knn_scan_s(const float* __restrict__ x, u3
     float q[DIM];
 #pragma unroll
     for (int t = 0; t < DIM; ++t) q[t] = x[u64(qi) * DIM + t];
-    float bd[KMAX];
-    u32 bid[KMAX];
-#pragma unroll
-    for (int t = 0; t < KMAX; ++t) {
-        bd[t] = u32(t) < k ? INFINITY : -INFINITY;   // slots past k never change
-        bid[t] = 0xffffffffu;
-    }
-    float kth = INFINITY;
-    u32 kthj = 0xffffffffu;
-    float reject = INFINITY;   // keys above this cannot enter
-    float c[DIM];
-#pragma unroll
-    for (int t = 0; t < DIM; ++t) c[t] = x[u64(min(jlo, n - 1)) * DIM + t];
-    for (u32 j = jlo; j < jhi; ++j) {
-        float cur[DIM];
-#pragma unroll
-        for (int t = 0; t < DIM; ++t) cur[t] = c[t];
-        const u32 jn = j + 1 < jhi ? j + 1 : j;
-#pragma unroll
-        for (int t = 0; t < DIM; ++t) c[t] = x[u64(jn) * DIM + t];
-        const float key = pdist<V, DIM, true>(cur, q, DIM, ma);
-        if (key <= reject && j != qi) {
-            const float d = key_to_dist<V>(key);
-            if (d < kth || (d == kth && j < kthj)) {
-                float cd = d;
-                u32 cj = j;
-#pragma unroll
-                for (int t = 0; t < KMAX; ++t) {
-                    const bool sw = cd < bd[t] || (cd == bd[t] && cj < bid[t]);
-                    const float td = bd[t];
-                    const u32 tj = bid[t];
-                    bd[t] = sw ? cd : td;
-                    bid[t] = sw ? cj : tj;
-                    cd = sw ? td : cd;
-                    cj = sw ? tj : cj;
-                }
-                float nk = bd[0];
-                u32 nj = bid[0];
-#pragma unroll
-                for (int t = 1; t < KMAX; ++t)
-                    if (u32(t) == k - 1) {
-                        nk = bd[t];
-                        nj = bid[t];
-                    }
-                kth = nk;
-                kthj = nj;
-                reject = key_reject_above<V>(kth);
-            }
-        }
-    }
+    KnnList<KMAX> L;
+    L.init(k);
+    knn_scan_static<V, DIM, KMAX>(x, jlo, jhi, qi, q, ma, k, L);
     if (i < n) {
         if (gridDim.y == 1) {
 #pragma unroll
             for (int t = 0; t < KMAX; ++t)
-                if (u32(t) < k) ids[u64(i) * k + t] = bid[t];
+                if (u32(t) < k) ids[u64(i) * k + t] = L.bid[t];
         } else {
             const u64 base = (u64(blockIdx.y) * n + i) * k;
 #pragma unroll
             for (int t = 0; t < KMAX; ++t)
                 if (u32(t) < k) {
-                    pd[base + t] = bd[t];
-                    pi[base + t] = bid[t];
+                    pd[base + t] = L.bd[t];
+                    pi[base + t] = L.bid[t];
                 }
         }
     }
@@ -470,7 +361,7 @@ void knn_one(u32 blocks, hipStream_t s, const float* x, u32 n, u32 dim, MetricAr
         knn_scan_s<V, DIM, KMAX><<<dim3(blocks, kc.chunks), GB, 0, s>>>(x, n, ma, k, ids, kc.pd, kc.pi);
         if (kc.chunks > 1) knn_merge<<<blocks, GB, 0, s>>>(kc.pd, kc.pi, n, k, kc.chunks, ids);
     } else {
-        knn_scan<V, DIM, KMAX><<<blocks, GB, 0, s>>>(x, n, dim, ma, k, ids);
+        knn_scan<V, KMAX><<<blocks, GB, 0, s>>>(x, n, dim, ma, k, ids);
     }
 }
 

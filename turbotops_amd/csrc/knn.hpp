@@ -1,0 +1,108 @@
+// SPDX-License-Identifier: MIT
+// knn.hpp — one query's exact k nearest neighbours, one query per lane,
+// shared by the single-cloud scan (graph.hip) and the batched one
+// (batch_knn.hip). The reference's rule (knn_query, graph.hpp:123-166): the
+// k smallest (distance, id) pairs in lexicographic order over j != query,
+// k = min(k, n - 1) (graph.hpp:214).
+#pragma once
+
+#include "pairs.hpp"
+
+namespace ttg {
+
+// The k best (distance, id) pairs so far, sorted, in VGPRs: an insertion
+// network over KMAX slots (slots past k hold -inf and never change).
+// Candidates may arrive in any id order: admission compares (distance, id)
+// with the current k-th, as the reference's lexicographic rule does.
+template <int KMAX>
+struct KnnList {
+    float bd[KMAX];
+    u32 bid[KMAX];
+    float kth;    // the current k-th (distance, id); id ~0 while slots are free
+    u32 kthj;
+
+    __device__ __forceinline__ void init(u32 k) {
+#pragma unroll
+        for (int t = 0; t < KMAX; ++t) {
+            bd[t] = u32(t) < k ? INFINITY : -INFINITY;
+            bid[t] = 0xffffffffu;
+        }
+        kth = INFINITY;
+        kthj = 0xffffffffu;
+    }
+    __device__ __forceinline__ bool admits(float d, u32 j) const { return d < kth || (d == kth && j < kthj); }
+    __device__ __forceinline__ void insert(float d, u32 j, u32 k) {
+        float cd = d;
+        u32 cj = j;
+#pragma unroll
+        for (int t = 0; t < KMAX; ++t) {
+            const bool sw = cd < bd[t] || (cd == bd[t] && cj < bid[t]);
+            const float td = bd[t];
+            const u32 tj = bid[t];
+            bd[t] = sw ? cd : td;
+            bid[t] = sw ? cj : tj;
+            cd = sw ? td : cd;
+            cj = sw ? tj : cj;
+        }
+        float nk = bd[0];
+        u32 nj = bid[0];
+#pragma unroll
+        for (int t = 1; t < KMAX; ++t)
+            if (u32(t) == k - 1) {
+                nk = bd[t];
+                nj = bid[t];
+            }
+        kth = nk;
+        kthj = nj;
+    }
+};
+
+// Candidates j in [jlo, jhi) of the points x (row-major, DIM floats) against
+// the query q (its own index qi is skipped), compile-time dimension. The
+// candidate is the same for every lane, so its coordinates come through
+// scalar loads and feed the VALU as SGPR operands; the next candidate is
+// loaded while the current one is measured. The ordering key (pdist<KEY>:
+// for L2 the sum before the square root) rejects most candidates without the
+// exact square root: only keys at or below key_reject_above(k-th distance)
+// become distances and meet the (distance, id) rule.
+template <int V, int DIM, int KMAX>
+__device__ __forceinline__ void knn_scan_static(const float* __restrict__ x, u32 jlo, u32 jhi, u32 qi,
+                                                const float (&q)[DIM], const MetricArgs& ma, u32 k,
+                                                KnnList<KMAX>& L) {
+    if (jlo >= jhi) return;
+    float reject = key_reject_above<V>(L.kth);   // keys above this cannot enter
+    float c[DIM];
+#pragma unroll
+    for (int t = 0; t < DIM; ++t) c[t] = x[u64(jlo) * DIM + t];
+    for (u32 j = jlo; j < jhi; ++j) {
+        float cur[DIM];
+#pragma unroll
+        for (int t = 0; t < DIM; ++t) cur[t] = c[t];
+        const u32 jn = j + 1 < jhi ? j + 1 : j;
+#pragma unroll
+        for (int t = 0; t < DIM; ++t) c[t] = x[u64(jn) * DIM + t];
+        const float key = pdist<V, DIM, true>(cur, q, DIM, ma);
+        if (key <= reject && j != qi) {
+            const float d = key_to_dist<V>(key);
+            if (L.admits(d, j)) {
+                L.insert(d, j, k);
+                reject = key_reject_above<V>(L.kth);
+            }
+        }
+    }
+}
+
+// The same for a runtime dimension: query and candidate read from memory
+// (the query's row stays in the L1 cache; the candidate is one address per
+// wave).
+template <int V, int KMAX>
+__device__ __forceinline__ void knn_scan_runtime(const float* __restrict__ x, u32 dim, u32 jlo, u32 jhi, u32 qi,
+                                                 const MetricArgs& ma, u32 k, KnnList<KMAX>& L) {
+    const float* __restrict__ qp = x + u64(qi) * dim;
+    for (u32 j = jlo; j < jhi; ++j) {
+        const float d = pdist<V, 0>(qp, x + u64(j) * dim, dim, ma);
+        if (j != qi && L.admits(d, j)) L.insert(d, j, k);
+    }
+}
+
+}  // namespace ttg

@@ -648,4 +648,95 @@ std::vector<Result> BatchRunner::run(const f32* pts, bool use_graph, u32 threads
     return out;
 }
 
+// ---- batched kNN graphs --------------------------------------------------------
+BatchKnnRunner::BatchKnnRunner(const std::vector<u32>& n, const std::vector<u32>& dim, const std::string& metric,
+                               u32 k, u32 maxdim, Mode mode)
+    : n_(n), dim_(dim), maxdim_(maxdim), mode_(mode) {
+    if (n.size() != dim.size()) throw std::invalid_argument("persistence_batch: n and dim differ in length");
+    const Metric m = Metric::parse(metric);
+    check(tt_batch_knn_create(u32(n.size()), n.data(), dim.data(), m.kind, m.p, k, &h_));
+}
+
+BatchKnnRunner::~BatchKnnRunner() {
+    if (h_) tt_batch_knn_destroy(h_);
+}
+
+u64 BatchKnnRunner::total_points() const {
+    u64 t = 0;
+    for (size_t c = 0; c < n_.size(); ++c) t += u64(n_[c]) * dim_[c];
+    return t;
+}
+
+void BatchKnnRunner::front(const f32* pts, bool use_graph) {
+    check(tt_batch_knn_run(h_, pts, use_graph ? 1 : 0));
+    check(tt_batch_knn_info(h_, &device_ms_, &nnz_));
+}
+
+CsrGraph BatchKnnRunner::graph(u32 cloud) const {
+    if (cloud >= n_.size()) throw std::invalid_argument("persistence_batch: bad cloud");
+    const u32* ro = nullptr;
+    const u32* col = nullptr;
+    const f32* w = nullptr;
+    check(tt_batch_knn_cloud(h_, cloud, &ro, &col, &w));
+    CsrGraph g;
+    g.n = n_[cloud];
+    g.row_ptr.resize(u64(g.n) + 1);
+    for (u32 v = 0; v <= g.n; ++v) g.row_ptr[v] = u64(ro[v] - ro[0]);
+    g.col_idx.assign(col + ro[0], col + ro[g.n]);
+    g.weights.assign(w + ro[0], w + ro[g.n]);
+    g.m = g.col_idx.size() / 2;
+    return g;
+}
+
+KnnLists BatchKnnRunner::lists(u32 cloud) {
+    if (cloud >= n_.size()) throw std::invalid_argument("persistence_batch: bad cloud");
+    KnnLists L;
+    const u32* ids = nullptr;
+    check(tt_batch_knn_lists(h_, cloud, &L.k, &ids));
+    L.n = n_[cloud];
+    L.ids.assign(ids, ids + u64(L.n) * L.k);
+    return L;
+}
+
+std::vector<Result> BatchKnnRunner::run(const f32* pts, bool use_graph, u32 threads,
+                                        std::vector<CriticalEdges>* edges) {
+    const auto t0 = std::chrono::steady_clock::now();
+    front(pts, use_graph);
+    const double sg = since(t0);
+    const u32 nc = u32(n_.size());
+    std::vector<Result> out(nc);
+    if (edges) edges->assign(nc, {});
+    std::vector<u32> order(nc);
+    for (u32 c = 0; c < nc; ++c) order[c] = c;
+    std::stable_sort(order.begin(), order.end(), [&](u32 a, u32 b) { return n_[a] > n_[b]; });
+    std::atomic<u32> next{0};
+    std::vector<std::string> errors(nc);
+    auto worker = [&] {
+        for (u32 k; (k = next.fetch_add(1)) < nc;) {
+            const u32 c = order[k];
+            try {
+                const CsrGraph g = graph(c);
+                Result& r = out[c];
+                r = persistence(g, maxdim_, INFINITY, 0, mode_);
+                r.barcode.ensure(maxdim_);
+                r.stats.method = "sparse-batch";
+                r.stats.seconds_graph = sg;
+                if (edges) (*edges)[c] = critical_edges(r.barcode, n_[c], [&](u32 a, u32 b) { return g.weight(a, b); });
+            } catch (const std::exception& e) {
+                errors[c] = e.what();
+            }
+        }
+    };
+    const u32 nt = std::max<u32>(1, std::min<u32>(threads ? threads : num_threads(), nc));
+    std::vector<std::thread> pool;
+    for (u32 t = 1; t < nt; ++t) pool.emplace_back(worker);
+    worker();
+    for (auto& th : pool) th.join();
+    for (u32 c = 0; c < nc; ++c)
+        if (!errors[c].empty()) throw std::runtime_error("cloud " + std::to_string(c) + ": " + errors[c]);
+    const double secs = since(t0);
+    for (auto& r : out) r.stats.seconds_total = secs;
+    return out;
+}
+
 }  // namespace tops

@@ -163,9 +163,10 @@ py::tuple persistence_pairs(F32Array x, u32 maxdim, float threshold, const std::
     return py::make_tuple(to_python(r.barcode), e[0], e[1], stats_dict(r.stats));
 }
 
-// BatchRunner: points as one packed float32 array (clouds concatenated) or a
-// list of (n_i, d_i) arrays.
-std::vector<float> pack(BatchRunner& br, py::object clouds) {
+// Batch runners: points as one packed float32 array (clouds concatenated)
+// or a list of (n_i, d_i) arrays.
+template <class R>
+std::vector<float> pack_points(const R& br, py::object clouds) {
     std::vector<float> packed;
     if (py::isinstance<py::list>(clouds) || py::isinstance<py::tuple>(clouds)) {
         for (auto h : clouds) {
@@ -180,16 +181,37 @@ std::vector<float> pack(BatchRunner& br, py::object clouds) {
     return packed;
 }
 
-void batch_front(BatchRunner& br, py::object clouds, bool use_graph) {
-    const std::vector<float> packed = pack(br, clouds);
+template <class R>
+void batch_front(R& br, py::object clouds, bool use_graph) {
+    const std::vector<float> packed = pack_points(br, clouds);
     py::gil_scoped_release release;
     br.front(packed.data(), use_graph);
 }
 
+py::tuple knn_batch_graph(const BatchKnnRunner& br, u32 cloud) {
+    const CsrGraph g = br.graph(cloud);
+    py::array_t<uint64_t> rp(g.row_ptr.size());
+    std::copy(g.row_ptr.begin(), g.row_ptr.end(), rp.mutable_data());
+    py::array_t<uint32_t> col(g.col_idx.size());
+    std::copy(g.col_idx.begin(), g.col_idx.end(), col.mutable_data());
+    py::array_t<float> w(g.weights.size());
+    std::copy(g.weights.begin(), g.weights.end(), w.mutable_data());
+    return py::make_tuple(rp, col, w);
+}
+
+py::array_t<uint32_t> knn_batch_lists(BatchKnnRunner& br, u32 cloud) {
+    const KnnLists L = br.lists(cloud);
+    py::array_t<uint32_t> a({py::ssize_t(L.n), py::ssize_t(L.k)});
+    std::copy(L.ids.begin(), L.ids.end(), a.mutable_data());
+    return a;
+}
+
+
 // With pairs, each entry also carries the cloud's critical edges:
 // (dgms, birth_edges, death_edges, info) as persistence_pairs.
-py::list batch_run(BatchRunner& br, py::object clouds, bool use_graph, u32 threads, bool pairs) {
-    const std::vector<float> packed = pack(br, clouds);
+template <class R>
+py::list batch_run(R& br, py::object clouds, bool use_graph, u32 threads, bool pairs) {
+    const std::vector<float> packed = pack_points(br, clouds);
     const float* p = packed.data();
     std::vector<Result> rs;
     std::vector<CriticalEdges> ce;
@@ -228,14 +250,28 @@ PYBIND11_MODULE(_turbotops, m) {
              }),
              py::arg("n"), py::arg("dim"), py::arg("metric") = "l2", py::arg("maxdim") = 1,
              py::arg("mode") = "auto", py::arg("witness_flags") = true)
-        .def("front", &batch_front, py::arg("clouds"), py::arg("use_graph") = true)
-        .def("run", &batch_run, py::arg("clouds"), py::arg("use_graph") = true, py::arg("threads") = 0,
+        .def("front", &batch_front<BatchRunner>, py::arg("clouds"), py::arg("use_graph") = true)
+        .def("run", &batch_run<BatchRunner>, py::arg("clouds"), py::arg("use_graph") = true, py::arg("threads") = 0,
              py::arg("pairs") = false)
         .def("time_distance_stage", &BatchRunner::time_distance_stage)
         .def_property_readonly("device_ms", &BatchRunner::device_ms)
         .def_property_readonly("rows_per_wave", &BatchRunner::rows_per_wave)
         .def_property_readonly("edges_total", &BatchRunner::edges_total)
         .def_property_readonly("edges_kept", &BatchRunner::edges_kept);
+    py::class_<BatchKnnRunner>(m, "BatchKnnRunner")
+        .def(py::init([](const std::vector<u32>& n, const std::vector<u32>& dim, const std::string& metric, u32 k,
+                         u32 maxdim, const std::string& mode) {
+                 return std::make_unique<BatchKnnRunner>(n, dim, metric, k, maxdim, parse_mode(mode));
+             }),
+             py::arg("n"), py::arg("dim"), py::arg("metric") = "l2", py::arg("k") = 15, py::arg("maxdim") = 1,
+             py::arg("mode") = "auto")
+        .def("front", &batch_front<BatchKnnRunner>, py::arg("clouds"), py::arg("use_graph") = true)
+        .def("run", &batch_run<BatchKnnRunner>, py::arg("clouds"), py::arg("use_graph") = true, py::arg("threads") = 0,
+             py::arg("pairs") = false)
+        .def("graph", &knn_batch_graph, py::arg("cloud"))
+        .def("lists", &knn_batch_lists, py::arg("cloud"))
+        .def_property_readonly("device_ms", &BatchKnnRunner::device_ms)
+        .def_property_readonly("nnz", &BatchKnnRunner::nnz);
     py::register_exception_translator([](std::exception_ptr p) {
         try {
             if (p) std::rethrow_exception(p);

@@ -217,6 +217,36 @@ class BatchRunner {
     u64 edges_total_ = 0, edges_kept_ = 0;
 };
 
+// The batched kNN-graph front end (tt_batch_knn_*): every cloud's exact
+// kNN lists and symmetric graph in one device sequence, then the sparse
+// reductions of the graphs on host threads (largest first). Results equal
+// persistence(cloud, graph = knn, k) on each cloud alone.
+class BatchKnnRunner {
+  public:
+    BatchKnnRunner(const std::vector<u32>& n, const std::vector<u32>& dim, const std::string& metric, u32 k,
+                   u32 maxdim = 1, Mode mode = Mode::Auto);
+    ~BatchKnnRunner();
+    BatchKnnRunner(const BatchKnnRunner&) = delete;
+    BatchKnnRunner& operator=(const BatchKnnRunner&) = delete;
+    std::vector<Result> run(const f32* packed_points, bool use_graph = true, u32 threads = 0,
+                            std::vector<CriticalEdges>* edges = nullptr);
+    // the device stage alone; the graphs land in host memory
+    void front(const f32* packed_points, bool use_graph = true);
+    CsrGraph graph(u32 cloud) const;   // after front() or run()
+    KnnLists lists(u32 cloud);
+    float device_ms() const { return device_ms_; }
+    u64 nnz() const { return nnz_; }
+    u64 total_points() const;
+
+  private:
+    tt_batch_knn* h_ = nullptr;
+    std::vector<u32> n_, dim_;
+    u32 maxdim_ = 1;
+    Mode mode_ = Mode::Auto;
+    float device_ms_ = 0;
+    u64 nnz_ = 0;
+};
+
 void set_num_threads(u32 t);
 u32 num_threads();
 u64 default_memory_budget();
