@@ -65,13 +65,22 @@ __global__ __launch_bounds__(GB) void radius_tiles(const float* __restrict__ x, 
     const u32 j = bj * GT + lane;
     const u32 jc = j < n ? j : n - 1;
     const u32 ibase = bi * GT + wave * ROWS;
+    // Compile-time dimensions measure ordering keys (for L2 the sum before
+    // the square root): a key above key_reject_above(tau) cannot pass, so only
+    // the rest take the exact square root; the test itself stays d <= tau on
+    // the distance.
     float val[ROWS];
-    if constexpr (DIM > 0) rows_static<V, DIM>(x, n, ma, ibase, jc, val);
+    if constexpr (DIM > 0) rows_static<V, DIM, ROWS, true>(x, n, ma, ibase, jc, val);
     else rows_runtime<V>(x, n, dim_rt, ma, ibase, jc, val);
+    const float reject = DIM > 0 ? key_reject_above<V>(tau) : INFINITY;
 #pragma unroll
     for (u32 r = 0; r < ROWS; ++r) {
         const u32 i = ibase + r;
-        const bool keep = i < n && j < i && val[r] <= tau;
+        bool keep = i < n && j < i && val[r] <= reject;
+        if constexpr (DIM > 0) {
+            if (keep) val[r] = key_to_dist<V>(val[r]);
+        }
+        keep = keep && val[r] <= tau;
         const u64 b = __ballot(keep);
         if (b) {
             unsigned long long base = 0;
