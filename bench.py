@@ -320,6 +320,10 @@ def run_batched(args, metric="l2"):
         tf = time.perf_counter() - t0
         out["diagrams_per_sec_all_512"] = {"batched": round(len(clouds) / tf, 3), "whole_run_s": round(tf, 3)}
     del bp
+    bn = BatchPersistence([c.shape for c in clouds], metric=metric, witness_flags=False)
+    wall_n, dev_n = _front_timing(bn, clouds)
+    out["front_end_boundary_no_witness"] = {"device_ms": round(dev_n, 3), "wall_ms": round(wall_n * 1e3, 3)}
+    del bn
     bc = BatchPersistence([c.shape for c in clouds], metric=metric, mode="coboundary")
     wall_c, dev_c = _front_timing(bc, clouds)
     out["front_end_coboundary_order"] = {"device_ms": round(dev_c, 3), "wall_ms": round(wall_c * 1e3, 3),
@@ -350,6 +354,43 @@ def run_batched(args, metric="l2"):
                   f"{sum(c.shape[0] for c in sample)}); whole runs: GPU front end + host reductions "
                   f"(batched: all host threads; plain: persistence() per cloud)"}
     return out
+
+
+def run_batched_knn(args, k=15, plain_count=128):
+    """The 512-cloud set through the batch's kNN stage (graph="knn", k = 15,
+    maxdim 1): every cloud's exact kNN lists and symmetric graph in one
+    recorded device sequence, then the sparse reductions on host threads;
+    against persistence(cloud, graph="knn", k=15) per cloud on the first
+    `plain_count` clouds. Diagrams compared cloud by cloud on that sample."""
+    from turbotops_amd import BatchPersistence, persistence
+    clouds = c4_clouds()
+    bp = BatchPersistence([c.shape for c in clouds], graph="knn", k=k)
+    bp.front_end(clouds)                 # graph capture
+    fronts = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        bp.front_end(clouds)
+        fronts.append((time.perf_counter() - t0, bp.device_ms))
+    wall, dev = min(fronts, key=lambda r: r[1])
+    res = bp(clouds)
+    t0 = time.perf_counter()
+    res = bp(clouds)
+    tb = time.perf_counter() - t0
+    sample = clouds[:plain_count]
+    t0 = time.perf_counter()
+    plain = [persistence(c, maxdim=1, graph="knn", k=k)[0] for c in sample]
+    tp = time.perf_counter() - t0
+    same = all(all(np.array_equal(np.asarray(a), np.asarray(b)) for a, b in zip(r[0], p))
+               for r, p in zip(res, plain))
+    return {"workload": f"the 512 clouds, symmetric exact kNN graph k={k}, l2, maxdim 1",
+            "front_end_device_ms": round(dev, 3), "front_end_wall_ms": round(wall * 1e3, 3),
+            "csr_entries": int(bp.nnz),
+            "diagrams_per_sec": {"batched_all_512": round(len(clouds) / tb, 3),
+                                 "plain": round(len(sample) / tp, 3),
+                                 "ratio": round((tp / len(sample)) / (tb / len(clouds)), 2),
+                                 "identical_diagrams": bool(same),
+                                 "sample": f"plain: persistence() per cloud on the first {len(sample)} clouds; "
+                                           f"batched: whole runs of all 512 (all host threads)"}}
 
 
 def run_batched_linf(args):
@@ -474,6 +515,8 @@ def main():
                     help="skip the whole batched run of all 512 clouds")
     ap.add_argument("--no-c4-linf", dest="c4_linf", action="store_false",
                     help="skip the 512-cloud set under L-inf")
+    ap.add_argument("--no-c4-knn", dest="c4_knn", action="store_false",
+                    help="skip the 512-cloud set through the batch's kNN-graph stage")
     ap.add_argument("--c5-n", type=int, default=2048, help="points of the differentiable circle")
     ap.add_argument("--c5-batch", type=int, default=64, help="clouds in the differentiable batch")
     ap.add_argument("--c5-batch-nmax", type=int, default=4096, help="largest n_i taken into that batch")
@@ -496,6 +539,9 @@ def main():
         if args.c4_linf:
             print("bench: batched leg, L-inf", file=sys.stderr, flush=True)
             out["batched_linf"] = run_batched_linf(args)
+        if args.c4_knn:
+            print("bench: batched leg, kNN graphs", file=sys.stderr, flush=True)
+            out["batched_knn"] = run_batched_knn(args)
     if not args.no_diff:
         print("bench: differentiable leg", file=sys.stderr, flush=True)
         out["differentiable"] = run_differentiable(args)

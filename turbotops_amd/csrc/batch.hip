@@ -165,6 +165,12 @@ __global__ __launch_bounds__(256) void batch_compact(BatchShape b, const float* 
         const u64 M = u64(n) * (n - 1) / 2;
         const u64 cloud_bits = u64(c) << (31 + ib);
         u64 o = off[r];
+        // comb: the cloud's keys in row-major order, which is ascending
+        // index; otherwise mirrored within the cloud's segment, descending
+        // index. Either way the order the sort must keep among equal
+        // diameters is already there, so the sort runs over the (cloud,
+        // diameter) bits alone and keeps it by stability.
+        const u64 mirror = u64(off[b.row_base[c]]) + off[b.row_base[c + 1]] - 1;
         const u64 rowbase = u64(i) * (i - 1) / 2;
         for (u32 j0 = 0; j0 < i; j0 += 64) {
             const u32 j = j0 + lane;
@@ -174,7 +180,8 @@ __global__ __launch_bounds__(256) void batch_compact(BatchShape b, const float* 
             if (keep) {
                 const u32 rk = __builtin_amdgcn_mbcnt_hi(u32(bal >> 32), __builtin_amdgcn_mbcnt_lo(u32(bal), 0u));
                 const u64 idx = rowbase + j;
-                keys[o + rk] = cloud_bits | (u64(__float_as_uint(d)) << ib) | (comb ? idx : M - 1 - idx);
+                const u64 at = comb ? o + rk : mirror - (o + rk);
+                keys[at] = cloud_bits | (u64(__float_as_uint(d)) << ib) | (comb ? idx : M - 1 - idx);
             }
             o += __popcll(bal);
         }
@@ -512,8 +519,9 @@ struct BatchState {
                 batch_block_min<<<gw, 256, 0, s>>>(shape, d_mats, d_bmin);
                 batch_witness<<<ge, 256, 0, s>>>(shape, d_mats, d_bmin, d_keys, d_total, ib);
             }
-            radix_sort_u64_counted(d_keys, capacity, d_total, 0, cb + 31 + ib + fb, d_alt, d_hist, d_stmp, s);
-            batch_unpack_comb<<<ge, 256, 0, s>>>(d_keys, d_total, ib, fb, d_out_idx, d_out_d);
+            const u64* sorted =
+                radix_sort_u64_counted(d_keys, capacity, d_total, ib + fb, cb + 31 + ib + fb, d_alt, d_hist, d_stmp, s);
+            batch_unpack_comb<<<ge, 256, 0, s>>>(sorted, d_total, ib, fb, d_out_idx, d_out_d);
             copy_u32<<<1, 256, 0, s>>>(d_seg, nclouds + 1, d_kseg);
             TTG_LAUNCH_CHECK();
             return;
@@ -521,8 +529,9 @@ struct BatchState {
         if (tagged) {
             batch_block_min<<<gw, 256, 0, s>>>(shape, d_mats, d_bmin);
             batch_apparent_tag<<<ge, 256, 0, s>>>(shape, d_mats, d_bmin, d_keys, d_total, ib);
-            radix_sort_u64_counted(d_keys, capacity, d_total, 0, cb + 31 + ib + 1, d_alt, d_hist, d_stmp, s);
-            batch_unpack_tagged<<<ge, 256, 0, s>>>(d_keys, d_total, ib, d_n, d_out_idx, d_out_d, d_out_ap);
+            const u64* sorted =
+                radix_sort_u64_counted(d_keys, capacity, d_total, ib + 1, cb + 31 + ib + 1, d_alt, d_hist, d_stmp, s);
+            batch_unpack_tagged<<<ge, 256, 0, s>>>(sorted, d_total, ib, d_n, d_out_idx, d_out_d, d_out_ap);
             copy_u32<<<1, 256, 0, s>>>(d_seg, nclouds + 1, d_kseg);
             TTG_LAUNCH_CHECK();
             return;
@@ -536,8 +545,8 @@ struct BatchState {
         batch_select<<<ge, 256, 0, s>>>(d_keys, d_keep, d_pos, d_total, d_alt);
         batch_kept_segments<<<1, 256, 0, s>>>(d_seg, d_pos, d_keep, d_total, nclouds, d_kseg);
         TTG_LAUNCH_CHECK();
-        radix_sort_u64_counted(d_alt, capacity, d_kept, 0, cb + 31 + ib, d_keys, d_hist, d_stmp, s);
-        batch_unpack<<<ge, 256, 0, s>>>(d_alt, d_kept, ib, d_n, d_out_idx, d_out_d);
+        const u64* sorted = radix_sort_u64_counted(d_alt, capacity, d_kept, ib, cb + 31 + ib, d_keys, d_hist, d_stmp, s);
+        batch_unpack<<<ge, 256, 0, s>>>(sorted, d_kept, ib, d_n, d_out_idx, d_out_d);
         TTG_LAUNCH_CHECK();
     }
 };

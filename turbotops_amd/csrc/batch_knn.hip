@@ -224,12 +224,12 @@ struct BatchKnnState {
         TTG_LAUNCH_CHECK();
         bknn_keys<<<grid_for(total_rows), 256, 0, s>>>(shape, d_ids, rb, d_keys);
         TTG_LAUNCH_CHECK();
-        radix_sort_u64_counted(d_keys, m, d_count, 0, cb + 2 * rb, d_alt, d_hist, d_stmp, s);
-        bknn_mark<<<grid_for(m), 256, 0, s>>>(d_keys, m, d_flag);
+        const u64* sorted = radix_sort_u64_counted(d_keys, m, d_count, 0, cb + 2 * rb, d_alt, d_hist, d_stmp, s);
+        bknn_mark<<<grid_for(m), 256, 0, s>>>(sorted, m, d_flag);
         TTG_LAUNCH_CHECK();
         exclusive_scan_u32_buf(d_flag, d_pos, m, d_total, d_stmp, s);
         fill_device_u32(d_deg, total_rows + 1, 0u, s);
-        launch_fill(variant, grid_for(m), s, shape, ma, d_pts, d_keys, d_flag, d_pos, m, rb, d_col, d_w, d_deg);
+        launch_fill(variant, grid_for(m), s, shape, ma, d_pts, sorted, d_flag, d_pos, m, rb, d_col, d_w, d_deg);
         TTG_LAUNCH_CHECK();
         exclusive_scan_u32_buf(d_deg, d_rp, total_rows + 1, nullptr, d_stmp, s);
     }

@@ -83,9 +83,13 @@ __global__ __launch_bounds__(EB) void edge_count(const float* __restrict__ mat, 
     }
 }
 
+// Keys in row-major order, which is ascending edge index (comb), or mirrored
+// over [0, m), descending index (the coboundary order): the sort then runs
+// over the diameter bits alone and keeps the index order of equal diameters
+// by stability.
 __global__ __launch_bounds__(EB) void edge_compact(const float* __restrict__ mat, u32 n, float thr,
                                                    const u32* __restrict__ off, u32 dmin, int ib,
-                                                   u64 M, bool comb, u64* __restrict__ keys) {
+                                                   u64 M, u64 m, bool comb, u64* __restrict__ keys) {
     const u32 lane = threadIdx.x & 63;
     const u32 gw = (blockIdx.x * EB + threadIdx.x) >> 6;
     const u32 nw = (gridDim.x * EB) >> 6;
@@ -101,7 +105,8 @@ __global__ __launch_bounds__(EB) void edge_compact(const float* __restrict__ mat
             if (keep) {
                 const u32 r = __builtin_amdgcn_mbcnt_hi(u32(b >> 32), __builtin_amdgcn_mbcnt_lo(u32(b), 0u));
                 const u64 idx = rowbase + j;
-                keys[o + r] = (u64(__float_as_uint(d) - dmin) << ib) | (comb ? idx : M - 1 - idx);
+                const u64 at = comb ? o + r : m - 1 - (o + r);
+                keys[at] = (u64(__float_as_uint(d) - dmin) << ib) | (comb ? idx : M - 1 - idx);
             }
             o += __popcll(b);
         }
@@ -237,7 +242,7 @@ DenseEdges dense_edges(const float* d_mat, u32 n, float thr, bool want_apparent,
     if (ib + db + 1 > 64) fail(TT_ELENGTH, "dense_edges: key does not fit 64 bits");
     u64* keys = ws.get<u64>(out.m);
     u64* alt = ws.get<u64>(out.m);
-    edge_compact<<<grid_for(n), EB, 0, s>>>(d_mat, n, thr, cnt, dmin, ib, M, comb, keys);
+    edge_compact<<<grid_for(n), EB, 0, s>>>(d_mat, n, thr, cnt, dmin, ib, M, out.m, comb, keys);
     TTG_LAUNCH_CHECK();
     const int fb = want_apparent ? 1 : 0;   // the flag bit below the key (ib + db + 1 <= 64)
     if (want_apparent) {
@@ -249,7 +254,7 @@ DenseEdges dense_edges(const float* d_mat, u32 n, float thr, bool want_apparent,
         else apparent_scan<<<grid_for((out.m + 63) / 64), EB, 0, s>>>(d_mat, n, bmin, nb, keys, out.m, ib, M);
         TTG_LAUNCH_CHECK();
     }
-    radix_sort_u64(keys, nullptr, out.m, 0, ib + db + fb, alt, nullptr, ws, s);
+    radix_sort_u64(keys, nullptr, out.m, ib + fb, ib + db + fb, alt, nullptr, ws, s);
     const u32 ug = u32(std::min<u64>((out.m + EB - 1) / EB, 65536));
     if (want_apparent) {
         out.apparent = ws.get<u8>(out.m);

@@ -330,20 +330,10 @@ void exclusive_scan_u64(const u64* in, u64* out, u64 count, u64* d_total, Worksp
     exclusive_scan<u64>(in, out, count, d_total, ws, s);
 }
 
-namespace {
 
-__global__ void copy_counted(const u64* __restrict__ src, u64* __restrict__ dst, u64 cap,
-                             const u32* __restrict__ d_count) {
-    const u64 count = min<u64>(cap, *d_count);
-    for (u64 t = blockIdx.x * u64(blockDim.x) + threadIdx.x; t < count; t += u64(gridDim.x) * blockDim.x)
-        dst[t] = src[t];
-}
-
-}  // namespace
-
-void radix_sort_u64_counted(u64* keys, u64 capacity, const u32* d_count, int lo_bit, int hi_bit,
+u64* radix_sort_u64_counted(u64* keys, u64 capacity, const u32* d_count, int lo_bit, int hi_bit,
                             u64* keys_alt, u32* hist, u32* scan_tmp, hipStream_t s) {
-    if (capacity <= 1 || hi_bit <= lo_bit) return;
+    if (capacity <= 1 || hi_bit <= lo_bit) return keys;
     if (capacity >= (u64(1) << 32)) fail(TT_ELENGTH, "radix_sort: too many keys for one sort");
     const u32 nb = sort_blocks(capacity);
     u64 *ks = keys, *kd = keys_alt;
@@ -358,10 +348,7 @@ void radix_sort_u64_counted(u64* keys, u64 capacity, const u32* d_count, int lo_
         TTG_LAUNCH_CHECK();
         std::swap(ks, kd);
     }
-    if (ks != keys) {
-        copy_counted<<<4096, 256, 0, s>>>(ks, keys, capacity, d_count);
-        TTG_LAUNCH_CHECK();
-    }
+    return ks;
 }
 
 u64 radix_hist_entries(u64 capacity) { return u64(RADIX) * sort_blocks(capacity); }
