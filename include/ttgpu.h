@@ -145,7 +145,10 @@ tt_status tt_mst_deaths_dev_noalloc(const float* d_pts, uint32_t n, uint32_t dim
  * is the zero-pivot facet of its zero-pivot cofacet. threshold = +inf means
  * the enclosing radius (rips.hpp:1073), reported in *threshold_out.
  * The library mallocs *idx, *d and (if apparent != NULL) *apparent; release
- * them with tt_free. n <= 92682 (vertex pairs < 2^32). */
+ * them with tt_free. Any n whose matrix fits in device memory; the sort keys
+ * hold the edge index and the diameter's bits above the smallest kept one
+ * (TT_ELENGTH in the rare case that index and diameter range need more than
+ * 63 bits together). */
 tt_status tt_dense_edges(const float* pts, uint32_t n, uint32_t dim, int kind, float p,
                          float threshold, uint64_t* m, uint64_t** idx, float** d,
                          uint8_t** apparent, float* threshold_out);

@@ -461,11 +461,12 @@ struct BatchState {
     Workspace ws;
     float *d_pts = nullptr, *d_mats = nullptr, *d_thr = nullptr, *d_out_d = nullptr, *d_bmin = nullptr;
     u32 *d_rowmax = nullptr, *d_cnt = nullptr, *d_total = nullptr, *d_seg = nullptr, *d_keep = nullptr,
-        *d_pos = nullptr, *d_kept = nullptr, *d_kseg = nullptr, *d_hist = nullptr, *d_stmp = nullptr,
+        *d_pos = nullptr, *d_kept = nullptr, *d_kseg = nullptr, *d_stmp = nullptr,
         *d_n = nullptr, *d_dim = nullptr;
     u8* d_out_ap = nullptr;
     u64 *d_keys = nullptr, *d_alt = nullptr, *d_out_idx = nullptr, *d_pt_off = nullptr, *d_mat_off = nullptr,
         *d_row_base = nullptr, *d_tile_base = nullptr, *d_pair_base = nullptr, *d_bmin_off = nullptr;
+    u64* d_sort = nullptr;   // radix sort scratch (radix_scratch_entries)
     u32 *d_tile_cloud = nullptr, *d_row_cloud = nullptr;
     BatchShape shape{};
     // host (pinned)
@@ -520,7 +521,7 @@ struct BatchState {
                 batch_witness<<<ge, 256, 0, s>>>(shape, d_mats, d_bmin, d_keys, d_total, ib);
             }
             const u64* sorted =
-                radix_sort_u64_counted(d_keys, capacity, d_total, ib + fb, cb + 31 + ib + fb, d_alt, d_hist, d_stmp, s);
+                radix_sort_u64_counted(d_keys, capacity, d_total, ib + fb, cb + 31 + ib + fb, d_alt, d_sort, s);
             batch_unpack_comb<<<ge, 256, 0, s>>>(sorted, d_total, ib, fb, d_out_idx, d_out_d);
             copy_u32<<<1, 256, 0, s>>>(d_seg, nclouds + 1, d_kseg);
             TTG_LAUNCH_CHECK();
@@ -530,7 +531,7 @@ struct BatchState {
             batch_block_min<<<gw, 256, 0, s>>>(shape, d_mats, d_bmin);
             batch_apparent_tag<<<ge, 256, 0, s>>>(shape, d_mats, d_bmin, d_keys, d_total, ib);
             const u64* sorted =
-                radix_sort_u64_counted(d_keys, capacity, d_total, ib + 1, cb + 31 + ib + 1, d_alt, d_hist, d_stmp, s);
+                radix_sort_u64_counted(d_keys, capacity, d_total, ib + 1, cb + 31 + ib + 1, d_alt, d_sort, s);
             batch_unpack_tagged<<<ge, 256, 0, s>>>(sorted, d_total, ib, d_n, d_out_idx, d_out_d, d_out_ap);
             copy_u32<<<1, 256, 0, s>>>(d_seg, nclouds + 1, d_kseg);
             TTG_LAUNCH_CHECK();
@@ -545,7 +546,7 @@ struct BatchState {
         batch_select<<<ge, 256, 0, s>>>(d_keys, d_keep, d_pos, d_total, d_alt);
         batch_kept_segments<<<1, 256, 0, s>>>(d_seg, d_pos, d_keep, d_total, nclouds, d_kseg);
         TTG_LAUNCH_CHECK();
-        const u64* sorted = radix_sort_u64_counted(d_alt, capacity, d_kept, ib, cb + 31 + ib, d_keys, d_hist, d_stmp, s);
+        const u64* sorted = radix_sort_u64_counted(d_alt, capacity, d_kept, ib, cb + 31 + ib, d_keys, d_sort, s);
         batch_unpack<<<ge, 256, 0, s>>>(sorted, d_kept, ib, d_n, d_out_idx, d_out_d);
         TTG_LAUNCH_CHECK();
     }
@@ -640,8 +641,8 @@ void batch_setup(BatchState& b, u32 nclouds, const uint32_t* n, const uint32_t* 
     b.d_kept = ws.get<u32>(1);
     b.d_out_idx = ws.get<u64>(b.capacity);
     b.d_out_d = ws.get<float>(b.capacity);
-    b.d_hist = ws.get<u32>(radix_hist_entries(b.capacity));
-    const u64 stmp = std::max({scan_tmp_entries(radix_hist_entries(b.capacity)), scan_tmp_entries(b.capacity),
+    b.d_sort = ws.get<u64>(radix_scratch_entries(b.capacity));
+    const u64 stmp = std::max({scan_tmp_entries(b.capacity),
                                scan_tmp_entries(b.total_rows + 1), u64(1)});
     b.d_stmp = ws.get<u32>(stmp);
     b.d_n = ws.get<u32>(nclouds);

@@ -190,10 +190,11 @@ struct BatchKnnState {
     Workspace ws;
     float *d_pts = nullptr, *d_w = nullptr;
     u32 *d_ids = nullptr, *d_flag = nullptr, *d_pos = nullptr, *d_col = nullptr, *d_deg = nullptr,
-        *d_total = nullptr, *d_count = nullptr, *d_hist = nullptr, *d_stmp = nullptr, *d_n = nullptr,
+        *d_total = nullptr, *d_count = nullptr, *d_stmp = nullptr, *d_n = nullptr,
         *d_dim = nullptr, *d_row_cloud = nullptr, *d_qb_cloud = nullptr, *d_qb_first = nullptr,
         *d_rp = nullptr;   // row offsets over every cloud's rows (total_rows + 1)
     u64 *d_keys = nullptr, *d_alt = nullptr, *d_pt_off = nullptr, *d_row_base = nullptr, *d_list_off = nullptr;
+    u64* d_sort = nullptr;   // radix sort scratch (radix_scratch_entries)
     KnnBatchShape shape{};
     float* h_pts = nullptr;
     std::vector<u32> h_rp, h_col, lists;
@@ -224,7 +225,7 @@ struct BatchKnnState {
         TTG_LAUNCH_CHECK();
         bknn_keys<<<grid_for(total_rows), 256, 0, s>>>(shape, d_ids, rb, d_keys);
         TTG_LAUNCH_CHECK();
-        const u64* sorted = radix_sort_u64_counted(d_keys, m, d_count, 0, cb + 2 * rb, d_alt, d_hist, d_stmp, s);
+        const u64* sorted = radix_sort_u64_counted(d_keys, m, d_count, 0, cb + 2 * rb, d_alt, d_sort, s);
         bknn_mark<<<grid_for(m), 256, 0, s>>>(sorted, m, d_flag);
         TTG_LAUNCH_CHECK();
         exclusive_scan_u32_buf(d_flag, d_pos, m, d_total, d_stmp, s);
@@ -306,8 +307,8 @@ void bknn_setup(BatchKnnState& b, u32 nclouds, const uint32_t* n, const uint32_t
     b.d_rp = ws.get<u32>(b.total_rows + 1);
     b.d_total = ws.get<u32>(1);
     b.d_count = ws.get<u32>(1);
-    b.d_hist = ws.get<u32>(radix_hist_entries(m1));
-    b.d_stmp = ws.get<u32>(std::max({scan_tmp_entries(radix_hist_entries(m1)), scan_tmp_entries(m1),
+    b.d_sort = ws.get<u64>(radix_scratch_entries(m1));
+    b.d_stmp = ws.get<u32>(std::max({scan_tmp_entries(m1),
                                      scan_tmp_entries(b.total_rows + 1), u64(1)}));
     b.d_n = ws.get<u32>(nclouds);
     b.d_dim = ws.get<u32>(nclouds);

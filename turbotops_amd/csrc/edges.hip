@@ -50,7 +50,7 @@ __device__ __forceinline__ void decode_edge(u64 idx, u32& i, u32& j) {
 }
 
 __global__ __launch_bounds__(EB) void edge_count(const float* __restrict__ mat, u32 n, float thr,
-                                                 u32* __restrict__ cnt, u32* __restrict__ mm) {
+                                                 u64* __restrict__ cnt, u32* __restrict__ mm) {
     // one wave per row, rows grid-strided
     const u32 lane = threadIdx.x & 63;
     const u32 gw = (blockIdx.x * EB + threadIdx.x) >> 6;
@@ -88,7 +88,7 @@ __global__ __launch_bounds__(EB) void edge_count(const float* __restrict__ mat, 
 // over the diameter bits alone and keeps the index order of equal diameters
 // by stability.
 __global__ __launch_bounds__(EB) void edge_compact(const float* __restrict__ mat, u32 n, float thr,
-                                                   const u32* __restrict__ off, u32 dmin, int ib,
+                                                   const u64* __restrict__ off, u32 dmin, int ib,
                                                    u64 M, u64 m, bool comb, u64* __restrict__ keys) {
     const u32 lane = threadIdx.x & 63;
     const u32 gw = (blockIdx.x * EB + threadIdx.x) >> 6;
@@ -150,7 +150,8 @@ __global__ __launch_bounds__(EB) void block_min(const float* __restrict__ mat, u
 // not change the order of the sort that follows.
 __global__ __launch_bounds__(EB) void apparent_scan(const float* __restrict__ mat, u32 n,
                                                     const float* __restrict__ bmin, u32 nb,
-                                                    u64* __restrict__ keys, u64 m, int ib, u64 M) {
+                                                    u64* __restrict__ keys, u64 m, int ib, u64 M,
+                                                    u32* __restrict__ flags) {
     const u32 lane = threadIdx.x & 63;
     const u64 gw = (blockIdx.x * u64(EB) + threadIdx.x) >> 6;
     const u64 nw = (u64(gridDim.x) * EB) >> 6;
@@ -167,7 +168,10 @@ __global__ __launch_bounds__(EB) void apparent_scan(const float* __restrict__ ma
             d = mat[u64(i) * n + j];
         }
         const bool ap = edges_apparent_lanes<TOPK>(mat, bmin, n, n, nb, valid, i, j, d, lane);
-        if (valid) keys[e] = (k << 1) | (ap ? 1u : 0u);
+        if (valid) {
+            if (flags) flags[e] = ap ? 1u : 0u;
+            else keys[e] = (k << 1) | (ap ? 1u : 0u);
+        }
     }
 }
 
@@ -176,7 +180,8 @@ __global__ __launch_bounds__(EB) void apparent_scan(const float* __restrict__ ma
 // becomes the key's lowest bit, as in apparent_scan.
 __global__ __launch_bounds__(EB) void witness_scan(const float* __restrict__ mat, u32 n,
                                                    const float* __restrict__ bmin, u32 nb,
-                                                   u64* __restrict__ keys, u64 m, int ib) {
+                                                   u64* __restrict__ keys, u64 m, int ib,
+                                                   u32* __restrict__ flags) {
     const u32 lane = threadIdx.x & 63;
     const u64 gw = (blockIdx.x * u64(EB) + threadIdx.x) >> 6;
     const u64 nw = (u64(gridDim.x) * EB) >> 6;
@@ -193,13 +198,22 @@ __global__ __launch_bounds__(EB) void witness_scan(const float* __restrict__ mat
             d = mat[u64(i) * n + j];
         }
         const bool wf = edges_apparent_lanes<TOPK, true>(mat, bmin, n, n, nb, valid, i, j, d, lane);
-        if (valid) keys[e] = (k << 1) | (wf ? 1u : 0u);
+        if (valid) {
+            if (flags) flags[e] = wf ? 1u : 0u;
+            else keys[e] = (k << 1) | (wf ? 1u : 0u);
+        }
     }
 }
 
-// Sorted keys carrying the apparent flag in bit 0 -> flags.
-__global__ __launch_bounds__(EB) void edge_flags(u64* __restrict__ keys, u64 m, u8* __restrict__ flags) {
+// Sorted keys carrying the apparent flag in bit 0 -> flags; with `sorted`
+// (the flags sorted alongside as a payload) the keys hold no flag bit.
+__global__ __launch_bounds__(EB) void edge_flags(u64* __restrict__ keys, u64 m, const u32* __restrict__ sorted,
+                                                 u8* __restrict__ flags) {
     for (u64 t = blockIdx.x * u64(EB) + threadIdx.x; t < m; t += u64(gridDim.x) * EB) {
+        if (sorted) {
+            flags[t] = u8(sorted[t]);
+            continue;
+        }
         const u64 k = keys[t];
         flags[t] = u8(k & 1);
         keys[t] = k >> 1;
@@ -219,46 +233,51 @@ DenseEdges dense_edges(const float* d_mat, u32 n, float thr, bool want_apparent,
     // want_apparent: in comb order, the ready-witness flags
     DenseEdges out;
     if (n < 2) return out;
-    if (u64(n) * (n - 1) / 2 >= (u64(1) << 32))
-        fail(TT_ELENGTH, "dense_edges: more than 2^32 vertex pairs (n > 92682)");
-    const u64 M = u64(n) * (n - 1) / 2;
-    u32* cnt = ws.get<u32>(u64(n) + 1);
+    const u64 M = u64(n) * (n - 1) / 2;   // may pass 2^32 (n > 92682): counts and offsets are 64-bit
+    u64* cnt = ws.get<u64>(u64(n) + 1);
     u32* mm = ws.get<u32>(2);
     const u32 init[2] = {0xffffffffu, 0u};
     TTG_HIP(hipMemcpyAsync(mm, init, sizeof(init), hipMemcpyHostToDevice, s));
     edge_count<<<grid_for(n), EB, 0, s>>>(d_mat, n, thr, cnt, mm);
     TTG_LAUNCH_CHECK();
-    u32* total = ws.get<u32>(1);
-    exclusive_scan_u32(cnt, cnt, n, total, ws, s);
-    u32 hm[3];
+    u64* total = ws.get<u64>(1);
+    exclusive_scan_u64(cnt, cnt, n, total, ws, s);
+    u32 hm[2];
+    u64 hm_total = 0;
     TTG_HIP(hipMemcpyAsync(hm, mm, 2 * sizeof(u32), hipMemcpyDeviceToHost, s));
-    TTG_HIP(hipMemcpyAsync(hm + 2, total, sizeof(u32), hipMemcpyDeviceToHost, s));
+    TTG_HIP(hipMemcpyAsync(&hm_total, total, sizeof(u64), hipMemcpyDeviceToHost, s));
     TTG_HIP(hipStreamSynchronize(s));
-    out.m = hm[2];
+    out.m = hm_total;
     if (!out.m) return out;
     const u32 dmin = hm[0], dmax = hm[1];
     const int ib = std::max(1, bit_length(M - 1));
     const int db = bit_length(u64(dmax - dmin));
-    if (ib + db + 1 > 64) fail(TT_ELENGTH, "dense_edges: key does not fit 64 bits");
+    if (ib + db > 64) fail(TT_ELENGTH, "dense_edges: key does not fit 64 bits");
     u64* keys = ws.get<u64>(out.m);
     u64* alt = ws.get<u64>(out.m);
     edge_compact<<<grid_for(n), EB, 0, s>>>(d_mat, n, thr, cnt, dmin, ib, M, out.m, comb, keys);
     TTG_LAUNCH_CHECK();
-    const int fb = want_apparent ? 1 : 0;   // the flag bit below the key (ib + db + 1 <= 64)
+    // The flag rides below the key where a bit is spare; otherwise (large n
+    // and a wide diameter range) as a 32-bit payload of the sort.
+    const bool payload = want_apparent && ib + db + 1 > 64;
+    const int fb = want_apparent && !payload ? 1 : 0;
+    u32* fl = payload ? ws.get<u32>(out.m) : nullptr;
+    u32* fl_alt = payload ? ws.get<u32>(out.m) : nullptr;
     if (want_apparent) {
         const u32 nb = (n + BLK - 1) / BLK;
         float* bmin = ws.get<float>(u64(n) * nb);
         block_min<<<grid_for(n), EB, 0, s>>>(d_mat, n, nb, bmin);
         TTG_LAUNCH_CHECK();
-        if (comb) witness_scan<<<grid_for((out.m + 63) / 64), EB, 0, s>>>(d_mat, n, bmin, nb, keys, out.m, ib);
-        else apparent_scan<<<grid_for((out.m + 63) / 64), EB, 0, s>>>(d_mat, n, bmin, nb, keys, out.m, ib, M);
+        const u32 g = grid_for((out.m + 63) / 64);
+        if (comb) witness_scan<<<g, EB, 0, s>>>(d_mat, n, bmin, nb, keys, out.m, ib, fl);
+        else apparent_scan<<<g, EB, 0, s>>>(d_mat, n, bmin, nb, keys, out.m, ib, M, fl);
         TTG_LAUNCH_CHECK();
     }
-    radix_sort_u64(keys, nullptr, out.m, ib + fb, ib + db + fb, alt, nullptr, ws, s);
+    radix_sort_u64(keys, fl, out.m, ib + fb, ib + db + fb, alt, fl_alt, ws, s);
     const u32 ug = u32(std::min<u64>((out.m + EB - 1) / EB, 65536));
     if (want_apparent) {
         out.apparent = ws.get<u8>(out.m);
-        edge_flags<<<ug, EB, 0, s>>>(keys, out.m, out.apparent);
+        edge_flags<<<ug, EB, 0, s>>>(keys, out.m, fl, out.apparent);
         TTG_LAUNCH_CHECK();
     }
     out.idx = alt;   // reuse the ping-pong buffer for the unpacked indices

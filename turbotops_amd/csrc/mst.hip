@@ -207,10 +207,10 @@ struct Carve {
 // Every device buffer a Boruvka run on n vertices uses, carved from `c` in
 // a fixed order (a measuring Carve gives the total).
 struct MstBuffers {
-    u32 *comp, *parent, *cw, *bj, *todo, *scal, *hist, *scan_tmp;
+    u32 *comp, *parent, *cw, *bj, *todo, *scal;
     unsigned long long* cp;
     float *bk, *keys;
-    u64 *sk, *alt;
+    u64 *sk, *alt, *sort_scratch;
     MstBuffers(Carve& c, u32 n) {
         comp = c.get<u32>(n);
         parent = c.get<u32>(n);
@@ -223,9 +223,7 @@ struct MstBuffers {
         keys = c.get<float>(n);
         sk = c.get<u64>(n);
         alt = c.get<u64>(n);
-        const u64 he = radix_hist_entries(n);
-        hist = c.get<u32>(he);
-        scan_tmp = c.get<u32>(std::max<u64>(1, scan_tmp_entries(he)));
+        sort_scratch = c.get<u64>(radix_scratch_entries(n));
     }
 };
 
@@ -298,8 +296,8 @@ void mst_deaths_in(const float* d_x, u32 n, u32 dim, int kind, float p, float* d
         default: mst_finish<MV_L1><<<blocks_of(n), MB, 0, s>>>(keys, n - 1, B.sk); break;
     }
     TTG_LAUNCH_CHECK();
-    radix_sort_u64_counted(B.sk, n - 1, scal + 3, 0, 31, B.alt, B.hist, B.scan_tmp, s);
-    mst_unpack<<<blocks_of(n), MB, 0, s>>>(B.sk, n - 1, d_deaths);
+    const u64* sorted = radix_sort_u64_counted(B.sk, n - 1, scal + 3, 0, 31, B.alt, B.sort_scratch, s);
+    mst_unpack<<<blocks_of(n), MB, 0, s>>>(sorted, n - 1, d_deaths);
     TTG_LAUNCH_CHECK();
 }
 

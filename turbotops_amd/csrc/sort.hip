@@ -7,7 +7,8 @@
 // are taken from the key count on the device, so a sort sized for a large
 // capacity costs little when few keys are present):
 //   upsweep    per-block digit histogram over its range (LDS atomics, one
-//              counter row per wave64) -> hist[digit][block]
+//              counter row per wave64) -> hist[digit][block] (64-bit, so the
+//              global offsets of the scan below may pass 2^32)
 //   scan       exclusive scan of hist in digit-major order: the global start
 //              of every (digit, block) run
 //   downsweep  each block walks its tiles in order, ranks a tile's keys
@@ -175,7 +176,7 @@ __device__ __forceinline__ void block_tiles(u64 count, u32 nb, u32 b, u64& t0, u
 
 __global__ __launch_bounds__(RB) void rs_upsweep(const u64* __restrict__ keys, u64 count,
                                                  const u32* __restrict__ d_count, u32 shift,
-                                                 u32 dmask, u32* __restrict__ hist) {
+                                                 u32 dmask, u64* __restrict__ hist) {
     __shared__ u32 h[RB / 64][RADIX];
     if (d_count) count = min<u64>(count, *d_count);
     u64 t0, t1;
@@ -212,12 +213,12 @@ __global__ __launch_bounds__(RB) void rs_downsweep(const u64* __restrict__ keys,
                                                    const u32* __restrict__ vals, u64 count,
                                                    const u32* __restrict__ d_count,
                                                    u32 shift, u32 dmask, u32 dbits,
-                                                   const u32* __restrict__ offs,
+                                                   const u64* __restrict__ offs,
                                                    u64* __restrict__ keys_out,
                                                    u32* __restrict__ vals_out) {
     __shared__ u32 wrun[RB / 64][RADIX];
     __shared__ u32 bstart[RADIX];
-    __shared__ u32 run[RADIX];      // this block's next output position per digit
+    __shared__ u64 run[RADIX];      // this block's next output position per digit
     __shared__ u64 skeys[RTILE];
     __shared__ u32 svals[VALS ? RTILE : 1];
 
@@ -298,7 +299,7 @@ __global__ __launch_bounds__(RB) void rs_downsweep(const u64* __restrict__ keys,
             if (q < tile_count) {
                 const u64 key = skeys[q];
                 const u32 d = u32(key >> shift) & dmask;
-                const u64 g = u64(run[d]) + (q - bstart[d]);
+                const u64 g = run[d] + (q - bstart[d]);
                 keys_out[g] = key;
                 if constexpr (VALS) vals_out[g] = svals[q];
             }
@@ -332,17 +333,19 @@ void exclusive_scan_u64(const u64* in, u64* out, u64 count, u64* d_total, Worksp
 
 
 u64* radix_sort_u64_counted(u64* keys, u64 capacity, const u32* d_count, int lo_bit, int hi_bit,
-                            u64* keys_alt, u32* hist, u32* scan_tmp, hipStream_t s) {
+                            u64* keys_alt, u64* scratch, hipStream_t s) {
     if (capacity <= 1 || hi_bit <= lo_bit) return keys;
-    if (capacity >= (u64(1) << 32)) fail(TT_ELENGTH, "radix_sort: too many keys for one sort");
+    if (capacity >= (u64(1) << 32)) fail(TT_ELENGTH, "radix_sort: a counted sort takes fewer than 2^32 keys");
     const u32 nb = sort_blocks(capacity);
+    u64* hist = scratch;
+    u64* scan_tmp = scratch + u64(RADIX) * nb;
     u64 *ks = keys, *kd = keys_alt;
     for (int shift = lo_bit; shift < hi_bit; shift += 8) {
         const u32 dbits = u32(std::min(8, hi_bit - shift));
         const u32 dmask = (1u << dbits) - 1u;
         rs_upsweep<<<nb, RB, 0, s>>>(ks, capacity, d_count, u32(shift), dmask, hist);
         TTG_LAUNCH_CHECK();
-        exclusive_scan_buf<u32>(hist, hist, u64(RADIX) * nb, nullptr, scan_tmp, s);
+        exclusive_scan_buf<u64>(hist, hist, u64(RADIX) * nb, nullptr, scan_tmp, s);
         rs_downsweep<false><<<nb, RB, 0, s>>>(ks, nullptr, capacity, d_count, u32(shift), dmask, dbits, hist,
                                               kd, nullptr);
         TTG_LAUNCH_CHECK();
@@ -351,15 +354,17 @@ u64* radix_sort_u64_counted(u64* keys, u64 capacity, const u32* d_count, int lo_
     return ks;
 }
 
-u64 radix_hist_entries(u64 capacity) { return u64(RADIX) * sort_blocks(capacity); }
+u64 radix_scratch_entries(u64 capacity) {
+    const u64 h = u64(RADIX) * sort_blocks(capacity);
+    return h + std::max<u64>(1, scan_tmp_entries(h));
+}
 
 void radix_sort_u64(u64* keys, u32* vals, u64 count, int lo_bit, int hi_bit, u64* keys_alt,
                     u32* vals_alt, Workspace& ws, hipStream_t s) {
     if (count <= 1 || hi_bit <= lo_bit) return;
-    if (count >= (u64(1) << 32)) fail(TT_ELENGTH, "radix_sort: too many keys for one sort");
     const u32 nb = sort_blocks(count);
-    u32* hist = ws.get<u32>(u64(RADIX) * nb);
-    u32* stmp = ws.get<u32>(std::max<u64>(1, scan_tmp_entries(u64(RADIX) * nb)));
+    u64* hist = ws.get<u64>(u64(RADIX) * nb);
+    u64* stmp = ws.get<u64>(std::max<u64>(1, scan_tmp_entries(u64(RADIX) * nb)));
     u64 *ks = keys, *kd = keys_alt;
     u32 *vs = vals, *vd = vals_alt;
     for (int shift = lo_bit; shift < hi_bit; shift += 8) {
@@ -367,7 +372,7 @@ void radix_sort_u64(u64* keys, u32* vals, u64 count, int lo_bit, int hi_bit, u64
         const u32 dmask = (1u << dbits) - 1u;
         rs_upsweep<<<nb, RB, 0, s>>>(ks, count, nullptr, u32(shift), dmask, hist);
         TTG_LAUNCH_CHECK();
-        exclusive_scan_buf<u32>(hist, hist, u64(RADIX) * nb, nullptr, stmp, s);
+        exclusive_scan_buf<u64>(hist, hist, u64(RADIX) * nb, nullptr, stmp, s);
         if (vals)
             rs_downsweep<true><<<nb, RB, 0, s>>>(ks, vs, count, nullptr, u32(shift), dmask, dbits, hist, kd, vd);
         else

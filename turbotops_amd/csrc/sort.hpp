@@ -34,23 +34,23 @@ void exclusive_scan_u64(const u64* in, u64* out, u64 count, u64* d_total, Worksp
 
 // Stable ascending sort of keys[0, count) by bits [lo_bit, hi_bit) carrying
 // vals (may be null). Uses keys_alt / vals_alt as ping-pong buffers; the
-// sorted data ends up in keys / vals. count < 2^32.
+// sorted data ends up in keys / vals. Any count.
 void radix_sort_u64(u64* keys, u32* vals, u64 count, int lo_bit, int hi_bit, u64* keys_alt,
                     u32* vals_alt, Workspace& ws, hipStream_t s);
 
 // Capturable forms (no allocation; fixed grids; counts read on the device):
 // temporaries of scan_tmp_entries(count) entries for the scan, and for the
-// sort a histogram of radix_hist_entries(capacity) entries plus
-// scan_tmp_entries(radix_hist_entries(capacity)) scan temporaries. The sort
-// orders keys[0, *d_count) (count <= capacity) by bits [lo_bit, hi_bit) and
-// returns the buffer that holds the result: keys after an even number of
-// 8-bit passes, keys_alt after an odd one (no copy back).
+// sort radix_scratch_entries(capacity) 64-bit entries (the digit histograms
+// and their scan). The sort orders keys[0, *d_count) (count <= capacity <
+// 2^32) by bits [lo_bit, hi_bit) and returns the buffer that holds the
+// result: keys after an even number of 8-bit passes, keys_alt after an odd
+// one (no copy back).
 u64 scan_tmp_entries(u64 count);
-u64 radix_hist_entries(u64 capacity);
+u64 radix_scratch_entries(u64 capacity);
 void exclusive_scan_u32_buf(const u32* in, u32* out, u64 count, u32* d_total, u32* tmp,
                             hipStream_t s);
 u64* radix_sort_u64_counted(u64* keys, u64 capacity, const u32* d_count, int lo_bit, int hi_bit,
-                            u64* keys_alt, u32* hist, u32* scan_tmp, hipStream_t s);
+                            u64* keys_alt, u64* scratch, hipStream_t s);
 
 // Bits needed to represent v (0 for v == 0).
 inline int bit_length(u64 v) { return v ? 64 - __builtin_clzll(v) : 0; }
