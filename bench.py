@@ -224,8 +224,21 @@ def run_graph(args):
         pairs = float(n) * (n - 1) / 2             # lower-triangle tiles
         size = {"edges": int(res.row_ptr[-1]) // 2}
     else:
-        pairs = None
-        size = {"deaths": int(res.size)}
+        # the Boruvka rounds and vertex scans of one call (tt_mst_deaths_dev),
+        # each scan measuring one vertex against n - 1 candidates
+        import ctypes as C
+        import torch
+        from turbotops_amd import capi
+        from turbotops_amd.frontend import parse_metric
+        kind, p = parse_metric("l2")
+        xt = torch.from_numpy(x).cuda()
+        dd = torch.empty(n - 1, dtype=torch.float32, device="cuda")
+        rounds, scans = C.c_uint32(), C.c_uint64()
+        capi.lib().call("tt_mst_deaths_dev", C.c_void_p(xt.data_ptr()), n, d, kind, C.c_float(p),
+                        C.c_void_p(dd.data_ptr()), C.byref(rounds), C.byref(scans), None)
+        torch.cuda.synchronize()
+        pairs = float(scans.value) * (n - 1)
+        size = {"deaths": int(res.size), "rounds": int(rounds.value), "vertex_scans": int(scans.value)}
     out = {"metric": f"{w} graph stage, seconds per call", "value": round(per, 5), "unit": "s",
            "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(per * 1e3, 3),
            "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
