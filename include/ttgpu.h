@@ -311,15 +311,16 @@ tt_status tt_batch_cloud_apparent(tt_batch* b, uint32_t cloud, const uint8_t** a
  * spare: 0, and the ids are plain). */
 tt_status tt_batch_cloud_edges(tt_batch* b, uint32_t cloud, const float** mat, float* threshold,
                                uint64_t* m, const uint32_t** verts, const float** d, int* witness_bit);
-/* Device time of the last run, the autotuned rows per wave (8/16/32), and
- * the kept / non-apparent edge counts (both the kept count in orders 1 and
- * 2). */
+/* Device time of the last run, the autotuned rows per wave (8/16/32; the
+ * tile shape is tuned per dimension group, and this is the one of the group
+ * with the most tiles), and the kept / non-apparent edge counts (both the
+ * kept count in orders 1 and 2). */
 tt_status tt_batch_info(tt_batch* b, float* device_ms, int* rows_per_wave, uint64_t* edges_total,
                         uint64_t* edges_kept);
 /* The distance stage of the batch alone, re-run on the points of the last
- * run with the tuned tile shape and timed with HIP events on the batch's
- * stream (best of two launches after a warm-up), in milliseconds. The
- * matrices it rewrites are bit-identical. */
+ * run with the tuned tile shapes and timed with HIP events on the batch's
+ * stream (the median of five launches after a warm-up), in milliseconds.
+ * The matrices it rewrites are bit-identical. */
 tt_status tt_batch_time_dist(tt_batch* b, float* ms);
 void tt_batch_destroy(tt_batch* b);
 

@@ -122,7 +122,8 @@ class BatchPersistence:
         return self._r.device_ms
 
     def time_distance_stage(self):
-        """The distance stage alone, re-run on the last points (ms)."""
+        """The distance stage alone, re-run on the last points: the median of
+        five launches after a warm-up (ms)."""
         return self._r.time_distance_stage()
 
     def graph(self, cloud):
@@ -148,6 +149,8 @@ class BatchPersistence:
 
     @property
     def rows_per_wave(self):
+        """Tuned rows per wave of the distance tiles (of the dimension group
+        with the most tiles; each group has its own)."""
         return self._r.rows_per_wave
 
 

@@ -33,12 +33,26 @@ __device__ __forceinline__ float wave_min(float v) {
 }
 
 // bmin_row[b] for every block b of one row (whole wave; lane = lane id).
+// Eight blocks per step: their loads are issued together (one 256-byte
+// segment at a time left the pass latency-bound), and lane u stores the
+// minimum of the step's u-th block.
 __device__ __forceinline__ void block_min_row(const float* __restrict__ row, u32 n, u32 nb,
                                               float* __restrict__ bmin_row, u32 lane) {
-    for (u32 b = 0; b < nb; ++b) {
-        const u32 w = b * BLK + lane;
-        const float v = wave_min(w < n ? row[w] : INFINITY);
-        if (lane == 0) bmin_row[b] = v;
+    constexpr u32 STEP = 8;
+    for (u32 b0 = 0; b0 < nb; b0 += STEP) {
+        float v[STEP];
+#pragma unroll
+        for (u32 u = 0; u < STEP; ++u) {
+            const u32 w = (b0 + u) * BLK + lane;
+            v[u] = (b0 + u < nb && w < n) ? row[w] : INFINITY;
+        }
+        float mine = INFINITY;
+#pragma unroll
+        for (u32 u = 0; u < STEP; ++u) {
+            const float m = wave_min(v[u]);
+            if (lane == u) mine = m;
+        }
+        if (lane < STEP && b0 + lane < nb) bmin_row[b0 + lane] = mine;
     }
 }
 

@@ -126,6 +126,8 @@ __global__ __launch_bounds__(256) void batch_radius(BatchShape b, const u32* __r
     }
 }
 
+constexpr u32 BCH = 8;   // row chunks of 64 columns in flight per wave (batch_count, batch_compact)
+
 __global__ __launch_bounds__(256) void batch_count(BatchShape b, const float* __restrict__ mats,
                                                    const float* __restrict__ thr, u32* __restrict__ cnt) {
     const u32 lane = threadIdx.x & 63;
@@ -138,7 +140,18 @@ __global__ __launch_bounds__(256) void batch_count(BatchShape b, const float* __
         const float* row = mats + b.mat_off[c] + u64(i) * row_ld(n);
         const float th = thr[c];
         u32 k = 0;
-        for (u32 j = lane; j < i; j += 64) k += row[j] <= th;
+        // BCH loads of 64 columns issued before any is used (one per step
+        // left the stage latency-bound)
+        for (u32 j0 = 0; j0 < i; j0 += 64 * BCH) {
+            float d[BCH];
+#pragma unroll
+            for (u32 u = 0; u < BCH; ++u) {
+                const u32 j = j0 + 64 * u + lane;
+                d[u] = j < i ? row[j] : 0.0f;
+            }
+#pragma unroll
+            for (u32 u = 0; u < BCH; ++u) k += (j0 + 64 * u + lane < i && d[u] <= th) ? 1u : 0u;
+        }
 #pragma unroll
         for (u32 off = 32; off > 0; off >>= 1) k += __shfl_xor(k, off);
         if (lane == 0) cnt[r] = k;
@@ -172,18 +185,26 @@ __global__ __launch_bounds__(256) void batch_compact(BatchShape b, const float* 
         // diameter) bits alone and keeps it by stability.
         const u64 mirror = u64(off[b.row_base[c]]) + off[b.row_base[c + 1]] - 1;
         const u64 rowbase = u64(i) * (i - 1) / 2;
-        for (u32 j0 = 0; j0 < i; j0 += 64) {
-            const u32 j = j0 + lane;
-            const float d = j < i ? row[j] : INFINITY;
-            const bool keep = j < i && d <= th;
-            const u64 bal = __ballot(keep);
-            if (keep) {
-                const u32 rk = __builtin_amdgcn_mbcnt_hi(u32(bal >> 32), __builtin_amdgcn_mbcnt_lo(u32(bal), 0u));
-                const u64 idx = rowbase + j;
-                const u64 at = comb ? o + rk : mirror - (o + rk);
-                keys[at] = cloud_bits | (u64(__float_as_uint(d)) << ib) | (comb ? idx : M - 1 - idx);
+        for (u32 j0 = 0; j0 < i; j0 += 64 * BCH) {
+            float d[BCH];
+#pragma unroll
+            for (u32 u = 0; u < BCH; ++u) {
+                const u32 j = j0 + 64 * u + lane;
+                d[u] = j < i ? row[j] : 0.0f;
             }
-            o += __popcll(bal);
+#pragma unroll
+            for (u32 u = 0; u < BCH; ++u) {   // chunks in column order
+                const u32 j = j0 + 64 * u + lane;
+                const bool keep = j < i && d[u] <= th;
+                const u64 bal = __ballot(keep);
+                if (keep) {
+                    const u32 rk = __builtin_amdgcn_mbcnt_hi(u32(bal >> 32), __builtin_amdgcn_mbcnt_lo(u32(bal), 0u));
+                    const u64 idx = rowbase + j;
+                    const u64 at = comb ? o + rk : mirror - (o + rk);
+                    keys[at] = cloud_bits | (u64(__float_as_uint(d[u])) << ib) | (comb ? idx : M - 1 - idx);
+                }
+                o += __popcll(bal);
+            }
         }
     }
 }
@@ -396,48 +417,54 @@ __global__ void batch_kept_segments(const u32* __restrict__ seg, const u32* __re
 }
 
 template <int V, int RPW>
-void launch_batch_dist_rpw(const u64* gt, hipStream_t s, const BatchShape& b, MetricArgs ma, const float* pts,
-                           float* mats, u32* rowmax) {
+void launch_batch_dist_group(int g, const u64* gt, hipStream_t s, const BatchShape& b, MetricArgs ma,
+                             const float* pts, float* mats, u32* rowmax) {
     constexpr u32 threads = 64 * (64 / RPW);
-    for (int g = 0; g < kDimGroups; ++g) {
-        const u64 t0 = gt[g], t1 = gt[g + 1];
-        if (t1 <= t0) continue;
-        const u32 grid = u32(t1 - t0);   // one tile per workgroup
-        switch (kGroupDim[g]) {
-            case 2: batch_dist<V, RPW, 2><<<grid, threads, 0, s>>>(b, ma, pts, mats, rowmax, t0, t1); break;
-            case 3: batch_dist<V, RPW, 3><<<grid, threads, 0, s>>>(b, ma, pts, mats, rowmax, t0, t1); break;
-            case 8: batch_dist<V, RPW, 8><<<grid, threads, 0, s>>>(b, ma, pts, mats, rowmax, t0, t1); break;
-            default: batch_dist<V, RPW, 0><<<grid, threads, 0, s>>>(b, ma, pts, mats, rowmax, t0, t1); break;
-        }
+    const u64 t0 = gt[g], t1 = gt[g + 1];
+    if (t1 <= t0) return;
+    const u32 grid = u32(t1 - t0);   // one tile per workgroup
+    switch (kGroupDim[g]) {
+        case 2: batch_dist<V, RPW, 2><<<grid, threads, 0, s>>>(b, ma, pts, mats, rowmax, t0, t1); break;
+        case 3: batch_dist<V, RPW, 3><<<grid, threads, 0, s>>>(b, ma, pts, mats, rowmax, t0, t1); break;
+        case 8: batch_dist<V, RPW, 8><<<grid, threads, 0, s>>>(b, ma, pts, mats, rowmax, t0, t1); break;
+        default: batch_dist<V, RPW, 0><<<grid, threads, 0, s>>>(b, ma, pts, mats, rowmax, t0, t1); break;
     }
 }
 
 template <int V>
-void launch_batch_dist(int rpw, const u64* gt, hipStream_t s, const BatchShape& b, MetricArgs ma,
-                       const float* pts, float* mats, u32* rowmax) {
+void launch_batch_dist_rpw(int g, int rpw, const u64* gt, hipStream_t s, const BatchShape& b, MetricArgs ma,
+                           const float* pts, float* mats, u32* rowmax) {
     switch (rpw) {
-        case 8: launch_batch_dist_rpw<V, 8>(gt, s, b, ma, pts, mats, rowmax); break;
-        case 32: launch_batch_dist_rpw<V, 32>(gt, s, b, ma, pts, mats, rowmax); break;
-        default: launch_batch_dist_rpw<V, 16>(gt, s, b, ma, pts, mats, rowmax); break;
+        case 8: launch_batch_dist_group<V, 8>(g, gt, s, b, ma, pts, mats, rowmax); break;
+        case 32: launch_batch_dist_group<V, 32>(g, gt, s, b, ma, pts, mats, rowmax); break;
+        default: launch_batch_dist_group<V, 16>(g, gt, s, b, ma, pts, mats, rowmax); break;
     }
 }
 
-// gt: kDimGroups + 1 tile-range boundaries, one range per dimension group.
-void launch_batch_dist_v(MetricVariant v, int rpw, const u64* gt, hipStream_t s, const BatchShape& b,
-                         MetricArgs ma, const float* pts, float* mats, u32* rowmax) {
-    switch (v) {
-        case MV_L1: launch_batch_dist<MV_L1>(rpw, gt, s, b, ma, pts, mats, rowmax); break;
-        case MV_L2: launch_batch_dist<MV_L2>(rpw, gt, s, b, ma, pts, mats, rowmax); break;
-        case MV_LINF: launch_batch_dist<MV_LINF>(rpw, gt, s, b, ma, pts, mats, rowmax); break;
-        case MV_LP_INT: launch_batch_dist<MV_LP_INT>(rpw, gt, s, b, ma, pts, mats, rowmax); break;
-        default: launch_batch_dist<MV_LP_POW>(rpw, gt, s, b, ma, pts, mats, rowmax); break;
+// One dimension group g (its tile range gt[g], gt[g + 1]) with `rpw` rows per
+// wave; g < 0 launches every group, group h with rpw_g[h].
+void launch_batch_dist_v(MetricVariant v, int g, int rpw, const int* rpw_g, const u64* gt, hipStream_t s,
+                         const BatchShape& b, MetricArgs ma, const float* pts, float* mats, u32* rowmax) {
+    for (int h = (g < 0 ? 0 : g); h < (g < 0 ? kDimGroups : g + 1); ++h) {
+        const int r = g < 0 ? rpw_g[h] : rpw;
+        switch (v) {
+            case MV_L1: launch_batch_dist_rpw<MV_L1>(h, r, gt, s, b, ma, pts, mats, rowmax); break;
+            case MV_L2: launch_batch_dist_rpw<MV_L2>(h, r, gt, s, b, ma, pts, mats, rowmax); break;
+            case MV_LINF: launch_batch_dist_rpw<MV_LINF>(h, r, gt, s, b, ma, pts, mats, rowmax); break;
+            case MV_LP_INT: launch_batch_dist_rpw<MV_LP_INT>(h, r, gt, s, b, ma, pts, mats, rowmax); break;
+            default: launch_batch_dist_rpw<MV_LP_POW>(h, r, gt, s, b, ma, pts, mats, rowmax); break;
+        }
     }
 }
 
-// Autotuned tile shape per (metric variant, dimension set): the fastest
-// rows-per-wave of {8, 16, 32} measured on the batch itself.
+// Autotuned tile shape per (metric variant, dimension set) and dimension
+// group: the fastest rows-per-wave of {8, 16, 32} for each group's launch,
+// measured on the batch itself.
+struct TileShapes {
+    int rpw[kDimGroups];
+};
 std::mutex g_tune_mu;
-std::map<std::pair<int, u64>, int> g_tune;
+std::map<std::pair<int, u64>, TileShapes> g_tune;
 
 }  // namespace
 
@@ -447,7 +474,8 @@ struct BatchState {
     float p = 2.0f, threshold = INFINITY;
     MetricVariant variant = MV_L2;
     MetricArgs ma{};
-    int rpw = 16, ib = 1, cb = 0;
+    int rpw_g[kDimGroups] = {16, 16, 16, 16};   // rows per wave of each dimension group's tiles
+    int ib = 1, cb = 0;
     bool tuned = false;
     bool comb = false;   // hand over every kept edge in (diameter, comb index) order
     bool witness = false;   // ... with the ready-witness flags (asked for, and the keys have a bit to spare)
@@ -498,10 +526,19 @@ struct BatchState {
     }
 
     // the device part of one batch (recorded into the graph)
-    void enqueue(int rpw_use) {
+    // rows per wave of the group with the most tiles (reported)
+    int rpw_main() const {
+        int g0 = 0;
+        for (int g = 1; g < kDimGroups; ++g)
+            if (group_tiles[g + 1] - group_tiles[g] > group_tiles[g0 + 1] - group_tiles[g0]) g0 = g;
+        return rpw_g[g0];
+    }
+
+    void enqueue() {
         hipStream_t s = stream;
         fill_device_u32(d_rowmax, std::max<u64>(total_rows, 1), 0u, s);
-        if (total_tiles) launch_batch_dist_v(variant, rpw_use, group_tiles, s, shape, ma, d_pts, d_mats, d_rowmax);
+        if (total_tiles)
+            launch_batch_dist_v(variant, -1, 0, rpw_g, group_tiles, s, shape, ma, d_pts, d_mats, d_rowmax);
         TTG_LAUNCH_CHECK();
         batch_radius<<<nclouds, 256, 0, s>>>(shape, d_rowmax, threshold, d_thr);
         fill_device_u32(d_cnt, total_rows + 1, 0u, s);
@@ -691,8 +728,10 @@ void batch_setup(BatchState& b, u32 nclouds, const uint32_t* n, const uint32_t* 
     b.thr.resize(nclouds);
 }
 
-// Times each tile shape on this batch (points already on the device) and
-// keeps the fastest for this (metric variant, dimension set).
+// Times each tile shape on each dimension group of this batch (points already
+// on the device): one warm-up launch, then the fastest of three, per shape;
+// keeps the fastest shape of every group for this (metric variant,
+// dimension set). Results never depend on the shapes.
 void batch_autotune(BatchState& b) {
     u64 dimkey = 0;
     for (u32 d : b.dim) dimkey |= u64(1) << std::min<u32>(d, 63);
@@ -701,34 +740,37 @@ void batch_autotune(BatchState& b) {
         std::lock_guard<std::mutex> lk(g_tune_mu);
         auto it = g_tune.find(key);
         if (it != g_tune.end()) {
-            b.rpw = it->second;
+            std::copy(it->second.rpw, it->second.rpw + kDimGroups, b.rpw_g);
             b.tuned = true;
             return;
         }
     }
-    float best = 1e30f;
-    int best_rpw = 16;
-    for (int rpw : {8, 16, 32}) {
-        for (int rep = 0; rep < 2; ++rep) {   // first launch warms up
-            fill_device_u32(b.d_rowmax, std::max<u64>(b.total_rows, 1), 0u, b.stream);
-            TTG_HIP(hipEventRecord(b.t0, b.stream));
-            if (b.total_tiles)
-                launch_batch_dist_v(b.variant, rpw, b.group_tiles, b.stream, b.shape, b.ma, b.d_pts, b.d_mats,
-                                    b.d_rowmax);
-            TTG_HIP(hipEventRecord(b.t1, b.stream));
-            TTG_HIP(hipEventSynchronize(b.t1));
-            float ms = 0;
-            TTG_HIP(hipEventElapsedTime(&ms, b.t0, b.t1));
-            if (rep == 1 && ms < best) {
-                best = ms;
-                best_rpw = rpw;
+    TileShapes ts{};
+    fill_device_u32(b.d_rowmax, std::max<u64>(b.total_rows, 1), 0u, b.stream);
+    for (int g = 0; g < kDimGroups; ++g) {
+        ts.rpw[g] = 16;
+        if (b.group_tiles[g + 1] <= b.group_tiles[g]) continue;
+        float best = 1e30f;
+        for (int rpw : {8, 16, 32}) {
+            for (int rep = 0; rep < 4; ++rep) {   // the first launch warms up
+                TTG_HIP(hipEventRecord(b.t0, b.stream));
+                launch_batch_dist_v(b.variant, g, rpw, nullptr, b.group_tiles, b.stream, b.shape, b.ma, b.d_pts,
+                                    b.d_mats, b.d_rowmax);
+                TTG_HIP(hipEventRecord(b.t1, b.stream));
+                TTG_HIP(hipEventSynchronize(b.t1));
+                float ms = 0;
+                TTG_HIP(hipEventElapsedTime(&ms, b.t0, b.t1));
+                if (rep > 0 && ms < best) {
+                    best = ms;
+                    ts.rpw[g] = rpw;
+                }
             }
         }
     }
-    b.rpw = best_rpw;
+    std::copy(ts.rpw, ts.rpw + kDimGroups, b.rpw_g);
     b.tuned = true;
     std::lock_guard<std::mutex> lk(g_tune_mu);
-    g_tune[key] = best_rpw;
+    g_tune[key] = ts;
 }
 
 void batch_run(BatchState& b, const float* pts, int use_graph) {
@@ -742,13 +784,13 @@ void batch_run(BatchState& b, const float* pts, int use_graph) {
     if (use_graph) {
         if (!b.exec) {
             TTG_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-            b.enqueue(b.rpw);
+            b.enqueue();
             TTG_HIP(hipStreamEndCapture(s, &b.graph));
             TTG_HIP(hipGraphInstantiate(&b.exec, b.graph, nullptr, nullptr, 0));
         }
         TTG_HIP(hipGraphLaunch(b.exec, s));
     } else {
-        b.enqueue(b.rpw);
+        b.enqueue();
     }
     TTG_HIP(hipEventRecord(b.t1, s));
     // sizes of the data-dependent outputs
@@ -892,7 +934,7 @@ tt_status tt_batch_info(tt_batch* b, float* device_ms, int* rows_per_wave, uint6
     return guarded([&] {
         if (!b) fail(TT_EINVAL, "tt_batch_info: null handle");
         if (device_ms) *device_ms = b->st.last_ms;
-        if (rows_per_wave) *rows_per_wave = b->st.rpw;
+        if (rows_per_wave) *rows_per_wave = b->st.rpw_main();
         if (edges_total || edges_kept) {
             u32 total = 0;
             TTG_HIP(hipMemcpy(&total, b->st.d_total, 4, hipMemcpyDeviceToHost));
@@ -907,20 +949,23 @@ tt_status tt_batch_time_dist(tt_batch* b, float* ms) {
         if (!b || !ms) fail(TT_EINVAL, "tt_batch_time_dist: null argument");
         BatchState& st = b->st;
         if (!st.tuned) fail(TT_EINVAL, "tt_batch_time_dist: run the batch first");
-        float best = 1e30f;
-        for (int rep = 0; rep < 3; ++rep) {   // the first launch warms up
+        // one warm-up, then the median of five timed launches of the stage
+        // (every dimension group, each with its tuned tile shape)
+        std::vector<float> t;
+        for (int rep = 0; rep < 6; ++rep) {
             fill_device_u32(st.d_rowmax, std::max<u64>(st.total_rows, 1), 0u, st.stream);
             TTG_HIP(hipEventRecord(st.t0, st.stream));
             if (st.total_tiles)
-                launch_batch_dist_v(st.variant, st.rpw, st.group_tiles, st.stream, st.shape, st.ma, st.d_pts,
+                launch_batch_dist_v(st.variant, -1, 0, st.rpw_g, st.group_tiles, st.stream, st.shape, st.ma, st.d_pts,
                                     st.d_mats, st.d_rowmax);
             TTG_HIP(hipEventRecord(st.t1, st.stream));
             TTG_HIP(hipEventSynchronize(st.t1));
-            float t = 0;
-            TTG_HIP(hipEventElapsedTime(&t, st.t0, st.t1));
-            if (rep) best = std::min(best, t);
+            float ms1 = 0;
+            TTG_HIP(hipEventElapsedTime(&ms1, st.t0, st.t1));
+            if (rep) t.push_back(ms1);
         }
-        *ms = best;
+        std::sort(t.begin(), t.end());
+        *ms = t[t.size() / 2];
     });
 }
 

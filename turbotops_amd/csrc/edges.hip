@@ -49,6 +49,15 @@ __device__ __forceinline__ void decode_edge(u64 idx, u32& i, u32& j) {
     j = u32(idx - v * (v - 1) / 2);
 }
 
+// Rows are taken longest first (row i holds i kept-edge candidates), so the
+// last waves of the grid get the short rows. Each step of a wave issues
+// ECH loads of 64 columns before it uses any, so a wave keeps ECH x 256
+// bytes in flight instead of one segment (one segment per wave left the
+// kernel latency-bound at a fifth of the read rate).
+constexpr u32 ECH = 8;
+
+__device__ __forceinline__ u32 row_longest_first(u32 r, u32 n) { return n - 1 - r; }
+
 __global__ __launch_bounds__(EB) void edge_count(const float* __restrict__ mat, u32 n, float thr,
                                                  u64* __restrict__ cnt, u32* __restrict__ mm) {
     // one wave per row, rows grid-strided
@@ -56,16 +65,25 @@ __global__ __launch_bounds__(EB) void edge_count(const float* __restrict__ mat, 
     const u32 gw = (blockIdx.x * EB + threadIdx.x) >> 6;
     const u32 nw = (gridDim.x * EB) >> 6;
     u32 lo = 0xffffffffu, hi = 0u;
-    for (u32 i = gw; i < n; i += nw) {
+    for (u32 r = gw; r < n; r += nw) {
+        const u32 i = row_longest_first(r, n);
         const float* __restrict__ row = mat + u64(i) * n;
         u32 c = 0;
-        for (u32 j = lane; j < i; j += 64) {
-            const float d = row[j];
-            if (d <= thr) {
-                ++c;
-                const u32 b = __float_as_uint(d);
-                lo = min(lo, b);
-                hi = max(hi, b);
+        for (u32 j0 = 0; j0 < i; j0 += 64 * ECH) {
+            float d[ECH];
+#pragma unroll
+            for (u32 u = 0; u < ECH; ++u) {
+                const u32 j = j0 + 64 * u + lane;
+                d[u] = j < i ? row[j] : 0.0f;
+            }
+#pragma unroll
+            for (u32 u = 0; u < ECH; ++u) {
+                if (j0 + 64 * u + lane < i && d[u] <= thr) {
+                    ++c;
+                    const u32 b = __float_as_uint(d[u]);
+                    lo = min(lo, b);
+                    hi = max(hi, b);
+                }
             }
         }
 #pragma unroll
@@ -77,9 +95,15 @@ __global__ __launch_bounds__(EB) void edge_count(const float* __restrict__ mat, 
         lo = min(lo, __shfl_xor(lo, off));
         hi = max(hi, __shfl_xor(hi, off));
     }
+    // Every wave folding its extremes into the same two words serialised
+    // 4e4 atomics on one cache line (half of the kernel's time at n = 2e4):
+    // a wave now skips the atomic unless its value improves on what it reads
+    // (a stale read only costs a redundant atomic; min and max are exact).
     if (lane == 0) {
-        atomicMin(mm, lo);
-        atomicMax(mm + 1, hi);
+        const u32 cur_lo = __hip_atomic_load(mm, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const u32 cur_hi = __hip_atomic_load(mm + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (lo < cur_lo) atomicMin(mm, lo);
+        if (hi > cur_hi) atomicMax(mm + 1, hi);
     }
 }
 
@@ -93,32 +117,48 @@ __global__ __launch_bounds__(EB) void edge_compact(const float* __restrict__ mat
     const u32 lane = threadIdx.x & 63;
     const u32 gw = (blockIdx.x * EB + threadIdx.x) >> 6;
     const u32 nw = (gridDim.x * EB) >> 6;
-    for (u32 i = gw; i < n; i += nw) {
+    for (u32 r = gw; r < n; r += nw) {
+        const u32 i = row_longest_first(r, n);
         const float* __restrict__ row = mat + u64(i) * n;
         u64 o = off[i];
         const u64 rowbase = u64(i) * (i - 1) / 2;
-        for (u32 j0 = 0; j0 < i; j0 += 64) {
-            const u32 j = j0 + lane;
-            const float d = j < i ? row[j] : INFINITY;
-            const bool keep = j < i && d <= thr;
-            const u64 b = __ballot(keep);
-            if (keep) {
-                const u32 r = __builtin_amdgcn_mbcnt_hi(u32(b >> 32), __builtin_amdgcn_mbcnt_lo(u32(b), 0u));
-                const u64 idx = rowbase + j;
-                const u64 at = comb ? o + r : m - 1 - (o + r);
-                keys[at] = (u64(__float_as_uint(d) - dmin) << ib) | (comb ? idx : M - 1 - idx);
+        for (u32 j0 = 0; j0 < i; j0 += 64 * ECH) {
+            float d[ECH];
+#pragma unroll
+            for (u32 u = 0; u < ECH; ++u) {
+                const u32 j = j0 + 64 * u + lane;
+                d[u] = j < i ? row[j] : 0.0f;
             }
-            o += __popcll(b);
+            // chunks in column order: the keys of a row stay in ascending j
+#pragma unroll
+            for (u32 u = 0; u < ECH; ++u) {
+                const u32 j = j0 + 64 * u + lane;
+                const bool keep = j < i && d[u] <= thr;
+                const u64 b = __ballot(keep);
+                if (keep) {
+                    const u32 rk = __builtin_amdgcn_mbcnt_hi(u32(b >> 32), __builtin_amdgcn_mbcnt_lo(u32(b), 0u));
+                    const u64 idx = rowbase + j;
+                    const u64 at = comb ? o + rk : m - 1 - (o + rk);
+                    keys[at] = (u64(__float_as_uint(d[u]) - dmin) << ib) | (comb ? idx : M - 1 - idx);
+                }
+                o += __popcll(b);
+            }
         }
     }
 }
 
+// Sorted keys -> (idx, d), and the apparent / witness flags where asked for:
+// from bit 0 of each key (fb = 1), or from the payload sorted alongside
+// (`sorted`). One pass over the keys.
 __global__ __launch_bounds__(EB) void edge_unpack(const u64* __restrict__ keys, u64 m, u32 dmin,
-                                                  int ib, u64 M, bool comb, u64* __restrict__ idx,
-                                                  float* __restrict__ d) {
+                                                  int ib, u64 M, bool comb, int fb,
+                                                  const u32* __restrict__ sorted, u8* __restrict__ flags,
+                                                  u64* __restrict__ idx, float* __restrict__ d) {
     const u64 mask = ib >= 64 ? ~0ull : ((u64(1) << ib) - 1);
     for (u64 t = blockIdx.x * u64(EB) + threadIdx.x; t < m; t += u64(gridDim.x) * EB) {
-        const u64 k = keys[t];
+        u64 k = keys[t];
+        if (flags) flags[t] = sorted ? u8(sorted[t]) : u8(k & 1);
+        k >>= fb;
         if (idx) idx[t] = comb ? (k & mask) : M - 1 - (k & mask);
         if (d) d[t] = __uint_as_float(u32(k >> ib) + dmin);
     }
@@ -205,21 +245,6 @@ __global__ __launch_bounds__(EB) void witness_scan(const float* __restrict__ mat
     }
 }
 
-// Sorted keys carrying the apparent flag in bit 0 -> flags; with `sorted`
-// (the flags sorted alongside as a payload) the keys hold no flag bit.
-__global__ __launch_bounds__(EB) void edge_flags(u64* __restrict__ keys, u64 m, const u32* __restrict__ sorted,
-                                                 u8* __restrict__ flags) {
-    for (u64 t = blockIdx.x * u64(EB) + threadIdx.x; t < m; t += u64(gridDim.x) * EB) {
-        if (sorted) {
-            flags[t] = u8(sorted[t]);
-            continue;
-        }
-        const u64 k = keys[t];
-        flags[t] = u8(k & 1);
-        keys[t] = k >> 1;
-    }
-}
-
 u32 grid_for(u64 waves) {
     return u32(std::max<u64>(1, std::min<u64>((waves + 3) / 4, 256 * 64)));
 }
@@ -275,14 +300,10 @@ DenseEdges dense_edges(const float* d_mat, u32 n, float thr, bool want_apparent,
     }
     radix_sort_u64(keys, fl, out.m, ib + fb, ib + db + fb, alt, fl_alt, ws, s);
     const u32 ug = u32(std::min<u64>((out.m + EB - 1) / EB, 65536));
-    if (want_apparent) {
-        out.apparent = ws.get<u8>(out.m);
-        edge_flags<<<ug, EB, 0, s>>>(keys, out.m, fl, out.apparent);
-        TTG_LAUNCH_CHECK();
-    }
+    if (want_apparent) out.apparent = ws.get<u8>(out.m);
     out.idx = alt;   // reuse the ping-pong buffer for the unpacked indices
     out.d = ws.get<float>(out.m);
-    edge_unpack<<<ug, EB, 0, s>>>(keys, out.m, dmin, ib, M, comb, out.idx, out.d);
+    edge_unpack<<<ug, EB, 0, s>>>(keys, out.m, dmin, ib, M, comb, fb, fl, out.apparent, out.idx, out.d);
     TTG_LAUNCH_CHECK();
     return out;
 }
