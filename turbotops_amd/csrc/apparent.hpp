@@ -101,16 +101,32 @@ __device__ __forceinline__ bool edges_apparent_lanes(const float* __restrict__ m
                                                      u32 n, u32 ld, u32 nb, bool valid, u32 i, u32 j,
                                                      float d, u32 lane) {
     bool done = !valid, ap = false;
-    for (u32 k = 0; k < TOPK; ++k) {
-        if (!done) {
-            if (k >= n) {
-                done = true;                         // every w examined: no cofacet
-            } else {
-                const u32 w = n - 1 - k;
-                if (w != i && w != j) {
-                    const float a = mat[u64(i) * ld + w];
-                    const float c = mat[u64(w) * ld + j];
-                    if (fmax_ref(a, c) <= d) {
+    // Phase 1 in steps of P1 candidates: a lane loads the step's d(i, w) and
+    // d(j, w) together, then examines them in descending w, as the one-by-one
+    // walk does (one dependent load pair per candidate left the walk bound by
+    // cache latency).
+    constexpr u32 P1 = 8;
+    static_assert(TOPK % P1 == 0, "phase 1 runs in whole steps");
+    for (u32 k0 = 0; k0 < TOPK; k0 += P1) {
+        float av[P1], cv[P1];
+#pragma unroll
+        for (u32 u = 0; u < P1; ++u) {
+            const u32 k = k0 + u;
+            const u32 w = n - 1 - k;
+            const bool in = !done && k < n && w != i && w != j;
+            av[u] = in ? mat[u64(i) * ld + w] : INFINITY;
+            cv[u] = in ? mat[u64(w) * ld + j] : INFINITY;
+        }
+#pragma unroll
+        for (u32 u = 0; u < P1; ++u) {
+            const u32 k = k0 + u;
+            if (!done) {
+                if (k >= n) {
+                    done = true;                         // every w examined: no cofacet
+                } else {
+                    const u32 w = n - 1 - k;
+                    const float a = av[u], c = cv[u];
+                    if (w != i && w != j && fmax_ref(a, c) <= d) {
                         if constexpr (WITNESS) {
                             if (witness_rule(w, j, i, d, c, a)) {
                                 ap = true;
