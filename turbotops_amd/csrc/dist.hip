@@ -66,7 +66,18 @@ __global__ __launch_bounds__(1024) void min_rowmax(const u32* __restrict__ rowma
                                                    float* __restrict__ out) {
     __shared__ u32 red[1024 / 64];
     u32 m = 0x7f800000u;   // +inf
-    for (u32 t = threadIdx.x; t < n; t += blockDim.x) m = min(m, rowmax[t]);
+    // eight loads in flight per thread (one at a time left this one-block
+    // kernel waiting on each)
+    for (u32 t0 = threadIdx.x; t0 < n; t0 += 8 * blockDim.x) {
+        u32 v[8];
+#pragma unroll
+        for (u32 u = 0; u < 8; ++u) {
+            const u32 t = t0 + u * blockDim.x;
+            v[u] = t < n ? rowmax[t] : 0x7f800000u;
+        }
+#pragma unroll
+        for (u32 u = 0; u < 8; ++u) m = min(m, v[u]);
+    }
     for (int off = 32; off > 0; off >>= 1) m = min(m, __shfl_xor(m, off));
     if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
     __syncthreads();
