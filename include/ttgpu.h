@@ -332,8 +332,10 @@ void tt_batch_destroy(tt_batch* b);
  * symmetric union as a CSR graph, rows sorted by neighbour id, weights
  * recomputed as c(row, col) (knn_graph, graph.hpp:286-304), in one launch
  * sequence recorded into a HIP graph on the first run. Each cloud's graph
- * equals tt_knn_graph_exact on that cloud alone, bit for bit. 1 <= k <= 64
- * (tt_knn_graph_exact takes any k). No reference counterpart: the reference
+ * equals tt_knn_graph_exact on that cloud alone, bit for bit. Any k >= 1: a
+ * cloud whose list is longer than 64 takes the single-cloud selection path,
+ * and a batch holding such a cloud runs its sequence directly (use_graph is
+ * then ignored). No reference counterpart: the reference
  * has no batch API; its per-cloud path is persistence(..., graph = knn)
  * (turbotops.hpp:275-277). */
 typedef struct tt_batch_knn tt_batch_knn;

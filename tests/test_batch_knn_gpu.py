@@ -34,8 +34,11 @@ def _clouds(seed, count=16, nmax=600, dims=(2, 3, 8, 5)):
     return out
 
 
-@pytest.mark.parametrize("metric,k", [("l2", 15), ("l1", 8), ("linf", 33), ("lp:1.5", 5), ("lp:3", 64)])
+@pytest.mark.parametrize("metric,k", [("l2", 15), ("l1", 8), ("linf", 33), ("lp:1.5", 5), ("lp:3", 64),
+                                      ("l2", 100), ("linf", 65), ("l1", 300)])
 def test_graphs_equal_single_cloud(gpu, metric, k):
+    """k above 64: clouds with more than k + 1 points take the selection path,
+    smaller ones (k_c = n_c - 1 <= 64) the register-list scan, in one batch."""
     from turbotops_amd import BatchPersistence, knn_graph_exact, knn_lists_exact
     clouds = _clouds(31, count=14, nmax=500)
     print(f"case batch knn metric={metric} k={k} clouds={len(clouds)}", flush=True)
@@ -78,6 +81,20 @@ def test_diagrams_vs_reference(gpu, reference):
         assert canon(dgms) == canon(want)
 
 
+def test_large_k_diagrams_and_reruns(gpu):
+    """A batch with lists longer than 64 runs its sequence directly (use_graph
+    is ignored) and gives the single-cloud diagrams on every run."""
+    from turbotops_amd import BatchPersistence, persistence
+    clouds = _clouds(35, count=8, nmax=400)
+    print(f"case batch knn large k clouds={len(clouds)}", flush=True)
+    bp = BatchPersistence([c.shape for c in clouds], graph="knn", k=90, maxdim=1)
+    for rep in range(2):
+        res = bp(clouds, use_graph=True)
+        for c, (dgms, _) in zip(clouds, res):
+            want, _ = persistence(c, maxdim=1, graph="knn", k=90)
+            assert canon(dgms) == canon(want)
+
+
 def test_replays_with_new_points(gpu):
     """Replays of the recorded graph on new data of the same shapes, with eager
     library work in between, each checked against the single-cloud graphs."""
@@ -100,8 +117,6 @@ def test_rejects(gpu):
     shapes = [(10, 3), (20, 2)]
     with pytest.raises(ValueError, match="k must be >= 1"):
         BatchPersistence(shapes, graph="knn", k=0)
-    with pytest.raises(ValueError, match="k must be <= 64"):
-        BatchPersistence(shapes, graph="knn", k=65)
     with pytest.raises(ValueError, match="graph must be"):
         BatchPersistence(shapes, graph="radius")
     bp = BatchPersistence(shapes, graph="knn", k=4)

@@ -85,7 +85,9 @@ class BatchPersistence:
 
     graph="knn": each cloud's flag complex of its symmetric exact k-NN graph
     instead (the kNN stage of the batch: every cloud's lists and graph in one
-    device sequence, 1 <= k <= 64), each result equal to
+    device sequence; any k >= 1, lists longer than 64 through the
+    single-cloud selection path, run without the recorded graph), each result
+    equal to
     persistence(cloud, maxdim=maxdim, metric=metric, graph="knn", k=k);
     graph(c) and lists(c) give cloud c's graph and lists after a call."""
 

@@ -18,7 +18,11 @@
 //   scan        row offsets over every cloud's rows at once
 // Fixed grids and counts known at creation (every cloud's list length is
 // min(k, n_c - 1)), so the sequence replays from a graph; the host then
-// copies the offsets, ids and weights once.
+// copies the offsets, ids and weights once. Clouds whose list is longer than
+// 64 (k > 64 and n_c > 65) take the single-cloud selection path
+// (knn_large_k) in place of the register-list scan, one cloud after another;
+// its temporaries are allocated per run, so a batch holding such a cloud runs
+// its sequence directly instead of from the recorded graph.
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -181,6 +185,10 @@ u32 grid_for(u64 count, u32 per = 256) {
 
 struct BatchKnnState {
     u32 nclouds = 0, k = 0, kmax = 8;
+    int kind = 1;
+    float p = 2.0f;
+    std::vector<u32> large;   // clouds with a list longer than 64: the selection path
+    Workspace ws_large;       // its temporaries, for one run
     int rb = 1, cb = 0;
     MetricVariant variant = MV_L2;
     MetricArgs ma{};
@@ -223,6 +231,9 @@ struct BatchKnnState {
         }
         launch_scan(variant, kmax, gq.data(), s, shape, ma, d_pts, d_qb_cloud, d_qb_first, d_ids);
         TTG_LAUNCH_CHECK();
+        for (u32 c : large)
+            knn_large_k(d_pts + pt_off[c], n[c], dim[c], kind, p, std::min(k, n[c] - 1), d_ids + list_off[c],
+                        ws_large, s);
         bknn_keys<<<grid_for(total_rows), 256, 0, s>>>(shape, d_ids, rb, d_keys);
         TTG_LAUNCH_CHECK();
         const u64* sorted = radix_sort_u64_counted(d_keys, m, d_count, 0, cb + 2 * rb, d_alt, d_sort, s);
@@ -243,9 +254,10 @@ void bknn_setup(BatchKnnState& b, u32 nclouds, const uint32_t* n, const uint32_t
     if (nclouds == 0) fail(TT_EINVAL, "tt_batch_knn_create: no clouds");
     if (!n || !dim) fail(TT_EINVAL, "tt_batch_knn_create: null shapes");
     if (k == 0) fail(TT_EINVAL, "persistence: k must be >= 1");
-    if (k > 64) fail(TT_EINVAL, "tt_batch_knn_create: k must be <= 64 in a batch (tt_knn_graph_exact takes any k)");
     b.nclouds = nclouds;
     b.k = k;
+    b.kind = kind;
+    b.p = p;
     b.kmax = k <= 8 ? 8 : k <= 16 ? 16 : k <= 32 ? 32 : 64;
     b.variant = variant_of(kind, p);
     b.ma = metric_args(kind, p);
@@ -272,10 +284,17 @@ void bknn_setup(BatchKnnState& b, u32 nclouds, const uint32_t* n, const uint32_t
     b.cb = bit_length(u64(nclouds - 1));
     if (b.cb + 2 * b.rb > 64) fail(TT_ELENGTH, "tt_batch_knn_create: graph keys exceed 64 bits");
 
-    // query blocks, grouped by dimension (cloud-major within a group)
+    // query blocks of the register-list scan, grouped by dimension
+    // (cloud-major within a group); longer lists go to the selection path
     std::vector<u32> qc[kDimGroups], qf[kDimGroups];
     for (u32 c = 0; c < nclouds; ++c) {
         if (b.n[c] < 2) continue;
+        if (std::min(k, b.n[c] - 1) > 64) {
+            if (u64(b.n[c]) * (b.n[c] - 1) >= (u64(1) << 32))
+                fail(TT_ELENGTH, "knn: n * k must be below 2^32");
+            b.large.push_back(c);
+            continue;
+        }
         const int g = dim_group(b.dim[c]);
         for (u32 q = 0; q < b.n[c]; q += QB) {
             qc[g].push_back(c);
@@ -348,8 +367,9 @@ void bknn_run(BatchKnnState& b, const float* pts, int use_graph) {
         std::memcpy(b.h_pts, pts, b.total_pts * 4);
         TTG_HIP(hipMemcpyAsync(b.d_pts, b.h_pts, b.total_pts * 4, hipMemcpyHostToDevice, s));
     }
+    b.ws_large.release();   // the previous run has finished: its temporaries go
     TTG_HIP(hipEventRecord(b.t0, s));
-    if (use_graph) {
+    if (use_graph && b.large.empty()) {
         if (!b.exec) {
             TTG_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
             b.enqueue();

@@ -35,6 +35,12 @@ DenseEdges dense_edges(const float* d_mat, u32 n, float thr, bool want_flags, Wo
 // 31 of each larger id carries the edge's flag.
 void edge_vertices(const u64* d_idx, u64 m, int32_t* d_verts, hipStream_t s, const u8* d_flags = nullptr);
 
+// graph.hip: the kNN lists of one cloud for k = kk > 64 (kk <= n - 1): one
+// workgroup per query selects its kk best (distance, id) pairs, then two
+// stable sorts order them; temporaries come from ws
+void knn_large_k(const float* d_x, u32 n, u32 dim, int kind, float p, u32 kk, u32* d_ids, Workspace& ws,
+                 hipStream_t s);
+
 // sparse.hip
 struct SparseEdges {
     u64 m = 0;

@@ -521,8 +521,8 @@ DevCsr radius_graph_dev(const float* d_x, u32 n, u32 dim, int kind, float p, flo
 
 // ---- kNN --------------------------------------------------------------------
 // k > 64: per-query selection, then the lists ordered by (query, distance, id)
-static void knn_large_k(const float* d_x, u32 n, u32 dim, int kind, float p, u32 kk, u32* d_ids, Workspace& ws,
-                        hipStream_t s) {
+void knn_large_k(const float* d_x, u32 n, u32 dim, int kind, float p, u32 kk, u32* d_ids, Workspace& ws,
+                 hipStream_t s) {
     const u64 total = u64(n) * kk;
     if (total >= (u64(1) << 32)) fail(TT_ELENGTH, "knn: n * k must be below 2^32");
     const int jb = std::max(1, bit_length(n - 1));

@@ -13,8 +13,14 @@ namespace ttg {
 // A grow-only bag of device allocations released together.
 struct Workspace {
     std::vector<void*> ptrs;
-    ~Workspace() {
+    Workspace() = default;
+    Workspace(const Workspace&) = delete;
+    Workspace& operator=(const Workspace&) = delete;
+    ~Workspace() { release(); }
+    // frees every allocation (the work that used them must have finished)
+    void release() {
         for (void* p : ptrs) (void)hipFree(p);
+        ptrs.clear();
     }
     template <typename T>
     T* get(u64 count) {
