@@ -289,9 +289,9 @@ void bknn_setup(BatchKnnState& b, u32 nclouds, const uint32_t* n, const uint32_t
     std::vector<u32> qc[kDimGroups], qf[kDimGroups];
     for (u32 c = 0; c < nclouds; ++c) {
         if (b.n[c] < 2) continue;
-        if (std::min(k, b.n[c] - 1) > 64) {
-            if (u64(b.n[c]) * (b.n[c] - 1) >= (u64(1) << 32))
-                fail(TT_ELENGTH, "knn: n * k must be below 2^32");
+        const u32 kc = std::min(k, b.n[c] - 1);
+        if (kc > 64) {
+            if (u64(b.n[c]) * kc >= (u64(1) << 32)) fail(TT_ELENGTH, "knn: n * k must be below 2^32");
             b.large.push_back(c);
             continue;
         }
