@@ -11,8 +11,10 @@ front-end throughput in GB/s of algorithmic bytes (4 n^2 matrix bytes
 written + 4 n d point bytes read per step).
 
 Prints ONE JSON line. Besides the contract keys it carries:
-  roofline        the dominant kernel's achieved algorithmic GB/s (HIP events on
-                  the stream the kernel is launched on) against the 8 TB/s HBM peak
+  roofline        the dominant kernel's achieved algorithmic GB/s (HIP events that
+                  the library records around the distance-tile launch alone, on
+                  the stream it is launched on) against the 8 TB/s HBM peak;
+                  step_frac: the same bytes over the whole step's events
   cpu_baseline    the reference's own make_dense + enclosing_radius (compiled
                   from its sources into oracle/_ref, SIMD build, all allotted
                   host cores) on the same cloud
@@ -102,22 +104,33 @@ def run_dense(args):
     radius = torch.empty(1, dtype=torch.float32, device=dev)
     stream = torch.cuda.current_stream(dev)
 
-    def step():
-        dist_matrix_dev(xt, metric, "full", out=mat, rowmax=rowmax, radius=radius, stream=stream.cuda_stream)
+    # per step: events around the whole step (row-maxima fill, distance
+    # tiles, radius reduction) and, recorded by the library on the same
+    # stream, around the distance-tile kernel alone
+    starts = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
+    ends = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
+    kbeg = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
+    kend = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
+
+    def step(k=None):
+        ev = None if k is None else (kbeg[k], kend[k])
+        dist_matrix_dev(xt, metric, "full", out=mat, rowmax=rowmax, radius=radius, stream=stream, events=ev)
 
     for _ in range(args.warmup):
         step()
+    for k in range(args.steps):   # torch creates an event's handle on its first record
+        for e in (starts[k], ends[k], kbeg[k], kend[k]):
+            e.record(stream)
     torch.cuda.synchronize()
-    starts = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
-    ends = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
     t0 = time.perf_counter()
     for k in range(args.steps):
         starts[k].record(stream)
-        step()
+        step(k)
         ends[k].record(stream)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
-    kern_ms = sum(s.elapsed_time(e) for s, e in zip(starts, ends)) / args.steps
+    step_ms = sum(s.elapsed_time(e) for s, e in zip(starts, ends)) / args.steps
+    kern_ms = sum(s.elapsed_time(e) for s, e in zip(kbeg, kend)) / args.steps
 
     algo_bytes = 4.0 * n * n + 4.0 * n * d
     ms_per_step = wall * 1e3 / args.steps
@@ -142,8 +155,10 @@ def run_dense(args):
                    "n": n, "dim": d, "metric": metric, "layout": "full", "seed": 2 if d == 3 else 3},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                     "kernel": "dist_tiles (+ fill_u32 and min_rowmax, about 2.5% of the step)",
-                     "launch_ms": round(kern_ms, 5), "algorithmic_bytes": algo_bytes},
+                     "kernel": "dist_tiles (HIP events around its launch alone)",
+                     "launch_ms": round(kern_ms, 5), "algorithmic_bytes": algo_bytes,
+                     "step_ms_events": round(step_ms, 5),
+                     "step_frac": round(algo_bytes / (step_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)},
     }
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(x, metric, algo_bytes)
@@ -224,8 +239,10 @@ def run_graph(args):
         pairs = float(n) * (n - 1) / 2             # lower-triangle tiles
         size = {"edges": int(res.row_ptr[-1]) // 2}
     else:
-        # the Boruvka rounds and vertex scans of one call (tt_mst_deaths_dev),
-        # each scan measuring one vertex against n - 1 candidates
+        # the Boruvka rounds, vertex searches and point pairs measured by one
+        # call (tt_mst_deaths_dev): the searches skip boxes of points that
+        # cannot hold a nearer vertex outside the component, so the pairs are
+        # far fewer than searches x (n - 1)
         import ctypes as C
         import torch
         from turbotops_amd import capi
@@ -233,12 +250,14 @@ def run_graph(args):
         kind, p = parse_metric("l2")
         xt = torch.from_numpy(x).cuda()
         dd = torch.empty(n - 1, dtype=torch.float32, device="cuda")
-        rounds, scans = C.c_uint32(), C.c_uint64()
+        rounds, scans, measured = C.c_uint32(), C.c_uint64(), C.c_uint64()
         capi.lib().call("tt_mst_deaths_dev", C.c_void_p(xt.data_ptr()), n, d, kind, C.c_float(p),
-                        C.c_void_p(dd.data_ptr()), C.byref(rounds), C.byref(scans), None)
+                        C.c_void_p(dd.data_ptr()), C.byref(rounds), C.byref(scans), C.byref(measured), None)
         torch.cuda.synchronize()
-        pairs = float(scans.value) * (n - 1)
-        size = {"deaths": int(res.size), "rounds": int(rounds.value), "vertex_scans": int(scans.value)}
+        pairs = float(measured.value)
+        size = {"deaths": int(res.size), "rounds": int(rounds.value), "vertex_searches": int(scans.value),
+                "pairs_measured": int(measured.value),
+                "pairs_brute_force": int(scans.value) * (n - 1)}
     out = {"metric": f"{w} graph stage, seconds per call", "value": round(per, 5), "unit": "s",
            "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(per * 1e3, 3),
            "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",

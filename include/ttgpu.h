@@ -80,6 +80,16 @@ tt_status tt_dist_matrix_dev(const float* d_pts, uint32_t n, uint32_t dim, int k
                              int layout, float* d_mat, uint32_t* d_rowmax, float* d_radius,
                              void* stream);
 
+/* tt_dist_matrix_dev with a measurement hook (no counterpart in the
+ * reference: for benchmarks that time the distance kernel by itself).
+ * ev_begin / ev_end are hipEvent_t handles (either may be NULL), recorded on
+ * `stream` immediately before and after the distance-tile launch, i.e.
+ * around that one kernel and not the row-maxima fill or the radius
+ * reduction. n = 0 with an event gives TT_EINVAL (nothing is launched). */
+tt_status tt_dist_matrix_dev_events(const float* d_pts, uint32_t n, uint32_t dim, int kind, float p,
+                                    int layout, float* d_mat, uint32_t* d_rowmax, float* d_radius,
+                                    void* stream, void* ev_begin, void* ev_end);
+
 /* ---- neighbourhood graphs ----------------------------------------------- */
 
 /* radius_graph(PointCloud, tau) (graph.hpp:170-201) with
@@ -120,16 +130,20 @@ tt_status tt_knn_graph_exact(const float* pts, uint32_t n, uint32_t dim, int kin
 tt_status tt_mst_deaths(const float* pts, uint32_t n, uint32_t dim, int kind, float p,
                         float* deaths);
 
-/* Device form; *rounds and *scans (may be NULL) report the Boruvka rounds
- * and the number of vertex scans (each over n - 1 candidates). */
+/* Device form; *rounds, *scans and *pairs (each may be NULL) report the
+ * Boruvka rounds, the vertex searches (each for the nearest vertex outside
+ * the vertex's component, over a spatially pruned candidate set) and the
+ * point pairs those searches measured. */
 tt_status tt_mst_deaths_dev(const float* d_pts, uint32_t n, uint32_t dim, int kind, float p,
-                            float* d_deaths, uint32_t* rounds, uint64_t* scans, void* stream);
+                            float* d_deaths, uint32_t* rounds, uint64_t* scans, uint64_t* pairs,
+                            void* stream);
 
 /* tt_mst_deaths_dev without any device allocation. No counterpart in the
  * reference: it is for callers that cannot allocate device memory (code
  * under HIP graph capture, serving loops, harnesses that forbid hipMalloc).
- * Every temporary lives in one static 56 MiB scratch region per device, held
- * until the process ends; n <= 1048576 (TT_ELENGTH beyond). The stream must
+ * Every temporary lives in one static 128 MiB scratch region per device,
+ * held until the process ends; n <= 1048576 and about n (52.6 + 4.125 dim)
+ * bytes of temporaries (dim <= 19 at n = 2^20; TT_ELENGTH beyond). The stream must
  * belong to the current device. Calls that overlap on one device race on
  * that region: serialize them. Same results as tt_mst_deaths_dev. */
 tt_status tt_mst_deaths_dev_noalloc(const float* d_pts, uint32_t n, uint32_t dim, int kind, float p,

@@ -225,3 +225,30 @@ def test_invalid_arguments(gpu):
     x = torch.zeros((4, 2), device="cuda")
     with pytest.raises(ValueError, match="null buffer"):
         capi.lib().call("tt_dist_matrix_dev", x.data_ptr(), 4, 2, 1, 2.0, 0, None, None, None, None)
+
+
+def test_events_variant_times_the_tile_kernel(gpu, restatement):
+    """tt_dist_matrix_dev_events writes what tt_dist_matrix_dev writes and
+    records its two events around the tile launch (the second completes after
+    the first, with a positive elapsed time)."""
+    import torch
+    from turbotops_amd import dist_matrix_dev
+    x = _cloud(700, 3, seed=70)
+    n = x.shape[0]
+    xt = torch.from_numpy(x).cuda()
+    out = []
+    for timed in (False, True):
+        rowmax = torch.zeros(n, dtype=torch.int32, device="cuda")
+        radius = torch.zeros(1, dtype=torch.float32, device="cuda")
+        ev = (torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) if timed else None
+        mat = dist_matrix_dev(xt, "l2", "full", rowmax=rowmax, radius=radius,
+                              stream=torch.cuda.current_stream(), events=ev)
+        torch.cuda.synchronize()
+        out.append((mat.cpu().numpy(), rowmax.cpu().numpy(), float(radius.cpu()[0])))
+        if timed:
+            assert ev[0].query() and ev[1].query()
+            assert ev[0].elapsed_time(ev[1]) > 0.0
+    want = restatement.make_dense(x, "l2", True)
+    assert np.array_equal(bits(out[0][0].reshape(n, n)), bits(want))
+    assert np.array_equal(bits(out[1][0]), bits(out[0][0]))
+    assert np.array_equal(out[1][1], out[0][1]) and out[1][2] == out[0][2]

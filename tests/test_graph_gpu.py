@@ -233,6 +233,40 @@ def test_mst_deaths(gpu, restatement, metric, n, d, kind):
         assert ulp_diff(got, want).max() <= 2
 
 
+def _stress_cloud(shape, n, seed):
+    rng = np.random.default_rng(seed)
+    if shape == "clusters":    # tight clusters far apart, stored interleaved
+        centres = rng.uniform(-50, 50, size=(7, 3))
+        return (centres[rng.integers(0, 7, size=n)] + rng.normal(scale=0.01, size=(n, 3))).astype(np.float32)
+    if shape == "duplicates":  # 40 distinct points, each repeated
+        pts = rng.normal(size=(40, 2))
+        return pts[rng.integers(0, 40, size=n)].astype(np.float32)
+    if shape == "line":        # a segment in R^3: boxes of zero width in two coordinates
+        t = rng.uniform(size=(n, 1))
+        return np.hstack([t, np.full((n, 1), 0.25), np.full((n, 1), -1.0)]).astype(np.float32)
+    return (rng.uniform(size=(n, 5)) * np.array([1, 10, 100, 0.1, 1])).astype(np.float32)   # "aniso", runtime dim
+
+
+@pytest.mark.parametrize("metric", METRICS)
+@pytest.mark.parametrize("shape", ["clusters", "duplicates", "line", "aniso"])
+def test_mst_pruned_search_cases(gpu, restatement, metric, shape):
+    """The Boruvka search skips blocks and super-blocks by their boxes and
+    component tags; these inputs stress the skips over several 4096-point
+    super-blocks (n = 9000): clusters far apart (late rounds search across
+    empty space, whole blocks in one component), many duplicates (keys of 0,
+    ties), a segment (boxes of zero width), and an anisotropic 5-D cloud (the
+    runtime-dimension kernel)."""
+    from turbotops_amd import mst_deaths
+    x = _stress_cloud(shape, 9000, 41)
+    print(f"case mst-stress metric={metric} shape={shape}", flush=True)
+    got = mst_deaths(x, metric)
+    want = restatement.mst_deaths(x, metric)
+    if metric in ("l1", "l2", "linf"):
+        assert np.array_equal(bits(got), bits(want))
+    else:
+        assert ulp_diff(got, want).max() <= 2
+
+
 def test_mst_h0_bars_and_threshold(gpu, reference):
     """Bars against the compiled reference's mst::h0, with and without a cut
     (src/turbotops.cpp:529-547)."""

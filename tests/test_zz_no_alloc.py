@@ -22,7 +22,7 @@ def _deaths(torch, lib, name, x, kind, p):
     xd = torch.from_numpy(x).to("cuda:0")
     out = torch.full((max(n - 1, 1),), -1.0, dtype=torch.float32, device="cuda:0")
     if name == "tt_mst_deaths_dev":
-        lib.call(name, C.c_void_p(xd.data_ptr()), n, dim, kind, p, C.c_void_p(out.data_ptr()), None, None,
+        lib.call(name, C.c_void_p(xd.data_ptr()), n, dim, kind, p, C.c_void_p(out.data_ptr()), None, None, None,
                  _stream(torch))
     else:
         lib.call(name, C.c_void_p(xd.data_ptr()), n, dim, kind, p, C.c_void_p(out.data_ptr()), _stream(torch))
@@ -68,3 +68,18 @@ def test_noalloc_at_the_size_limit(gpu):
     want = _deaths(torch, lib, "tt_mst_deaths_dev", x, capi.METRIC_L2, 2.0)
     got = _deaths(torch, lib, "tt_mst_deaths_dev_noalloc", x, capi.METRIC_L2, 2.0)
     assert np.array_equal(got.view(np.uint32), want.view(np.uint32))
+
+
+def test_noalloc_rejects_temporaries_beyond_scratch(gpu):
+    """n = 2^20 in 20 dimensions needs more temporaries (the sorted copy of
+    the points grows with dim) than the static region holds: TT_ELENGTH
+    before any kernel runs."""
+    import torch
+    lib = capi.lib()
+    n, dim = 1 << 20, 20
+    x = torch.zeros((n, dim), dtype=torch.float32, device="cuda:0")
+    out = torch.zeros(n, dtype=torch.float32, device="cuda:0")
+    rc = lib.dll.tt_mst_deaths_dev_noalloc(C.c_void_p(x.data_ptr()), n, dim, capi.METRIC_L2, 2.0,
+                                           C.c_void_p(out.data_ptr()), _stream(torch))
+    assert rc == capi.TT_ELENGTH
+    assert "scratch region too small" in lib.last_error()

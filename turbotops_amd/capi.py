@@ -35,6 +35,8 @@ _SIGS = {
     "tt_make_dense": ([_vp, C.c_uint32, C.c_uint32, C.c_int, C.c_float, C.c_int, _vp, _vp], None),
     "tt_dist_matrix_dev": ([_vp, C.c_uint32, C.c_uint32, C.c_int, C.c_float, C.c_int, _vp, _vp, _vp, _vp],
                            None),
+    "tt_dist_matrix_dev_events": ([_vp, C.c_uint32, C.c_uint32, C.c_int, C.c_float, C.c_int, _vp, _vp, _vp, _vp,
+                                   _vp, _vp], None),
     "tt_radius_graph": ([_vp, C.c_uint32, C.c_uint32, C.c_int, C.c_float, C.c_float, _vp, _u64p, _vp, _vp],
                         None),
     "tt_knn_exact": ([_vp, C.c_uint32, C.c_uint32, C.c_int, C.c_float, C.c_uint32, _u32p, _vp], None),
@@ -43,7 +45,7 @@ _SIGS = {
     "tt_knn_graph_exact": ([_vp, C.c_uint32, C.c_uint32, C.c_int, C.c_float, C.c_uint32, _vp, _u64p, _vp, _vp],
                            None),
     "tt_mst_deaths": ([_vp, C.c_uint32, C.c_uint32, C.c_int, C.c_float, _vp], None),
-    "tt_mst_deaths_dev": ([_vp, C.c_uint32, C.c_uint32, C.c_int, C.c_float, _vp, _u32p, _u64p, _vp], None),
+    "tt_mst_deaths_dev": ([_vp, C.c_uint32, C.c_uint32, C.c_int, C.c_float, _vp, _u32p, _u64p, _u64p, _vp], None),
     "tt_mst_deaths_dev_noalloc": ([_vp, C.c_uint32, C.c_uint32, C.c_int, C.c_float, _vp, _vp], None),
     "tt_dense_edges": ([_vp, C.c_uint32, C.c_uint32, C.c_int, C.c_float, C.c_float, _u64p, _vp, _vp, _vp,
                         _f32p], None),

@@ -152,7 +152,8 @@ void fill_device_u32(u32* p, u64 count, u32 v, hipStream_t s) {
 
 // Stream-ordered distance matrix (+ row maxima, + enclosing radius).
 void dist_matrix(const float* d_x, u32 n, u32 dim, int kind, float p, int layout, float* d_mat,
-                 u32* d_rowmax, float* d_radius, hipStream_t s, int rpw_override) {
+                 u32* d_rowmax, float* d_radius, hipStream_t s, int rpw_override,
+                 hipEvent_t ev_begin, hipEvent_t ev_end) {
     check_metric(kind, p);
     if (layout != TT_LAYOUT_FULL && layout != TT_LAYOUT_LOWER)
         fail(TT_EINVAL, "dist_matrix: unknown layout " + std::to_string(layout));
@@ -168,11 +169,13 @@ void dist_matrix(const float* d_x, u32 n, u32 dim, int kind, float p, int layout
         const MetricVariant v = variant_of(kind, p);
         const MetricArgs ma = metric_args(kind, p);
         const int rpw = rpw_override > 0 ? rpw_override : tile_rows_per_wave();
+        if (ev_begin) TTG_HIP(hipEventRecord(ev_begin, s));
         if (layout == TT_LAYOUT_FULL)
             launch_rpw<TT_LAYOUT_FULL>(rpw, v, u32(blocks), s, d_x, n, dim, ma, d_mat, d_rowmax);
         else
             launch_rpw<TT_LAYOUT_LOWER>(rpw, v, u32(blocks), s, d_x, n, dim, ma, d_mat, d_rowmax);
         TTG_LAUNCH_CHECK();
+        if (ev_end) TTG_HIP(hipEventRecord(ev_end, s));
     }
     if (d_radius) {
         min_rowmax<<<1, 1024, 0, s>>>(d_rowmax, n, d_radius);
@@ -191,6 +194,16 @@ tt_status tt_dist_matrix_dev(const float* d_pts, uint32_t n, uint32_t dim, int k
                              void* stream) {
     return guarded([&] {
         dist_matrix(d_pts, n, dim, kind, p, layout, d_mat, d_rowmax, d_radius, as_stream(stream));
+    });
+}
+
+tt_status tt_dist_matrix_dev_events(const float* d_pts, uint32_t n, uint32_t dim, int kind, float p,
+                                    int layout, float* d_mat, uint32_t* d_rowmax, float* d_radius,
+                                    void* stream, void* ev_begin, void* ev_end) {
+    return guarded([&] {
+        if (!n && (ev_begin || ev_end)) fail(TT_EINVAL, "tt_dist_matrix_dev_events: no tile launch to time at n = 0");
+        dist_matrix(d_pts, n, dim, kind, p, layout, d_mat, d_rowmax, d_radius, as_stream(stream), 0,
+                    static_cast<hipEvent_t>(ev_begin), static_cast<hipEvent_t>(ev_end));
     });
 }
 

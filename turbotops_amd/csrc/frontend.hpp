@@ -8,11 +8,23 @@
 namespace ttg {
 
 // dist.hip
+// ev_begin / ev_end (may be null): recorded on s right before and right after
+// the tile launch, so a caller can time that kernel alone
 void dist_matrix(const float* d_x, u32 n, u32 dim, int kind, float p, int layout, float* d_mat,
-                 u32* d_rowmax, float* d_radius, hipStream_t s, int rpw_override = 0);
+                 u32* d_rowmax, float* d_radius, hipStream_t s, int rpw_override = 0,
+                 hipEvent_t ev_begin = nullptr, hipEvent_t ev_end = nullptr);
 void fill_device_u32(u32* p, u64 count, u32 v, hipStream_t s);
 int tile_rows_per_wave();
 void set_tile_rows_per_wave(int rpw);
+
+// spatial.hip: xs = the n points of d_x (row-major, dim floats) sorted by a
+// Morton-style grid code (stable), perm[pos] = the original index of xs's
+// row pos (perm may be null). d_n holds n on the device (the counted sort).
+// Temporaries: keys and alt (n entries each), sort_scratch
+// (radix_scratch_entries(n)), ext (spatial_ext_entries(dim)).
+u64 spatial_ext_entries(u32 dim);
+void spatial_order(const float* d_x, u32 n, u32 dim, const u32* d_n, u64* keys, u64* alt, u64* sort_scratch,
+                   u32* ext, float* xs, u32* perm, hipStream_t s);
 
 // hostcopy.hip: device -> pageable host memory, staged through pinned
 // buffers for large sizes; returns when the data is in dst

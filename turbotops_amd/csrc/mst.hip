@@ -12,11 +12,31 @@
 // that order refines the distance order and the tree is an MST for the
 // distances too; the deaths are sqrtf(key) of its edges.
 //
+// Spatial order (once): points are sorted by a Morton-style code of their
+// grid cell and copied in that order; Boruvka runs on the sorted positions.
+// Any strict edge order gives a minimum spanning tree, and every MST has the
+// same multiset of weights, so the sorted deaths do not depend on the labels.
+// Blocks of 64 consecutive points and super-blocks of 64 blocks carry an
+// axis-aligned box (the min and max of their members' coordinates).
+//
 // Per round:
-//   nearest     one lane per vertex scans every other vertex (points and
-//               component labels staged in LDS) for its smallest (key, id)
-//               edge into another component; a vertex whose previous best
-//               still leads outside keeps it (the outside set only shrinks)
+//   todo        vertices that need a fresh search: all in round 0, afterwards
+//               those whose previous best has joined their own component (the
+//               outside set only shrinks, so any other keeps its edge)
+//   comp tags   per block and super-block: the one component of all its
+//               members, or "mixed"
+//   nearest     one lane per todo vertex finds its smallest (key, j) edge into
+//               another component. Super-blocks, then blocks, are visited from
+//               the wave's own block outward in sorted order; one is skipped
+//               when it lies wholly in the vertex's component, or when the key
+//               to the nearest point of its box exceeds the best so far. The
+//               float key is monotone in each coordinate difference (the
+//               differences, squares, fused multiply-adds, sums and maxima all
+//               round monotonically) and no member is nearer than the box's
+//               nearest point in any coordinate, so that key is a lower bound
+//               of every member's key: skipping never changes the result. For
+//               Lp the bound is the L-inf distance, which the computed Lp
+//               distance never goes below (m * pow(s, 1/p) with s >= 1).
 //   comp_min    per component, atomicMin of the key bits, then atomicMin of
 //               (min id << 32 | max id) among the vertices that reach it
 //   hook        each component points at the one across its edge; of a
@@ -36,69 +56,183 @@ namespace ttg {
 namespace {
 
 constexpr u32 MB = 256;
+constexpr u32 PB = 64;          // points per block (one wave)
+constexpr u32 SBB = 64;         // blocks per super-block
+constexpr u32 MIXED = 0xffffffffu;
 
+// Key from q to the nearest point of the box [lo, hi] (see the file comment):
+// the same arithmetic as pdist<V, DIM, true> on the clamped coordinates for
+// L2 and L1, the largest coordinate difference for L-inf and Lp.
 template <int V, int DIM>
-__global__ __launch_bounds__(MB) void mst_nearest(const float* __restrict__ x, u32 n, u32 dim_rt,
+__device__ __forceinline__ float box_key(const float* __restrict__ q, const float* __restrict__ lo,
+                                         const float* __restrict__ hi, u32 dim_rt) {
+    const u32 dim = DIM > 0 ? u32(DIM) : dim_rt;
+    float s = 0.0f;
+#pragma unroll
+    for (u32 k = 0; k < dim; ++k) {
+        const float z = __builtin_fminf(__builtin_fmaxf(q[k], lo[k]), hi[k]);
+        const float dz = __builtin_fabsf(z - q[k]);
+        if constexpr (V == MV_L2) s = __builtin_fmaf(dz, dz, s);
+        else if constexpr (V == MV_L1) s += dz;
+        else s = __builtin_fmaxf(s, dz);
+    }
+    return s;
+}
+
+// One lane per todo vertex (a sorted position): its smallest (key, j) edge
+// to another component, with blocks skipped as described above. Every
+// decision to scan a block is a ballot over the wave, and the block's points
+// are wave-uniform (scalar loads), so a scanned block costs its 64 pairs once
+// per wave.
+template <int V, int DIM>
+__global__ __launch_bounds__(MB) void mst_nearest(const float* __restrict__ xs, u32 n, u32 dim_rt,
                                                   MetricArgs ma, const u32* __restrict__ comp,
                                                   const u32* __restrict__ todo, u32 ntodo,
-                                                  float* __restrict__ best_key,
-                                                  u32* __restrict__ best_j) {
+                                                  const float* __restrict__ box, const u32* __restrict__ bcomp,
+                                                  const float* __restrict__ sbox,
+                                                  const u32* __restrict__ sbcomp, u32 nblk,
+                                                  float* __restrict__ best_key, u32* __restrict__ best_j,
+                                                  unsigned long long* __restrict__ pairs) {
     constexpr u32 DS = DIM > 0 ? DIM : 1;
-    __shared__ float tile[DIM > 0 ? MB * DS : 1];
-    __shared__ u32 tcomp[MB];
     const u32 dim = DIM > 0 ? u32(DIM) : dim_rt;
     const u32 t = blockIdx.x * MB + threadIdx.x;
-    const u32 i = t < ntodo ? todo[t] : todo[ntodo - 1];
+    const bool live = t < ntodo;
+    const u32 i = live ? todo[t] : todo[ntodo - 1];
     const u32 ci = comp[i];
-    float q[DS];
-    const float* __restrict__ qp = x + u64(i) * dim;
+    float qr[DS];
+    const float* __restrict__ q = xs + u64(i) * dim;
     if constexpr (DIM > 0) {
 #pragma unroll
-        for (int k = 0; k < DIM; ++k) q[k] = qp[k];
+        for (int k = 0; k < DIM; ++k) qr[k] = q[k];
+        q = qr;
     }
     float bk = INFINITY;
     u32 bj = 0xffffffffu;
-    for (u32 j0 = 0; j0 < n; j0 += MB) {
-        const u32 len = min(MB, n - j0);
-        __syncthreads();
-        if constexpr (DIM > 0)
-            for (u32 e = threadIdx.x; e < len * DS; e += MB) tile[e] = x[u64(j0) * DS + e];
-        if (threadIdx.x < len) tcomp[threadIdx.x] = comp[j0 + threadIdx.x];
-        __syncthreads();
-        for (u32 jj = 0; jj < len; ++jj) {
-            if (tcomp[jj] == ci) continue;
-            const u32 j = j0 + jj;
-            float kv;
-            if constexpr (DIM > 0) kv = pdist<V, DIM, true>(q, tile + jj * DS, dim, ma);
-            else kv = pdist<V, 0, true>(qp, x + u64(j) * dim, dim, ma);
-            // (key, j) order: for a fixed i this is the edge order (key, min, max)
-            if (kv < bk) {
-                bk = kv;
-                bj = j;
+    const u32 nsb = (nblk + SBB - 1) / SBB;
+    const u32 home = __builtin_amdgcn_readfirstlane(i) / PB;   // the block of the wave's first vertex
+    const u32 hsb = home / SBB;
+    const u32 nlive = u32(__popcll(__ballot(live)));
+    u64 measured = 0;   // pairs this wave evaluates
+    for (u32 su = 0; su < nsb; ++su) {
+        u32 sb = hsb + su;
+        if (sb >= nsb) sb -= nsb;
+        {
+            const float* bx = sbox + u64(sb) * 2 * dim;
+            const float lb = box_key<V, DIM>(q, bx, bx + dim, dim);
+            if (!__ballot(live && sbcomp[sb] != ci && lb <= bk)) continue;
+        }
+        const u32 b0 = sb * SBB, len = min(SBB, nblk - b0);
+        const u32 start = sb == hsb ? home - b0 : 0u;
+        for (u32 bu = 0; bu < len; ++bu) {
+            u32 b = start + bu;
+            if (b >= len) b -= len;
+            b += b0;
+            {
+                const float* bx = box + u64(b) * 2 * dim;
+                const float lb = box_key<V, DIM>(q, bx, bx + dim, dim);
+                if (!__ballot(live && bcomp[b] != ci && lb <= bk)) continue;
+            }
+            const u32 j1 = u32(min<u64>(n, u64(b + 1) * PB));
+            measured += u64(nlive) * (j1 - b * PB);
+            for (u32 j = b * PB; j < j1; ++j) {
+                const u32 cj = comp[j];
+                float kv;
+                if constexpr (DIM > 0) {
+                    float c[DIM];
+#pragma unroll
+                    for (int k = 0; k < DIM; ++k) c[k] = xs[u64(j) * DIM + k];
+                    kv = pdist<V, DIM, true>(c, q, dim, ma);
+                } else {
+                    kv = pdist<V, 0, true>(xs + u64(j) * dim, q, dim, ma);
+                }
+                // (key, j) order: for a fixed i this is the edge order (key, min, max)
+                if (cj != ci && (kv < bk || (kv == bk && j < bj))) {
+                    bk = kv;
+                    bj = j;
+                }
             }
         }
     }
-    if (t < ntodo) {
+    if (live) {
         best_key[i] = bk;
         best_j[i] = bj;
     }
+    if ((threadIdx.x & 63) == 0 && measured) atomicAdd(pairs, (unsigned long long)measured);
 }
 
 template <int V, int DIM>
 struct NearestLaunch {
-    static void run(u32 blocks, hipStream_t s, const float* x, u32 n, u32 dim, MetricArgs ma,
-                    const u32* comp, const u32* todo, u32 ntodo, float* bk, u32* bj) {
-        mst_nearest<V, DIM><<<blocks, MB, 0, s>>>(x, n, dim, ma, comp, todo, ntodo, bk, bj);
+    static void run(u32 blocks, hipStream_t s, const float* xs, u32 n, u32 dim, MetricArgs ma, const u32* comp,
+                    const u32* todo, u32 ntodo, const float* box, const u32* bcomp, const float* sbox,
+                    const u32* sbcomp, u32 nblk, float* bk, u32* bj, unsigned long long* pairs) {
+        mst_nearest<V, DIM><<<blocks, MB, 0, s>>>(xs, n, dim, ma, comp, todo, ntodo, box, bcomp, sbox, sbcomp,
+                                                  nblk, bk, bj, pairs);
     }
 };
 
-// Vertices that need a fresh scan: every vertex in round 0, afterwards those
-// whose best neighbour has joined their own component.
+// One wave per box: the min and max of each coordinate over its members
+// (points of a block, or the boxes of a super-block's blocks).
+__global__ __launch_bounds__(MB) void mst_boxes(const float* __restrict__ src, u32 count, u32 per, u32 dim,
+                                                int from_boxes, float* __restrict__ out) {
+    const u32 lane = threadIdx.x & 63;
+    const u32 nbox = (count + per - 1) / per;
+    for (u32 b = (blockIdx.x * MB + threadIdx.x) / 64; b < nbox; b += gridDim.x * (MB / 64)) {
+        const u32 m = min(per, count - b * per);
+        const u32 e = b * per + (lane < m ? lane : 0u);
+        for (u32 k = 0; k < dim; ++k) {
+            float lo, hi;
+            if (from_boxes) {
+                lo = src[u64(e) * 2 * dim + k];
+                hi = src[u64(e) * 2 * dim + dim + k];
+            } else {
+                lo = hi = src[u64(e) * dim + k];
+            }
+            for (int off = 32; off > 0; off >>= 1) {
+                lo = __builtin_fminf(lo, __shfl_xor(lo, off));
+                hi = __builtin_fmaxf(hi, __shfl_xor(hi, off));
+            }
+            if (lane == 0) {
+                out[u64(b) * 2 * dim + k] = lo;
+                out[u64(b) * 2 * dim + dim + k] = hi;
+            }
+        }
+    }
+}
+
+// One wave per block (or super-block): the one component of its members
+// (points' comp, or blocks' tags), or MIXED.
+__global__ __launch_bounds__(MB) void mst_tags(const u32* __restrict__ src, u32 count, u32* __restrict__ out) {
+    const u32 lane = threadIdx.x & 63;
+    const u32 nbox = (count + 63) / 64;
+    for (u32 b = (blockIdx.x * MB + threadIdx.x) / 64; b < nbox; b += gridDim.x * (MB / 64)) {
+        const u32 m = min(64u, count - b * 64);
+        const u32 c = src[b * 64 + (lane < m ? lane : 0u)];
+        const u32 c0 = __builtin_amdgcn_readfirstlane(c);
+        const bool one = c0 != MIXED && __ballot(c != c0) == 0;   // every lane votes
+        if (lane == 0) out[b] = one ? c0 : MIXED;
+    }
+}
+
+__device__ __forceinline__ u32 lane_rank_u64(u64 mask) {
+    return __builtin_amdgcn_mbcnt_hi(u32(mask >> 32), __builtin_amdgcn_mbcnt_lo(u32(mask), 0u));
+}
+
+// Vertices that need a fresh search: every vertex in round 0, afterwards
+// those whose best neighbour has joined their own component. Each wave
+// appends its vertices in lane order (one atomic per wave), so a wave of the
+// search takes neighbouring positions.
 __global__ void mst_todo(const u32* __restrict__ comp, const u32* __restrict__ best_j, u32 n,
                          int first, u32* __restrict__ todo, u32* __restrict__ ntodo) {
-    for (u32 v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x) {
-        const bool need = first || best_j[v] == 0xffffffffu || comp[best_j[v]] == comp[v];
-        if (need) todo[atomicAdd(ntodo, 1u)] = v;
+    const u32 lane = threadIdx.x & 63;
+    for (u32 v0 = blockIdx.x * blockDim.x + (threadIdx.x & ~63u); v0 < n; v0 += gridDim.x * blockDim.x) {
+        const u32 v = v0 + lane;
+        const bool need = v < n && (first || best_j[v] == 0xffffffffu || comp[best_j[v]] == comp[v]);
+        const u64 mask = __ballot(need);
+        if (!mask) continue;
+        u32 base = 0;
+        if (lane == 0) base = atomicAdd(ntodo, u32(__popcll(mask)));
+        base = __builtin_amdgcn_readfirstlane(base);
+        if (need) todo[base + lane_rank_u64(mask)] = v;
     }
 }
 
@@ -184,7 +318,8 @@ u32 blocks_of(u64 count) {
 
 struct MstStats {
     u32 rounds = 0;
-    u64 scanned = 0;    // vertex scans (each n - 1 candidates)
+    u64 scanned = 0;    // vertex searches
+    u64 pairs = 0;      // point pairs measured by them
 };
 
 namespace {
@@ -204,14 +339,17 @@ struct Carve {
     }
 };
 
-// Every device buffer a Boruvka run on n vertices uses, carved from `c` in
-// a fixed order (a measuring Carve gives the total).
+// Every device buffer a Boruvka run on n vertices in R^dim uses, carved from
+// `c` in a fixed order (a measuring Carve gives the total).
 struct MstBuffers {
-    u32 *comp, *parent, *cw, *bj, *todo, *scal;
+    u32 nblk, nsb;
+    u32 *comp, *parent, *cw, *bj, *todo, *scal, *bcomp, *sbcomp, *ext;
     unsigned long long* cp;
-    float *bk, *keys;
+    float *bk, *keys, *xs, *box, *sbox;
     u64 *sk, *alt, *sort_scratch;
-    MstBuffers(Carve& c, u32 n) {
+    MstBuffers(Carve& c, u32 n, u32 dim) {
+        nblk = (n + PB - 1) / PB;
+        nsb = (nblk + SBB - 1) / SBB;
         comp = c.get<u32>(n);
         parent = c.get<u32>(n);
         cw = c.get<u32>(n);
@@ -219,19 +357,25 @@ struct MstBuffers {
         bk = c.get<float>(n);
         bj = c.get<u32>(n);
         todo = c.get<u32>(n);
-        scal = c.get<u32>(4);      // [0] ntodo, [1] changed, [2] out count, [3] n - 1
+        scal = c.get<u32>(8);      // [0] ntodo, [1] changed, [2] out count, [3] n - 1, [4] n, [6..7] pairs
         keys = c.get<float>(n);
         sk = c.get<u64>(n);
         alt = c.get<u64>(n);
         sort_scratch = c.get<u64>(radix_scratch_entries(n));
+        xs = c.get<float>(u64(n) * dim);
+        box = c.get<float>(u64(nblk) * 2 * dim);
+        sbox = c.get<float>(u64(nsb) * 2 * dim);
+        bcomp = c.get<u32>(nblk);
+        sbcomp = c.get<u32>(nsb);
+        ext = c.get<u32>(spatial_ext_entries(dim));
     }
 };
 
 }  // namespace
 
-u64 mst_scratch_bytes(u32 n) {
+u64 mst_scratch_bytes(u32 n, u32 dim) {
     Carve c;
-    MstBuffers b(c, std::max<u32>(n, 2));
+    MstBuffers b(c, std::max<u32>(n, 2), std::max<u32>(dim, 1));
     (void)b;
     return c.used;
 }
@@ -244,7 +388,7 @@ void mst_deaths_in(const float* d_x, u32 n, u32 dim, int kind, float p, float* d
     const MetricVariant v = variant_of(kind, p);
     const MetricArgs ma = metric_args(kind, p);
     Carve c{base, cap, 0};
-    const MstBuffers B(c, n);
+    const MstBuffers B(c, n, dim);
     u32 *comp = B.comp, *parent = B.parent, *cw = B.cw, *bj = B.bj, *todo = B.todo, *scal = B.scal;
     unsigned long long* cp = B.cp;
     float *bk = B.bk, *keys = B.keys;
@@ -255,6 +399,14 @@ void mst_deaths_in(const float* d_x, u32 n, u32 dim, int kind, float p, float* d
     fill_device_u32(bj, n, 0xffffffffu, s);
     fill_device_u32(scal, 3, 0u, s);
     fill_device_u32(scal + 3, 1, n - 1, s);
+    fill_device_u32(scal + 4, 1, n, s);
+    fill_device_u32(scal + 6, 2, 0u, s);   // [6..7]: pairs evaluated (u64)
+    TTG_LAUNCH_CHECK();
+
+    // spatial order; the sorted copy B.xs is what the rounds read
+    spatial_order(d_x, n, dim, scal + 4, B.sk, B.alt, B.sort_scratch, B.ext, B.xs, nullptr, s);
+    mst_boxes<<<blocks_of(u64(B.nblk) * 64), MB, 0, s>>>(B.xs, n, PB, dim, 0, B.box);
+    mst_boxes<<<blocks_of(u64(B.nsb) * 64), MB, 0, s>>>(B.box, B.nblk, SBB, dim, 1, B.sbox);
     TTG_LAUNCH_CHECK();
     u32 edges = 0;
     for (u32 round = 0; edges < n - 1; ++round) {
@@ -265,8 +417,11 @@ void mst_deaths_in(const float* d_x, u32 n, u32 dim, int kind, float p, float* d
         TTG_HIP(hipMemcpyAsync(&ntodo, scal, sizeof(u32), hipMemcpyDeviceToHost, s));
         TTG_HIP(hipStreamSynchronize(s));
         if (ntodo) {
-            dispatch_metric_dim<NearestLaunch>(v, dim, (ntodo + MB - 1) / MB, s, d_x, n, dim, ma, comp, todo,
-                                               ntodo, bk, bj);
+            mst_tags<<<blocks_of(u64(B.nblk) * 64), MB, 0, s>>>(comp, n, B.bcomp);
+            mst_tags<<<blocks_of(u64(B.nsb) * 64), MB, 0, s>>>(B.bcomp, B.nblk, B.sbcomp);
+            dispatch_metric_dim<NearestLaunch>(v, dim, (ntodo + MB - 1) / MB, s, B.xs, n, dim, ma, comp, todo,
+                                               ntodo, B.box, B.bcomp, B.sbox, B.sbcomp, B.nblk, bk, bj,
+                                               reinterpret_cast<unsigned long long*>(scal + 6));
             TTG_LAUNCH_CHECK();
         }
         if (st) {
@@ -291,6 +446,12 @@ void mst_deaths_in(const float* d_x, u32 n, u32 dim, int kind, float p, float* d
         TTG_HIP(hipStreamSynchronize(s));
     }
     if (edges != n - 1) fail(TT_EDEVICE, "mst: spanning tree has the wrong edge count");
+    if (st) {
+        unsigned long long pairs = 0;
+        TTG_HIP(hipMemcpyAsync(&pairs, scal + 6, sizeof(pairs), hipMemcpyDeviceToHost, s));
+        TTG_HIP(hipStreamSynchronize(s));
+        st->pairs = pairs;
+    }
     switch (v) {
         case MV_L2: mst_finish<MV_L2><<<blocks_of(n), MB, 0, s>>>(keys, n - 1, B.sk); break;
         default: mst_finish<MV_L1><<<blocks_of(n), MB, 0, s>>>(keys, n - 1, B.sk); break;
@@ -305,7 +466,7 @@ void mst_deaths_in(const float* d_x, u32 n, u32 dim, int kind, float p, float* d
 void mst_deaths_dev(const float* d_x, u32 n, u32 dim, int kind, float p, float* d_deaths, Workspace& ws,
                     hipStream_t s, MstStats* st) {
     if (n < 2) return;
-    const u64 bytes = mst_scratch_bytes(n);
+    const u64 bytes = mst_scratch_bytes(n, dim);
     mst_deaths_in(d_x, n, dim, kind, p, d_deaths, ws.get<char>(bytes), bytes, s, st);
 }
 
@@ -316,7 +477,7 @@ void mst_deaths_dev(const float* d_x, u32 n, u32 dim, int kind, float p, float* 
 // symbol); it passes the device address hipGetSymbolAddress returns, looked
 // up once per device with that device current.
 constexpr u32 kMstNoAllocMaxN = 1u << 20;
-constexpr u64 kMstScratchBytes = u64(56) << 20;
+constexpr u64 kMstScratchBytes = u64(128) << 20;   // n (52.6 + 4.125 dim) bytes: dim <= 19 at n = 2^20
 
 static __device__ __align__(256) unsigned char scratch[kMstScratchBytes];
 
@@ -370,7 +531,8 @@ tt_status tt_mst_deaths(const float* pts, uint32_t n, uint32_t dim, int kind, fl
 }
 
 tt_status tt_mst_deaths_dev(const float* d_pts, uint32_t n, uint32_t dim, int kind, float p,
-                            float* d_deaths, uint32_t* rounds, uint64_t* scans, void* stream) {
+                            float* d_deaths, uint32_t* rounds, uint64_t* scans, uint64_t* pairs,
+                            void* stream) {
     return guarded([&] {
         check_metric(kind, p);
         if (n >= 2 && (!d_pts || !d_deaths)) fail(TT_EINVAL, "tt_mst_deaths_dev: null buffer");
@@ -382,6 +544,7 @@ tt_status tt_mst_deaths_dev(const float* d_pts, uint32_t n, uint32_t dim, int ki
         TTG_HIP(hipStreamSynchronize(s));
         if (rounds) *rounds = st.rounds;
         if (scans) *scans = st.scanned;
+        if (pairs) *pairs = st.pairs;
     });
 }
 
@@ -395,7 +558,7 @@ tt_status tt_mst_deaths_dev_noalloc(const float* d_pts, uint32_t n, uint32_t dim
         if (n >= 2 && (!d_pts || !d_deaths)) fail(TT_EINVAL, "tt_mst_deaths_dev_noalloc: null buffer");
         if (n >= 2 && dim == 0) fail(TT_EINVAL, "tt_mst_deaths_dev_noalloc: dim must be >= 1");
         if (n < 2) return;
-        const u64 bytes = mst_scratch_bytes(n);
+        const u64 bytes = mst_scratch_bytes(n, dim);
         if (bytes > kMstScratchBytes) fail(TT_ELENGTH, "tt_mst_deaths_dev_noalloc: scratch region too small");
         hipStream_t s = as_stream(stream);
         mst_deaths_in(d_pts, n, dim, kind, p, d_deaths, scratch_ptr(), kMstScratchBytes, s, nullptr);

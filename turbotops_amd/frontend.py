@@ -66,9 +66,13 @@ def make_dense(x, metric="l2", budget=2**64 - 1, with_radius=True):
 
 
 def _stream(stream):
+    """A stream argument as a hipStream_t: None (torch's current stream), a
+    torch.cuda.Stream, or a raw handle."""
     if stream is None:
         import torch
         return C.c_void_p(torch.cuda.current_stream().cuda_stream)
+    if hasattr(stream, "cuda_stream"):
+        return C.c_void_p(stream.cuda_stream)
     return C.c_void_p(int(stream))
 
 
@@ -76,10 +80,23 @@ def _dptr(t):
     return C.c_void_p(t.data_ptr()) if t is not None and t.numel() else None
 
 
-def dist_matrix_dev(x, metric="l2", layout="full", out=None, rowmax=None, radius=None, stream=None):
+def _event_handle(ev, stream):
+    """The hipEvent_t behind a torch.cuda.Event (torch creates it on the
+    first record, so an event never recorded is recorded once here)."""
+    if ev is None:
+        return None
+    if not ev.cuda_event:
+        ev.record(stream)
+    return C.c_void_p(ev.cuda_event)
+
+
+def dist_matrix_dev(x, metric="l2", layout="full", out=None, rowmax=None, radius=None, stream=None,
+                    events=None):
     """Device form: x is a (n, d) float32 torch tensor in HBM. Returns the
     matrix tensor (n*n or n(n-1)/2 floats); fills rowmax (n int32 holding
-    float bits) and radius (1 float) when given."""
+    float bits) and radius (1 float) when given. events: optional pair of
+    torch.cuda.Event recorded right before and after the distance-tile
+    kernel alone (stream must then be a torch.cuda.Stream)."""
     import torch
     if x.dtype != torch.float32 or not x.is_contiguous() or x.dim() != 2:
         raise ValueError("x must be a contiguous (n, d) float32 tensor")
@@ -91,8 +108,15 @@ def dist_matrix_dev(x, metric="l2", layout="full", out=None, rowmax=None, radius
         out = torch.empty(cells, dtype=torch.float32, device=x.device)
     elif out.numel() < cells:
         raise ValueError("out is too small")
-    capi.lib().call("tt_dist_matrix_dev", _dptr(x), n, d, kind, p, lay, _dptr(out), _dptr(rowmax),
-                    _dptr(radius), _stream(stream))
+    if events is None:
+        capi.lib().call("tt_dist_matrix_dev", _dptr(x), n, d, kind, p, lay, _dptr(out), _dptr(rowmax),
+                        _dptr(radius), _stream(stream))
+    else:
+        import torch
+        ts = stream if isinstance(stream, torch.cuda.Stream) else torch.cuda.current_stream(x.device)
+        ev0, ev1 = (_event_handle(e, ts) for e in events)
+        capi.lib().call("tt_dist_matrix_dev_events", _dptr(x), n, d, kind, p, lay, _dptr(out), _dptr(rowmax),
+                        _dptr(radius), C.c_void_p(ts.cuda_stream), ev0, ev1)
     return out
 
 
