@@ -128,6 +128,36 @@ def test_knn_candidate_chunks(gpu, restatement, metric, n, d, kind, k):
     assert ids.shape == want.shape and np.array_equal(ids, want)
 
 
+@pytest.mark.parametrize("metric", METRICS)
+@pytest.mark.parametrize("shape,k", [("clusters", 15), ("duplicates", 15), ("duplicates", 64), ("line", 30),
+                                     ("aniso", 15)])
+def test_knn_seeded_scan_cases(gpu, restatement, metric, shape, k):
+    """The scan walks a spatial order and admits only candidates within each
+    query's seed bound (its k-th distance among its 4096 neighbours in that
+    order). Clusters far apart put several clusters in one seed range;
+    duplicates make the bound 0 with the k best decided by id among equal
+    distances; a segment has boxes of zero width; the 5-D cloud runs the
+    runtime-dimension scan. n = 9000 spans three seed ranges and chunks."""
+    from turbotops_amd import knn_lists_exact
+    x = _stress_cloud(shape, 9000, 43)
+    print(f"case knn-seeded metric={metric} shape={shape} k={k}", flush=True)
+    ids = knn_lists_exact(x, metric, k)
+    want = restatement.knn(x, metric, k)
+    if metric.startswith("lp"):
+        # Lp distances may sit within 2 ulp of the oracle's, so lists may swap
+        # neighbours at near-equal distances: every list must reach the same
+        # distances as the oracle's, rank by rank, recomputed in float64
+        p = float(metric.split(":")[1])
+        assert ids.shape == want.shape
+        xd = x.astype(np.float64)
+
+        def d64(lists):
+            return np.sort((np.abs(xd[:, None, :] - xd[lists]) ** p).sum(-1) ** (1 / p), axis=1)
+        assert np.allclose(d64(ids), d64(want), rtol=1e-6, atol=1e-30)
+    else:
+        assert ids.shape == want.shape and np.array_equal(ids, want)
+
+
 @pytest.mark.parametrize("metric", ["l1", "l2", "linf"])
 @pytest.mark.parametrize("n,d,kind,k", [(300, 3, "grid", 65), (700, 4, "gauss", 100), (1500, 8, "unif", 500),
                                         (120, 2, "grid", 119), (2000, 3, "grid", 300)])
@@ -196,6 +226,30 @@ def test_radius_graph(gpu, restatement, metric, n, d, kind, tau):
         assert np.array_equal(g.row_ptr, rp) and np.array_equal(g.col_idx, col)
         assert np.array_equal(bits(g.weights), bits(w))
     else:   # an ulp at the threshold may move an edge; away from it the graphs agree
+        assert abs(int(g.row_ptr[-1]) - int(rp[-1])) <= 2
+        if rp[-1]:
+            _lp_graphs_agree(g, rp, col, w, tau)
+    assert g.valid()
+
+
+@pytest.mark.parametrize("metric", METRICS)
+@pytest.mark.parametrize("shape,tau", [("clusters", 0.03), ("clusters", 40.0), ("duplicates", 0.0),
+                                       ("duplicates", 0.5), ("line", 1e-3), ("aniso", 1.5)])
+def test_radius_graph_pruned_tiles(gpu, restatement, shape, tau, metric):
+    """Tiles whose blocks' boxes lie farther apart than tau are skipped
+    whole; these inputs put tau at and around the box gaps: tight clusters
+    (tau inside a cluster, and tau spanning some clusters), duplicates (tau =
+    0 keeps exactly the coincident pairs), a segment (boxes of zero width)
+    and an anisotropic 5-D cloud (runtime dimension), n = 3000."""
+    from turbotops_amd import radius_graph
+    x = _stress_cloud(shape, 3000, 47)
+    print(f"case radius-pruned metric={metric} shape={shape} tau={tau}", flush=True)
+    g = radius_graph(x, metric, tau)
+    rp, col, w = restatement.radius_graph(x, metric, tau)
+    if metric in ("l1", "l2", "linf"):
+        assert np.array_equal(g.row_ptr, rp) and np.array_equal(g.col_idx, col)
+        assert np.array_equal(bits(g.weights), bits(w))
+    else:
         assert abs(int(g.row_ptr[-1]) - int(rp[-1])) <= 2
         if rp[-1]:
             _lp_graphs_agree(g, rp, col, w, tau)

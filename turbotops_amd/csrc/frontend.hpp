@@ -25,6 +25,11 @@ void set_tile_rows_per_wave(int rpw);
 u64 spatial_ext_entries(u32 dim);
 void spatial_order(const float* d_x, u32 n, u32 dim, const u32* d_n, u64* keys, u64* alt, u64* sort_scratch,
                    u32* ext, float* xs, u32* perm, hipStream_t s);
+// Boxes of groups of `per` consecutive members: out[b] = the coordinate
+// minima (dim floats) then maxima (dim floats) of members [b per, (b+1) per)
+// of src, which holds count points (from_boxes false: dim floats each) or
+// count boxes (from_boxes true: 2 dim floats each).
+void spatial_boxes(const float* src, u32 count, u32 per, u32 dim, bool from_boxes, float* out, hipStream_t s);
 
 // hostcopy.hip: device -> pageable host memory, staged through pinned
 // buffers for large sizes; returns when the data is in dst

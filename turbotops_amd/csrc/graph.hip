@@ -7,15 +7,21 @@
 // reference prunes with a k-d tree; pruning never changes its result, so an
 // exhaustive tiled scan reproduces it exactly.
 //
-//   radius graph  64x64 tiles of the lower triangle (as the distance matrix);
-//                 per wave and row a ballot of d(i, j) <= tau, one atomic per
-//                 ballot reserves room in an edge list, mbcnt places each edge.
+//   radius graph  64x64 tiles of the lower triangle (as the distance matrix)
+//                 over the points in a spatial order; a tile whose blocks' boxes
+//                 lie farther apart than tau is skipped whole (a lower bound of
+//                 every pair's computed distance); per wave and row a ballot of
+//                 d(i, j) <= tau, one atomic per ballot reserves room in an edge
+//                 list, mbcnt places each edge.
 //                 The CSR comes from a radix sort of (row, col) keys carrying the
 //                 weight bits: rows sorted by neighbour id, as from_rows.
 //   kNN           one query per lane (knn.hpp); candidates stream through
 //                 scalar loads; the k best (distance, id) pairs sit in VGPRs as
 //                 a sorted insertion network, admitting by the reference's
 //                 (distance, id) lexicographic rule (graph.hpp:112, :145).
+//                 Every pair is measured; queries and candidates follow a
+//                 spatial order, and a bound from each query's neighbourhood
+//                 in that order keeps far candidates out of the lists.
 //   kNN, k > 64   one workgroup per query: a radix select over the 32-bit
 //                 distance bits (four 8-bit passes, recomputing the distances)
 //                 finds the k-th distance D*, a second select over the ids
@@ -50,48 +56,80 @@ __device__ __forceinline__ u32 mbcnt64(u64 m) {
     return __builtin_amdgcn_mbcnt_hi(u32(m >> 32), __builtin_amdgcn_mbcnt_lo(u32(m), 0u));
 }
 
-// Edge list of every pair j < i with d(i, j) <= tau; entries past `cap`
-// are counted but not stored.
+// Key between the nearest points of two boxes (coordinate minima then
+// maxima): per coordinate the gap between the intervals, combined as the
+// metric's key (the fma sum of squares for L2, the sum for L1, the largest
+// gap for L-inf and Lp). For points a, b in the two boxes each gap is at most
+// |fl(a_k - b_k)| (the subtractions round monotonically), and the sums, fused
+// multiply-adds and maxima round monotonically too, so this is a lower bound
+// of the pair's key as computed; for Lp the largest coordinate difference m is
+// one (m * pow(s, 1/p) with s >= 1 is never below m).
 template <int V, int DIM>
-__global__ __launch_bounds__(GB) void radius_tiles(const float* __restrict__ x, u32 n, u32 dim_rt,
-                                                   MetricArgs ma, float tau, u64 cap,
+__device__ __forceinline__ float box_gap_key(const float* __restrict__ a, const float* __restrict__ b, u32 dim_rt) {
+    const u32 dim = DIM > 0 ? u32(DIM) : dim_rt;
+    float s = 0.0f;
+#pragma unroll
+    for (u32 k = 0; k < dim; ++k) {
+        const float g = __builtin_fmaxf(0.0f, __builtin_fmaxf(a[k] - b[dim + k], b[k] - a[dim + k]));
+        if constexpr (V == MV_L2) s = __builtin_fmaf(g, g, s);
+        else if constexpr (V == MV_L1) s += g;
+        else s = __builtin_fmaxf(s, g);
+    }
+    return s;
+}
+
+// Edge list of every pair with d <= tau, over the points in spatial order
+// (xs; perm[pos] = original id): 64x64 tiles of the lower triangle of
+// positions, each workgroup looping over tiles. A tile whose two blocks'
+// boxes are farther apart than tau holds no edge and is skipped whole;
+// otherwise every pair j < i of it is measured. Edges carry original ids;
+// entries past `cap` are counted but not stored.
+template <int V, int DIM>
+__global__ __launch_bounds__(GB) void radius_tiles(const float* __restrict__ x, const u32* __restrict__ perm,
+                                                   const float* __restrict__ box, u32 n, u32 dim_rt,
+                                                   MetricArgs ma, float tau, u64 tiles, u64 cap,
                                                    unsigned long long* __restrict__ counter,
                                                    u32* __restrict__ ei, u32* __restrict__ ej,
                                                    float* __restrict__ ew) {
-    u32 bi, bj;
-    tile_of_g(blockIdx.x, bi, bj);
+    const u32 dim = DIM > 0 ? u32(DIM) : dim_rt;
     const u32 lane = threadIdx.x & 63;
     const u32 wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const u32 j = bj * GT + lane;
-    const u32 jc = j < n ? j : n - 1;
-    const u32 ibase = bi * GT + wave * ROWS;
     // Compile-time dimensions measure ordering keys (for L2 the sum before
     // the square root): a key above key_reject_above(tau) cannot pass, so only
     // the rest take the exact square root; the test itself stays d <= tau on
     // the distance.
-    float val[ROWS];
-    if constexpr (DIM > 0) rows_static<V, DIM, ROWS, true>(x, n, ma, ibase, jc, val);
-    else rows_runtime<V>(x, n, dim_rt, ma, ibase, jc, val);
     const float reject = DIM > 0 ? key_reject_above<V>(tau) : INFINITY;
+    for (u64 t = blockIdx.x; t < tiles; t += gridDim.x) {
+        u32 bi, bj;
+        tile_of_g(t, bi, bj);
+        if (key_to_dist<V>(box_gap_key<V, DIM>(box + u64(bi) * 2 * dim, box + u64(bj) * 2 * dim, dim)) > tau)
+            continue;   // block-uniform
+        const u32 j = bj * GT + lane;
+        const u32 jc = j < n ? j : n - 1;
+        const u32 ibase = bi * GT + wave * ROWS;
+        float val[ROWS];
+        if constexpr (DIM > 0) rows_static<V, DIM, ROWS, true>(x, n, ma, ibase, jc, val);
+        else rows_runtime<V>(x, n, dim_rt, ma, ibase, jc, val);
 #pragma unroll
-    for (u32 r = 0; r < ROWS; ++r) {
-        const u32 i = ibase + r;
-        bool keep = i < n && j < i && val[r] <= reject;
-        if constexpr (DIM > 0) {
-            if (keep) val[r] = key_to_dist<V>(val[r]);
-        }
-        keep = keep && val[r] <= tau;
-        const u64 b = __ballot(keep);
-        if (b) {
-            unsigned long long base = 0;
-            if (lane == 0) base = atomicAdd(counter, (unsigned long long)__popcll(b));
-            base = __shfl(base, 0);
-            if (keep) {
-                const u64 pos = base + mbcnt64(b);
-                if (pos < cap) {
-                    ei[pos] = i;
-                    ej[pos] = j;
-                    ew[pos] = val[r];
+        for (u32 r = 0; r < ROWS; ++r) {
+            const u32 i = ibase + r;
+            bool keep = i < n && j < i && val[r] <= reject;
+            if constexpr (DIM > 0) {
+                if (keep) val[r] = key_to_dist<V>(val[r]);
+            }
+            keep = keep && val[r] <= tau;
+            const u64 b = __ballot(keep);
+            if (b) {
+                unsigned long long base = 0;
+                if (lane == 0) base = atomicAdd(counter, (unsigned long long)__popcll(b));
+                base = __shfl(base, 0);
+                if (keep) {
+                    const u64 pos = base + mbcnt64(b);
+                    if (pos < cap) {
+                        ei[pos] = perm[i];
+                        ej[pos] = perm[j];
+                        ew[pos] = val[r];
+                    }
                 }
             }
         }
@@ -100,9 +138,11 @@ __global__ __launch_bounds__(GB) void radius_tiles(const float* __restrict__ x, 
 
 template <int V, int DIM>
 struct RadiusLaunch {
-    static void run(u32 blocks, hipStream_t s, const float* x, u32 n, u32 dim, MetricArgs ma,
-                    float tau, u64 cap, unsigned long long* counter, u32* ei, u32* ej, float* ew) {
-        radius_tiles<V, DIM><<<blocks, GB, 0, s>>>(x, n, dim, ma, tau, cap, counter, ei, ej, ew);
+    static void run(u32 blocks, hipStream_t s, const float* x, const u32* perm, const float* box, u32 n, u32 dim,
+                    MetricArgs ma, float tau, u64 tiles, u64 cap, unsigned long long* counter, u32* ei, u32* ej,
+                    float* ew) {
+        radius_tiles<V, DIM><<<blocks, GB, 0, s>>>(x, perm, box, n, dim, ma, tau, tiles, cap, counter, ei, ej,
+                                                   ew);
     }
 };
 
@@ -152,33 +192,61 @@ __global__ __launch_bounds__(GB) void knn_scan(const float* __restrict__ x, u32 
     }
 }
 
-// Compile-time dimension (knn_scan_static: scalar candidate loads, key
-// rejection). The candidates are split into gridDim.y chunks (blockIdx.y =
-// chunk), so that a few thousand queries still fill the machine. With one
-// chunk the ids are written directly; otherwise each chunk's sorted
-// (distance, id) list goes to pd / pi and knn_merge combines them: the k best
-// of the union are the k best of the per-chunk k best.
+// Compile-time dimension. The points come in a spatial order (xs, with
+// perm[pos] the original id), and query lanes take consecutive positions, so
+// a wave's queries are near one another.
+//
+// knn_seed: each query's k-th distance among the other points of its own
+// SEED-position range (its neighbourhood in that order): an upper bound of
+// its true k-th distance, +inf when the range holds fewer than k others.
+constexpr u32 SEED = 4096;
+
 template <int V, int DIM, int KMAX>
-__global__ __launch_bounds__(GB) void knn_scan_s(const float* __restrict__ x, u32 n, MetricArgs ma, u32 k,
-                                                 u32* __restrict__ ids, float* __restrict__ pd,
-                                                 u32* __restrict__ pi) {
-    const u32 i = blockIdx.x * GB + threadIdx.x;
-    const u32 chunk = (n + gridDim.y - 1) / gridDim.y;
-    const u32 jlo = blockIdx.y * chunk, jhi = min(n, jlo + chunk);
-    const u32 qi = i < n ? i : n - 1;
+__global__ __launch_bounds__(GB) void knn_seed(const float* __restrict__ xs, u32 n, MetricArgs ma, u32 k,
+                                               float* __restrict__ kth0) {
+    const u32 qp = blockIdx.x * GB + threadIdx.x;
+    const u32 qc = qp < n ? qp : n - 1;
     float q[DIM];
 #pragma unroll
-    for (int t = 0; t < DIM; ++t) q[t] = x[u64(qi) * DIM + t];
+    for (int t = 0; t < DIM; ++t) q[t] = xs[u64(qc) * DIM + t];
+    // the range of the wave's first lane, so candidates stay wave-uniform
+    const u32 h0 = __builtin_amdgcn_readfirstlane(qc) / SEED * SEED;
     KnnList<KMAX> L;
     L.init(k);
-    knn_scan_static<V, DIM, KMAX>(x, jlo, jhi, qi, q, ma, k, L);
-    if (i < n) {
+    knn_scan_static<V, DIM, KMAX>(xs, h0, min(n, h0 + SEED), qc, q, ma, k, L);
+    if (qp < n) kth0[qp] = L.kth;
+}
+
+// The scan proper: the candidates are split into gridDim.y chunks
+// (blockIdx.y = chunk), so that a few thousand queries still fill the
+// machine; each chunk admits only distances up to the query's seed bound
+// (knn_scan_sorted). With one chunk the ids are written directly; otherwise
+// each chunk's sorted (distance, id) list goes to pd / pi (row = position) and
+// knn_merge combines them: the k best of the union are the k best of the
+// per-chunk k best.
+template <int V, int DIM, int KMAX>
+__global__ __launch_bounds__(GB) void knn_scan_s(const float* __restrict__ xs, const u32* __restrict__ perm, u32 n,
+                                                 MetricArgs ma, u32 k, const float* __restrict__ kth0,
+                                                 u32* __restrict__ ids, float* __restrict__ pd,
+                                                 u32* __restrict__ pi) {
+    const u32 qp = blockIdx.x * GB + threadIdx.x;
+    const u32 chunk = (n + gridDim.y - 1) / gridDim.y;
+    const u32 jlo = blockIdx.y * chunk, jhi = min(n, jlo + chunk);
+    const u32 qc = qp < n ? qp : n - 1;
+    float q[DIM];
+#pragma unroll
+    for (int t = 0; t < DIM; ++t) q[t] = xs[u64(qc) * DIM + t];
+    KnnList<KMAX> L;
+    L.init(k);
+    knn_scan_sorted<V, DIM, KMAX>(xs, perm, jlo, jhi, qc, q, ma, k, kth0[qc], L);
+    if (qp < n) {
         if (gridDim.y == 1) {
+            const u64 row = u64(perm[qp]) * k;
 #pragma unroll
             for (int t = 0; t < KMAX; ++t)
-                if (u32(t) < k) ids[u64(i) * k + t] = L.bid[t];
+                if (u32(t) < k) ids[row + t] = L.bid[t];
         } else {
-            const u64 base = (u64(blockIdx.y) * n + i) * k;
+            const u64 base = (u64(blockIdx.y) * n + qp) * k;
 #pragma unroll
             for (int t = 0; t < KMAX; ++t)
                 if (u32(t) < k) {
@@ -190,11 +258,14 @@ __global__ __launch_bounds__(GB) void knn_scan_s(const float* __restrict__ x, u3
 }
 
 // Merge of the per-chunk lists (each sorted by (distance, id); unused slots
-// hold (+inf, ~0)), one lane per query: the k smallest of the union.
+// hold (+inf, ~0)), one lane per query position: the k smallest of the
+// union, written to the query's original row perm[i].
 __global__ __launch_bounds__(GB) void knn_merge(const float* __restrict__ pd, const u32* __restrict__ pi, u32 n,
-                                                u32 k, u32 chunks, u32* __restrict__ ids) {
+                                                u32 k, u32 chunks, const u32* __restrict__ perm,
+                                                u32* __restrict__ ids) {
     const u32 i = blockIdx.x * GB + threadIdx.x;
     if (i >= n) return;
+    const u64 row = u64(perm[i]) * k;
     u32 cur[16];
     for (u32 c = 0; c < chunks; ++c) cur[c] = 0;
     for (u32 t = 0; t < k; ++t) {
@@ -211,7 +282,7 @@ __global__ __launch_bounds__(GB) void knn_merge(const float* __restrict__ pd, co
                 bc = c;
             }
         }
-        ids[u64(i) * k + t] = bj;
+        ids[row + t] = bj;
         ++cur[bc];
     }
 }
@@ -361,14 +432,19 @@ struct KnnChunks {
     u32 chunks = 1;          // candidate chunks (compile-time dimensions only)
     float* pd = nullptr;     // chunks x n x k partial lists
     u32* pi = nullptr;
+    const float* xs = nullptr;   // the points in spatial order (compile-time dimensions)
+    const u32* perm = nullptr;   // original id of each position
+    float* kth0 = nullptr;       // per position: the seed bound
 };
 
 template <int V, int DIM, int KMAX>
 void knn_one(u32 blocks, hipStream_t s, const float* x, u32 n, u32 dim, MetricArgs ma, u32 k, u32* ids,
              const KnnChunks& kc) {
     if constexpr (DIM > 0) {
-        knn_scan_s<V, DIM, KMAX><<<dim3(blocks, kc.chunks), GB, 0, s>>>(x, n, ma, k, ids, kc.pd, kc.pi);
-        if (kc.chunks > 1) knn_merge<<<blocks, GB, 0, s>>>(kc.pd, kc.pi, n, k, kc.chunks, ids);
+        knn_seed<V, DIM, KMAX><<<blocks, GB, 0, s>>>(kc.xs, n, ma, k, kc.kth0);
+        knn_scan_s<V, DIM, KMAX><<<dim3(blocks, kc.chunks), GB, 0, s>>>(kc.xs, kc.perm, n, ma, k, kc.kth0, ids,
+                                                                        kc.pd, kc.pi);
+        if (kc.chunks > 1) knn_merge<<<blocks, GB, 0, s>>>(kc.pd, kc.pi, n, k, kc.chunks, kc.perm, ids);
     } else {
         knn_scan<V, KMAX><<<blocks, GB, 0, s>>>(x, n, dim, ma, k, ids);
     }
@@ -477,8 +553,19 @@ DevCsr radius_graph_dev(const float* d_x, u32 n, u32 dim, int kind, float p, flo
     const MetricVariant v = variant_of(kind, p);
     const MetricArgs ma = metric_args(kind, p);
     const u64 nb = (u64(n) + GT - 1) / GT;
-    const u64 blocks = nb * (nb + 1) / 2;
-    if (blocks > 0x7fffffffull) fail(TT_ELENGTH, "radius_graph: n too large for one launch");
+    const u64 tiles = nb * (nb + 1) / 2;
+    const u32 blocks = u32(std::min<u64>(tiles, 16384));
+    // the points in spatial order with a box per 64-point block
+    float* xs = ws.get<float>(u64(n) * dim);
+    u32* perm = ws.get<u32>(n);
+    float* box = ws.get<float>(nb * 2 * dim);
+    {
+        u32* dn = ws.get<u32>(1);
+        fill_device_u32(dn, 1, n, s);
+        spatial_order(d_x, n, dim, dn, ws.get<u64>(n), ws.get<u64>(n), ws.get<u64>(radix_scratch_entries(n)),
+                      ws.get<u32>(spatial_ext_entries(dim)), xs, perm, s);
+        spatial_boxes(xs, n, GT, dim, false, box, s);
+    }
     unsigned long long* counter = ws.get<unsigned long long>(1);
     u64 cap = std::min<u64>(u64(n) * (n - 1) / 2, std::max<u64>(u64(1) << 20, u64(n) * 32));
     u64 m = 0;
@@ -487,7 +574,7 @@ DevCsr radius_graph_dev(const float* d_x, u32 n, u32 dim, int kind, float p, flo
         u32* ej = ws.get<u32>(cap);
         float* ew = ws.get<float>(cap);
         fill_device_u32(reinterpret_cast<u32*>(counter), 2, 0u, s);
-        dispatch_metric_dim<RadiusLaunch>(v, dim, u32(blocks), s, d_x, n, dim, ma, tau, cap, counter,
+        dispatch_metric_dim<RadiusLaunch>(v, dim, blocks, s, xs, perm, box, n, dim, ma, tau, tiles, cap, counter,
                                           ei, ej, ew);
         TTG_LAUNCH_CHECK();
         TTG_HIP(hipMemcpyAsync(&m, counter, sizeof(u64), hipMemcpyDeviceToHost, s));
@@ -567,6 +654,17 @@ u32 knn_dev(const float* d_x, u32 n, u32 dim, int kind, float p, u32 k, u32* d_i
     if (kc.chunks > 1) {
         kc.pd = ws.get<float>(u64(kc.chunks) * n * kk);
         kc.pi = ws.get<u32>(u64(kc.chunks) * n * kk);
+    }
+    if (dim == 1 || dim == 2 || dim == 3 || dim == 4 || dim == 8 || dim == 16) {   // the compile-time scans
+        float* xs = ws.get<float>(u64(n) * dim);
+        u32* perm = ws.get<u32>(n);
+        u32* dn = ws.get<u32>(1);
+        fill_device_u32(dn, 1, n, s);
+        spatial_order(d_x, n, dim, dn, ws.get<u64>(n), ws.get<u64>(n), ws.get<u64>(radix_scratch_entries(n)),
+                      ws.get<u32>(spatial_ext_entries(dim)), xs, perm, s);
+        kc.xs = xs;
+        kc.perm = perm;
+        kc.kth0 = ws.get<float>(n);
     }
     dispatch_metric_dim<KnnLaunch>(v, dim, kmax, blocks, s, d_x, n, dim, ma, kk, d_ids, kc);
     TTG_LAUNCH_CHECK();

@@ -92,6 +92,55 @@ __device__ __forceinline__ void knn_scan_static(const float* __restrict__ x, u32
     }
 }
 
+// Candidates at sorted positions [jlo, jhi) of xs (points in a spatial
+// order; perm[j] is the original id of position j) against the query at
+// position qp, compile-time dimension. Only candidates with distance <= cap
+// are admitted, where cap is at least the query's true k-th distance (the
+// k-th distance within some subset of the candidates): every member of the
+// true k best passes, so the list this leaves is still the k best of the
+// range among them, while most insertions (which a whole wave pays for when
+// any lane makes one) are avoided. Ids and ties are the original ids.
+template <int V, int DIM, int KMAX>
+__device__ __forceinline__ void knn_scan_sorted(const float* __restrict__ xs, const u32* __restrict__ perm,
+                                                u32 jlo, u32 jhi, u32 qp, const float (&q)[DIM],
+                                                const MetricArgs& ma, u32 k, float cap, KnnList<KMAX>& L) {
+    float reject = key_reject_above<V>(cap);
+    auto admit = [&](u32 j, float key) {
+        if (key <= reject && j != qp) {
+            const float d = key_to_dist<V>(key);
+            const u32 id = perm[j];
+            if (d <= cap && L.admits(d, id)) {
+                L.insert(d, id, k);
+                reject = key_reject_above<V>(L.kth < cap ? L.kth : cap);
+            }
+        }
+    };
+    // four candidates per step: their keys first, then one test whether any
+    // can enter (most cannot), then each in order
+    u32 j = jlo;
+    for (; j + 4 <= jhi; j += 4) {
+        float key[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            float c[DIM];
+#pragma unroll
+            for (int t = 0; t < DIM; ++t) c[t] = xs[u64(j + u) * DIM + t];
+            key[u] = pdist<V, DIM, true>(c, q, DIM, ma);
+        }
+        const float kmin = __builtin_fminf(__builtin_fminf(key[0], key[1]), __builtin_fminf(key[2], key[3]));
+        if (kmin <= reject) {
+#pragma unroll
+            for (int u = 0; u < 4; ++u) admit(j + u, key[u]);
+        }
+    }
+    for (; j < jhi; ++j) {
+        float c[DIM];
+#pragma unroll
+        for (int t = 0; t < DIM; ++t) c[t] = xs[u64(j) * DIM + t];
+        admit(j, pdist<V, DIM, true>(c, q, DIM, ma));
+    }
+}
+
 // The same for a runtime dimension: query and candidate read from memory
 // (the query's row stays in the L1 cache; the candidate is one address per
 // wave).

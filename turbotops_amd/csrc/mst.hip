@@ -26,7 +26,12 @@
 //   comp tags   per block and super-block: the one component of all its
 //               members, or "mixed"
 //   nearest     one lane per todo vertex finds its smallest (key, j) edge into
-//               another component. Super-blocks, then blocks, are visited from
+//               another component, in two passes: first within the wave's own
+//               super-block only, which with the kept vertices' edges gives
+//               each component an upper bound of its smallest edge; then over
+//               everything, starting from that bound (a vertex with nothing at
+//               or below it cannot hold its component's smallest edge; it
+//               records none and is searched again next round). Super-blocks, then blocks, are visited from
 //               the wave's own block outward in sorted order; one is skipped
 //               when it lies wholly in the vertex's component, or when the key
 //               to the nearest point of its box exceeds the best so far. The
@@ -92,7 +97,8 @@ __global__ __launch_bounds__(MB) void mst_nearest(const float* __restrict__ xs, 
                                                   const float* __restrict__ sbox,
                                                   const u32* __restrict__ sbcomp, u32 nblk,
                                                   float* __restrict__ best_key, u32* __restrict__ best_j,
-                                                  unsigned long long* __restrict__ pairs) {
+                                                  unsigned long long* __restrict__ pairs,
+                                                  const u32* __restrict__ bound, int local) {
     constexpr u32 DS = DIM > 0 ? DIM : 1;
     const u32 dim = DIM > 0 ? u32(DIM) : dim_rt;
     const u32 t = blockIdx.x * MB + threadIdx.x;
@@ -106,14 +112,17 @@ __global__ __launch_bounds__(MB) void mst_nearest(const float* __restrict__ xs, 
         for (int k = 0; k < DIM; ++k) qr[k] = q[k];
         q = qr;
     }
-    float bk = INFINITY;
+    // bounded search: only edges with key <= the component's bound can be
+    // its smallest, so the search starts there (ties admitted: bj = ~0)
+    float bk = bound ? __uint_as_float(bound[ci]) : INFINITY;
     u32 bj = 0xffffffffu;
     const u32 nsb = (nblk + SBB - 1) / SBB;
     const u32 home = __builtin_amdgcn_readfirstlane(i) / PB;   // the block of the wave's first vertex
     const u32 hsb = home / SBB;
     const u32 nlive = u32(__popcll(__ballot(live)));
     u64 measured = 0;   // pairs this wave evaluates
-    for (u32 su = 0; su < nsb; ++su) {
+    const u32 nsu = local ? 1u : nsb;   // local: the wave's own super-block only
+    for (u32 su = 0; su < nsu; ++su) {
         u32 sb = hsb + su;
         if (sb >= nsb) sb -= nsb;
         {
@@ -154,7 +163,9 @@ __global__ __launch_bounds__(MB) void mst_nearest(const float* __restrict__ xs, 
         }
     }
     if (live) {
-        best_key[i] = bk;
+        // nothing at or below the bound: this vertex cannot hold its
+        // component's smallest edge, and it is searched again next round
+        best_key[i] = bj == 0xffffffffu ? INFINITY : bk;
         best_j[i] = bj;
     }
     if ((threadIdx.x & 63) == 0 && measured) atomicAdd(pairs, (unsigned long long)measured);
@@ -164,40 +175,12 @@ template <int V, int DIM>
 struct NearestLaunch {
     static void run(u32 blocks, hipStream_t s, const float* xs, u32 n, u32 dim, MetricArgs ma, const u32* comp,
                     const u32* todo, u32 ntodo, const float* box, const u32* bcomp, const float* sbox,
-                    const u32* sbcomp, u32 nblk, float* bk, u32* bj, unsigned long long* pairs) {
+                    const u32* sbcomp, u32 nblk, float* bk, u32* bj, unsigned long long* pairs,
+                    const u32* bound, int local) {
         mst_nearest<V, DIM><<<blocks, MB, 0, s>>>(xs, n, dim, ma, comp, todo, ntodo, box, bcomp, sbox, sbcomp,
-                                                  nblk, bk, bj, pairs);
+                                                  nblk, bk, bj, pairs, bound, local);
     }
 };
-
-// One wave per box: the min and max of each coordinate over its members
-// (points of a block, or the boxes of a super-block's blocks).
-__global__ __launch_bounds__(MB) void mst_boxes(const float* __restrict__ src, u32 count, u32 per, u32 dim,
-                                                int from_boxes, float* __restrict__ out) {
-    const u32 lane = threadIdx.x & 63;
-    const u32 nbox = (count + per - 1) / per;
-    for (u32 b = (blockIdx.x * MB + threadIdx.x) / 64; b < nbox; b += gridDim.x * (MB / 64)) {
-        const u32 m = min(per, count - b * per);
-        const u32 e = b * per + (lane < m ? lane : 0u);
-        for (u32 k = 0; k < dim; ++k) {
-            float lo, hi;
-            if (from_boxes) {
-                lo = src[u64(e) * 2 * dim + k];
-                hi = src[u64(e) * 2 * dim + dim + k];
-            } else {
-                lo = hi = src[u64(e) * dim + k];
-            }
-            for (int off = 32; off > 0; off >>= 1) {
-                lo = __builtin_fminf(lo, __shfl_xor(lo, off));
-                hi = __builtin_fmaxf(hi, __shfl_xor(hi, off));
-            }
-            if (lane == 0) {
-                out[u64(b) * 2 * dim + k] = lo;
-                out[u64(b) * 2 * dim + dim + k] = hi;
-            }
-        }
-    }
-}
 
 // One wave per block (or super-block): the one component of its members
 // (points' comp, or blocks' tags), or MIXED.
@@ -221,26 +204,101 @@ __device__ __forceinline__ u32 lane_rank_u64(u64 mask) {
 // those whose best neighbour has joined their own component. Each wave
 // appends its vertices in lane order (one atomic per wave), so a wave of the
 // search takes neighbouring positions.
-__global__ void mst_todo(const u32* __restrict__ comp, const u32* __restrict__ best_j, u32 n,
-                         int first, u32* __restrict__ todo, u32* __restrict__ ntodo) {
-    const u32 lane = threadIdx.x & 63;
-    for (u32 v0 = blockIdx.x * blockDim.x + (threadIdx.x & ~63u); v0 < n; v0 += gridDim.x * blockDim.x) {
-        const u32 v = v0 + lane;
+__global__ __launch_bounds__(MB) void mst_todo(const u32* __restrict__ comp, const u32* __restrict__ best_j, u32 n,
+                                               int first, u32* __restrict__ todo, u32* __restrict__ ntodo) {
+    __shared__ u32 wcount[MB / 64], sbase;
+    const u32 lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    // one atomic per workgroup and step (one per wave serialised on the counter)
+    for (u32 b0 = blockIdx.x * MB; b0 < n; b0 += gridDim.x * MB) {   // block-uniform
+        const u32 v = b0 + threadIdx.x;
         const bool need = v < n && (first || best_j[v] == 0xffffffffu || comp[best_j[v]] == comp[v]);
         const u64 mask = __ballot(need);
-        if (!mask) continue;
-        u32 base = 0;
-        if (lane == 0) base = atomicAdd(ntodo, u32(__popcll(mask)));
-        base = __builtin_amdgcn_readfirstlane(base);
-        if (need) todo[base + lane_rank_u64(mask)] = v;
+        if (lane == 0) wcount[w] = u32(__popcll(mask));
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            u32 tot = 0;
+            for (u32 q = 0; q < MB / 64; ++q) tot += wcount[q];
+            sbase = tot ? atomicAdd(ntodo, tot) : 0u;
+        }
+        __syncthreads();
+        u32 off = sbase;
+        for (u32 q = 0; q < w; ++q) off += wcount[q];
+        if (need) todo[off + lane_rank_u64(mask)] = v;
+        __syncthreads();
     }
 }
 
-__global__ void mst_comp_key(const u32* __restrict__ comp, const float* __restrict__ best_key,
-                             u32 n, u32* __restrict__ cw) {
-    for (u32 v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x)
-        atomicMin(cw + comp[v], __float_as_uint(best_key[v]));
+// Per component, the smallest key of its vertices. Neighbouring positions
+// mostly share a component once components are large, and atomics on one
+// address serialise (one per wave cost 1.4 ms per launch at n = 1e6 with a few
+// components left), so each workgroup takes a range of 4096 positions: a wave
+// keeps a running minimum over its 16 consecutive 64-vertex chunks while they
+// all hold one component, the four waves' runs are merged, and one atomic per
+// component and range remains. Mixed chunks fall back to one atomic per vertex.
+constexpr u32 CK_RANGE = 4096;
+
+__global__ __launch_bounds__(MB) void mst_comp_key(const u32* __restrict__ comp, const float* __restrict__ best_key,
+                                                   u32 n, u32* __restrict__ cw) {
+    __shared__ u32 sc[MB / 64], sk[MB / 64];
+    const u32 lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    constexpr u32 CHUNKS = CK_RANGE / MB;   // chunks of 64 per wave
+    for (u64 r0 = u64(blockIdx.x) * CK_RANGE; r0 < n; r0 += u64(gridDim.x) * CK_RANGE) {
+        u32 run_c = MIXED, run_k = 0xffffffffu;   // wave-uniform
+        for (u32 t = 0; t < CHUNKS; ++t) {
+            const u64 v = r0 + u64(w * CHUNKS + t) * 64 + lane;
+            const bool in = v < n;
+            if (!__ballot(in)) break;   // past the end: uniform
+            const u32 c = in ? comp[v] : 0u;
+            const u32 key = in ? __float_as_uint(best_key[v]) : 0xffffffffu;
+            const u32 c0 = __builtin_amdgcn_readfirstlane(c);   // lane 0 is in range
+            if (__ballot(in && c != c0) == 0) {
+                u32 km = key;
+                for (int off = 32; off > 0; off >>= 1) km = min(km, u32(__shfl_xor(int(km), off)));
+                if (c0 != run_c) {
+                    if (run_c != MIXED && lane == 0) atomicMin(cw + run_c, run_k);
+                    run_c = c0;
+                    run_k = km;
+                } else {
+                    run_k = min(run_k, km);
+                }
+            } else {
+                // a chunk across components: one minimum and one atomic per
+                // component present (per-lane atomics here serialised on the
+                // few large components of the late rounds)
+                u64 rem = __ballot(in);
+                while (rem) {
+                    const u32 cq = __builtin_amdgcn_readlane(c, int(__builtin_ctzll(rem)));
+                    const bool mine = in && c == cq;
+                    u32 km = mine ? key : 0xffffffffu;
+                    for (int off = 32; off > 0; off >>= 1) km = min(km, u32(__shfl_xor(int(km), off)));
+                    if (cq == run_c) run_k = min(run_k, km);
+                    else if (lane == 0) atomicMin(cw + cq, km);
+                    rem &= ~__ballot(mine);
+                }
+            }
+        }
+        if (lane == 0) {
+            sc[w] = run_c;
+            sk[w] = run_k;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            for (u32 q = 0; q < MB / 64; ++q) {
+                if (sc[q] == MIXED) continue;
+                u32 kk = sk[q];
+                for (u32 q2 = q + 1; q2 < MB / 64; ++q2)
+                    if (sc[q2] == sc[q]) {
+                        kk = min(kk, sk[q2]);
+                        sc[q2] = MIXED;
+                    }
+                atomicMin(cw + sc[q], kk);
+            }
+        }
+        __syncthreads();
+    }
 }
+
+u32 comp_key_blocks(u32 n) { return u32(std::max<u64>(1, std::min<u64>((u64(n) + CK_RANGE - 1) / CK_RANGE, 16384))); }
 
 __global__ void mst_comp_pair(const u32* __restrict__ comp, const float* __restrict__ best_key,
                               const u32* __restrict__ best_j, u32 n, const u32* __restrict__ cw,
@@ -405,9 +463,8 @@ void mst_deaths_in(const float* d_x, u32 n, u32 dim, int kind, float p, float* d
 
     // spatial order; the sorted copy B.xs is what the rounds read
     spatial_order(d_x, n, dim, scal + 4, B.sk, B.alt, B.sort_scratch, B.ext, B.xs, nullptr, s);
-    mst_boxes<<<blocks_of(u64(B.nblk) * 64), MB, 0, s>>>(B.xs, n, PB, dim, 0, B.box);
-    mst_boxes<<<blocks_of(u64(B.nsb) * 64), MB, 0, s>>>(B.box, B.nblk, SBB, dim, 1, B.sbox);
-    TTG_LAUNCH_CHECK();
+    spatial_boxes(B.xs, n, PB, dim, false, B.box, s);
+    spatial_boxes(B.box, B.nblk, SBB, dim, true, B.sbox, s);
     u32 edges = 0;
     for (u32 round = 0; edges < n - 1; ++round) {
         if (round > 64) fail(TT_EDEVICE, "mst: Boruvka did not converge");
@@ -419,16 +476,27 @@ void mst_deaths_in(const float* d_x, u32 n, u32 dim, int kind, float p, float* d
         if (ntodo) {
             mst_tags<<<blocks_of(u64(B.nblk) * 64), MB, 0, s>>>(comp, n, B.bcomp);
             mst_tags<<<blocks_of(u64(B.nsb) * 64), MB, 0, s>>>(B.bcomp, B.nblk, B.sbcomp);
-            dispatch_metric_dim<NearestLaunch>(v, dim, (ntodo + MB - 1) / MB, s, B.xs, n, dim, ma, comp, todo,
-                                               ntodo, B.box, B.bcomp, B.sbox, B.sbcomp, B.nblk, bk, bj,
-                                               reinterpret_cast<unsigned long long*>(scal + 6));
+            unsigned long long* pairs = reinterpret_cast<unsigned long long*>(scal + 6);
+            const u32 nblocks = (ntodo + MB - 1) / MB;
+            // 1: each todo vertex searches its own super-block only; with the
+            //    kept vertices' edges that bounds every component's smallest
+            //    edge from above (cw)
+            dispatch_metric_dim<NearestLaunch>(v, dim, nblocks, s, B.xs, n, dim, ma, comp, todo, ntodo, B.box,
+                                               B.bcomp, B.sbox, B.sbcomp, B.nblk, bk, bj, pairs,
+                                               static_cast<const u32*>(nullptr), 1);
+            mst_comp_key<<<comp_key_blocks(n), MB, 0, s>>>(comp, bk, n, cw);
+            // 2: the full search, from that bound
+            dispatch_metric_dim<NearestLaunch>(v, dim, nblocks, s, B.xs, n, dim, ma, comp, todo, ntodo, B.box,
+                                               B.bcomp, B.sbox, B.sbcomp, B.nblk, bk, bj, pairs,
+                                               static_cast<const u32*>(cw), 0);
+            fill_device_u32(cw, n, 0xffffffffu, s);
             TTG_LAUNCH_CHECK();
         }
         if (st) {
             st->rounds = round + 1;
             st->scanned += ntodo;
         }
-        mst_comp_key<<<blocks_of(n), MB, 0, s>>>(comp, bk, n, cw);
+        mst_comp_key<<<comp_key_blocks(n), MB, 0, s>>>(comp, bk, n, cw);
         mst_comp_pair<<<blocks_of(n), MB, 0, s>>>(comp, bk, bj, n, cw, cp);
         mst_hook<<<blocks_of(n), MB, 0, s>>>(comp, n, cw, cp, parent, keys, scal + 2);
         TTG_LAUNCH_CHECK();
