@@ -265,7 +265,14 @@ def run_graph(args):
     if pairs is not None:
         tflops = pairs * (3 * d + 1) / per / 1e12
         out["roofline"] = {"bound": "valu", "achieved": round(tflops, 2), "peak": VALU_PEAK_TFLOPS,
-                           "unit": "TFLOP/s", "frac": round(tflops / VALU_PEAK_TFLOPS, 4), "traffic": None}
+                           "unit": "TFLOP/s", "frac": round(tflops / VALU_PEAK_TFLOPS, 4), "traffic": None,
+                           "pairs": pairs,
+                           "note": {"c3a-knn": "every pair measured (brute force over a spatial order)",
+                                    "c3b-radius": "brute-force pair count: tiles whose boxes lie farther "
+                                                  "apart than tau are skipped, so fewer pairs are measured",
+                                    "c2c-mst": "pairs actually measured by the box-pruned searches; the "
+                                               "searches are bound by box tests and scalar loads, not "
+                                               "FP32 issue"}[w]}
     if args.whole_run and w in ("c3a-knn", "c3b-radius"):
         # the configuration's own persistence call end to end: the kNN graph
         # complex at maxdim 2, the radius-graph complex at maxdim 1

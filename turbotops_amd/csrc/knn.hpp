@@ -60,28 +60,17 @@ struct KnnList {
 // Candidates j in [jlo, jhi) of the points x (row-major, DIM floats) against
 // the query q (its own index qi is skipped), compile-time dimension. The
 // candidate is the same for every lane, so its coordinates come through
-// scalar loads and feed the VALU as SGPR operands; the next candidate is
-// loaded while the current one is measured. The ordering key (pdist<KEY>:
-// for L2 the sum before the square root) rejects most candidates without the
-// exact square root: only keys at or below key_reject_above(k-th distance)
-// become distances and meet the (distance, id) rule.
+// scalar loads and feed the VALU as SGPR operands. The ordering key
+// (pdist<KEY>: for L2 the sum before the square root) rejects most candidates
+// without the exact square root: only keys at or below key_reject_above(k-th
+// distance) become distances and meet the (distance, id) rule. Four
+// candidates per step: their keys first, then one test whether any can enter.
 template <int V, int DIM, int KMAX>
 __device__ __forceinline__ void knn_scan_static(const float* __restrict__ x, u32 jlo, u32 jhi, u32 qi,
                                                 const float (&q)[DIM], const MetricArgs& ma, u32 k,
                                                 KnnList<KMAX>& L) {
-    if (jlo >= jhi) return;
     float reject = key_reject_above<V>(L.kth);   // keys above this cannot enter
-    float c[DIM];
-#pragma unroll
-    for (int t = 0; t < DIM; ++t) c[t] = x[u64(jlo) * DIM + t];
-    for (u32 j = jlo; j < jhi; ++j) {
-        float cur[DIM];
-#pragma unroll
-        for (int t = 0; t < DIM; ++t) cur[t] = c[t];
-        const u32 jn = j + 1 < jhi ? j + 1 : j;
-#pragma unroll
-        for (int t = 0; t < DIM; ++t) c[t] = x[u64(jn) * DIM + t];
-        const float key = pdist<V, DIM, true>(cur, q, DIM, ma);
+    auto admit = [&](u32 j, float key) {
         if (key <= reject && j != qi) {
             const float d = key_to_dist<V>(key);
             if (L.admits(d, j)) {
@@ -89,6 +78,28 @@ __device__ __forceinline__ void knn_scan_static(const float* __restrict__ x, u32
                 reject = key_reject_above<V>(L.kth);
             }
         }
+    };
+    u32 j = jlo;
+    for (; j + 4 <= jhi; j += 4) {
+        float key[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            float c[DIM];
+#pragma unroll
+            for (int t = 0; t < DIM; ++t) c[t] = x[u64(j + u) * DIM + t];
+            key[u] = pdist<V, DIM, true>(c, q, DIM, ma);
+        }
+        const float kmin = __builtin_fminf(__builtin_fminf(key[0], key[1]), __builtin_fminf(key[2], key[3]));
+        if (kmin <= reject) {
+#pragma unroll
+            for (int u = 0; u < 4; ++u) admit(j + u, key[u]);
+        }
+    }
+    for (; j < jhi; ++j) {
+        float c[DIM];
+#pragma unroll
+        for (int t = 0; t < DIM; ++t) c[t] = x[u64(j) * DIM + t];
+        admit(j, pdist<V, DIM, true>(c, q, DIM, ma));
     }
 }
 
