@@ -86,7 +86,8 @@ __device__ __forceinline__ float box_gap_key(const float* __restrict__ a, const 
 // entries past `cap` are counted but not stored.
 template <int V, int DIM>
 __global__ __launch_bounds__(GB) void radius_tiles(const float* __restrict__ x, const u32* __restrict__ perm,
-                                                   const float* __restrict__ box, u32 n, u32 dim_rt,
+                                                   const float* __restrict__ box, const float* __restrict__ box16,
+                                                   u32 n, u32 dim_rt,
                                                    MetricArgs ma, float tau, u64 tiles, u64 cap,
                                                    unsigned long long* __restrict__ counter,
                                                    u32* __restrict__ ei, u32* __restrict__ ej,
@@ -102,11 +103,17 @@ __global__ __launch_bounds__(GB) void radius_tiles(const float* __restrict__ x, 
     for (u64 t = blockIdx.x; t < tiles; t += gridDim.x) {
         u32 bi, bj;
         tile_of_g(t, bi, bj);
-        if (key_to_dist<V>(box_gap_key<V, DIM>(box + u64(bi) * 2 * dim, box + u64(bj) * 2 * dim, dim)) > tau)
+        const float* bxj = box + u64(bj) * 2 * dim;
+        if (key_to_dist<V>(box_gap_key<V, DIM>(box + u64(bi) * 2 * dim, bxj, dim)) > tau)
             continue;   // block-uniform
         const u32 j = bj * GT + lane;
         const u32 jc = j < n ? j : n - 1;
         const u32 ibase = bi * GT + wave * ROWS;
+        // the same test for this wave's 16 rows alone (the waves share no LDS
+        // and no barrier, so each skips on its own)
+        if (ibase >= n ||
+            key_to_dist<V>(box_gap_key<V, DIM>(box16 + u64(ibase / ROWS) * 2 * dim, bxj, dim)) > tau)
+            continue;   // wave-uniform
         float val[ROWS];
         if constexpr (DIM > 0) rows_static<V, DIM, ROWS, true>(x, n, ma, ibase, jc, val);
         else rows_runtime<V>(x, n, dim_rt, ma, ibase, jc, val);
@@ -138,11 +145,11 @@ __global__ __launch_bounds__(GB) void radius_tiles(const float* __restrict__ x, 
 
 template <int V, int DIM>
 struct RadiusLaunch {
-    static void run(u32 blocks, hipStream_t s, const float* x, const u32* perm, const float* box, u32 n, u32 dim,
-                    MetricArgs ma, float tau, u64 tiles, u64 cap, unsigned long long* counter, u32* ei, u32* ej,
-                    float* ew) {
-        radius_tiles<V, DIM><<<blocks, GB, 0, s>>>(x, perm, box, n, dim, ma, tau, tiles, cap, counter, ei, ej,
-                                                   ew);
+    static void run(u32 blocks, hipStream_t s, const float* x, const u32* perm, const float* box,
+                    const float* box16, u32 n, u32 dim, MetricArgs ma, float tau, u64 tiles, u64 cap,
+                    unsigned long long* counter, u32* ei, u32* ej, float* ew) {
+        radius_tiles<V, DIM><<<blocks, GB, 0, s>>>(x, perm, box, box16, n, dim, ma, tau, tiles, cap, counter, ei,
+                                                   ej, ew);
     }
 };
 
@@ -559,12 +566,14 @@ DevCsr radius_graph_dev(const float* d_x, u32 n, u32 dim, int kind, float p, flo
     float* xs = ws.get<float>(u64(n) * dim);
     u32* perm = ws.get<u32>(n);
     float* box = ws.get<float>(nb * 2 * dim);
+    float* box16 = ws.get<float>((u64(n) + ROWS - 1) / ROWS * 2 * dim);   // per 16 positions
     {
         u32* dn = ws.get<u32>(1);
         fill_device_u32(dn, 1, n, s);
         spatial_order(d_x, n, dim, dn, ws.get<u64>(n), ws.get<u64>(n), ws.get<u64>(radix_scratch_entries(n)),
                       ws.get<u32>(spatial_ext_entries(dim)), xs, perm, s);
         spatial_boxes(xs, n, GT, dim, false, box, s);
+        spatial_boxes(xs, n, ROWS, dim, false, box16, s);
     }
     unsigned long long* counter = ws.get<unsigned long long>(1);
     u64 cap = std::min<u64>(u64(n) * (n - 1) / 2, std::max<u64>(u64(1) << 20, u64(n) * 32));
@@ -574,8 +583,8 @@ DevCsr radius_graph_dev(const float* d_x, u32 n, u32 dim, int kind, float p, flo
         u32* ej = ws.get<u32>(cap);
         float* ew = ws.get<float>(cap);
         fill_device_u32(reinterpret_cast<u32*>(counter), 2, 0u, s);
-        dispatch_metric_dim<RadiusLaunch>(v, dim, blocks, s, xs, perm, box, n, dim, ma, tau, tiles, cap, counter,
-                                          ei, ej, ew);
+        dispatch_metric_dim<RadiusLaunch>(v, dim, blocks, s, xs, perm, box, box16, n, dim, ma, tau, tiles, cap,
+                                          counter, ei, ej, ew);
         TTG_LAUNCH_CHECK();
         TTG_HIP(hipMemcpyAsync(&m, counter, sizeof(u64), hipMemcpyDeviceToHost, s));
         TTG_HIP(hipStreamSynchronize(s));
