@@ -92,7 +92,7 @@ __device__ __forceinline__ float box_key(const float* __restrict__ q, const floa
 template <int V, int DIM>
 __global__ __launch_bounds__(MB) void mst_nearest(const float* __restrict__ xs, u32 n, u32 dim_rt,
                                                   MetricArgs ma, const u32* __restrict__ comp,
-                                                  const u32* __restrict__ todo, u32 ntodo,
+                                                  const u32* __restrict__ todo, const u32* __restrict__ d_ntodo,
                                                   const float* __restrict__ box, const u32* __restrict__ bcomp,
                                                   const float* __restrict__ sbox,
                                                   const u32* __restrict__ sbcomp, u32 nblk,
@@ -101,7 +101,10 @@ __global__ __launch_bounds__(MB) void mst_nearest(const float* __restrict__ xs, 
                                                   const u32* __restrict__ bound, int local) {
     constexpr u32 DS = DIM > 0 ? DIM : 1;
     const u32 dim = DIM > 0 ? u32(DIM) : dim_rt;
+    const u32 ntodo = *d_ntodo;   // the grid covers every vertex; the todo count is read here
+    if (ntodo == 0) return;       // grid-uniform
     const u32 t = blockIdx.x * MB + threadIdx.x;
+    if (blockIdx.x * MB >= ntodo) return;   // block-uniform: no todo vertex in this block
     const bool live = t < ntodo;
     const u32 i = live ? todo[t] : todo[ntodo - 1];
     const u32 ci = comp[i];
@@ -174,7 +177,7 @@ __global__ __launch_bounds__(MB) void mst_nearest(const float* __restrict__ xs, 
 template <int V, int DIM>
 struct NearestLaunch {
     static void run(u32 blocks, hipStream_t s, const float* xs, u32 n, u32 dim, MetricArgs ma, const u32* comp,
-                    const u32* todo, u32 ntodo, const float* box, const u32* bcomp, const float* sbox,
+                    const u32* todo, const u32* ntodo, const float* box, const u32* bcomp, const float* sbox,
                     const u32* sbcomp, u32 nblk, float* bk, u32* bj, unsigned long long* pairs,
                     const u32* bound, int local) {
         mst_nearest<V, DIM><<<blocks, MB, 0, s>>>(xs, n, dim, ma, comp, todo, ntodo, box, bcomp, sbox, sbcomp,
@@ -470,39 +473,32 @@ void mst_deaths_in(const float* d_x, u32 n, u32 dim, int kind, float p, float* d
         if (round > 64) fail(TT_EDEVICE, "mst: Boruvka did not converge");
         fill_device_u32(scal, 1, 0u, s);
         mst_todo<<<blocks_of(n), MB, 0, s>>>(comp, bj, n, round == 0, todo, scal);
-        u32 ntodo = 0;
-        TTG_HIP(hipMemcpyAsync(&ntodo, scal, sizeof(u32), hipMemcpyDeviceToHost, s));
-        TTG_HIP(hipStreamSynchronize(s));
-        if (ntodo) {
-            mst_tags<<<blocks_of(u64(B.nblk) * 64), MB, 0, s>>>(comp, n, B.bcomp);
-            mst_tags<<<blocks_of(u64(B.nsb) * 64), MB, 0, s>>>(B.bcomp, B.nblk, B.sbcomp);
-            unsigned long long* pairs = reinterpret_cast<unsigned long long*>(scal + 6);
-            const u32 nblocks = (ntodo + MB - 1) / MB;
-            // 1: each todo vertex searches its own super-block only; with the
-            //    kept vertices' edges that bounds every component's smallest
-            //    edge from above (cw)
-            dispatch_metric_dim<NearestLaunch>(v, dim, nblocks, s, B.xs, n, dim, ma, comp, todo, ntodo, B.box,
-                                               B.bcomp, B.sbox, B.sbcomp, B.nblk, bk, bj, pairs,
-                                               static_cast<const u32*>(nullptr), 1);
-            mst_comp_key<<<comp_key_blocks(n), MB, 0, s>>>(comp, bk, n, cw);
-            // 2: the full search, from that bound
-            dispatch_metric_dim<NearestLaunch>(v, dim, nblocks, s, B.xs, n, dim, ma, comp, todo, ntodo, B.box,
-                                               B.bcomp, B.sbox, B.sbcomp, B.nblk, bk, bj, pairs,
-                                               static_cast<const u32*>(cw), 0);
-            fill_device_u32(cw, n, 0xffffffffu, s);
-            TTG_LAUNCH_CHECK();
-        }
-        if (st) {
-            st->rounds = round + 1;
-            st->scanned += ntodo;
-        }
+        // the searches read the todo count on the device (their grid covers n)
+        mst_tags<<<blocks_of(u64(B.nblk) * 64), MB, 0, s>>>(comp, n, B.bcomp);
+        mst_tags<<<blocks_of(u64(B.nsb) * 64), MB, 0, s>>>(B.bcomp, B.nblk, B.sbcomp);
+        unsigned long long* pairs = reinterpret_cast<unsigned long long*>(scal + 6);
+        const u32 nblocks = u32((u64(n) + MB - 1) / MB);
+        // 1: each todo vertex searches its own super-block only; with the kept
+        //    vertices' edges that bounds every component's smallest edge from
+        //    above (cw)
+        dispatch_metric_dim<NearestLaunch>(v, dim, nblocks, s, B.xs, n, dim, ma, comp, todo,
+                                           static_cast<const u32*>(scal), B.box, B.bcomp, B.sbox, B.sbcomp, B.nblk,
+                                           bk, bj, pairs, static_cast<const u32*>(nullptr), 1);
+        mst_comp_key<<<comp_key_blocks(n), MB, 0, s>>>(comp, bk, n, cw);
+        // 2: the full search, from that bound
+        dispatch_metric_dim<NearestLaunch>(v, dim, nblocks, s, B.xs, n, dim, ma, comp, todo,
+                                           static_cast<const u32*>(scal), B.box, B.bcomp, B.sbox, B.sbcomp, B.nblk,
+                                           bk, bj, pairs, static_cast<const u32*>(cw), 0);
+        fill_device_u32(cw, n, 0xffffffffu, s);
+        TTG_LAUNCH_CHECK();
         mst_comp_key<<<comp_key_blocks(n), MB, 0, s>>>(comp, bk, n, cw);
         mst_comp_pair<<<blocks_of(n), MB, 0, s>>>(comp, bk, bj, n, cw, cp);
         mst_hook<<<blocks_of(n), MB, 0, s>>>(comp, n, cw, cp, parent, keys, scal + 2);
         TTG_LAUNCH_CHECK();
-        for (int it = 0; it < 64; ++it) {
+        // pointer jumping, three steps per check of the flag
+        for (int it = 0; it < 32; ++it) {
             fill_device_u32(scal + 1, 1, 0u, s);
-            mst_jump<<<blocks_of(n), MB, 0, s>>>(parent, n, scal + 1);
+            for (int r = 0; r < 3; ++r) mst_jump<<<blocks_of(n), MB, 0, s>>>(parent, n, scal + 1);
             u32 changed = 0;
             TTG_HIP(hipMemcpyAsync(&changed, scal + 1, sizeof(u32), hipMemcpyDeviceToHost, s));
             TTG_HIP(hipStreamSynchronize(s));
@@ -510,8 +506,14 @@ void mst_deaths_in(const float* d_x, u32 n, u32 dim, int kind, float p, float* d
         }
         mst_relabel<<<blocks_of(n), MB, 0, s>>>(comp, parent, n, cw, cp);
         TTG_LAUNCH_CHECK();
-        TTG_HIP(hipMemcpyAsync(&edges, scal + 2, sizeof(u32), hipMemcpyDeviceToHost, s));
+        u32 counts[3] = {0, 0, 0};   // todo count, (jump flag), edges so far
+        TTG_HIP(hipMemcpyAsync(counts, scal, sizeof(counts), hipMemcpyDeviceToHost, s));
         TTG_HIP(hipStreamSynchronize(s));
+        edges = counts[2];
+        if (st) {
+            st->rounds = round + 1;
+            st->scanned += counts[0];
+        }
     }
     if (edges != n - 1) fail(TT_EDEVICE, "mst: spanning tree has the wrong edge count");
     if (st) {
