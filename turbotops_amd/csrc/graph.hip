@@ -209,18 +209,27 @@ __global__ __launch_bounds__(GB) void knn_scan(const float* __restrict__ x, u32 
 constexpr u32 SEED = 4096;
 
 template <int V, int DIM, int KMAX>
-__global__ __launch_bounds__(GB) void knn_seed(const float* __restrict__ xs, u32 n, MetricArgs ma, u32 k,
-                                               float* __restrict__ kth0) {
+__global__ __launch_bounds__(GB) void knn_seed(const float* __restrict__ xs, const u32* __restrict__ perm, u32 n,
+                                               MetricArgs ma, u32 k, float* __restrict__ kth0) {
     const u32 qp = blockIdx.x * GB + threadIdx.x;
     const u32 qc = qp < n ? qp : n - 1;
     float q[DIM];
 #pragma unroll
     for (int t = 0; t < DIM; ++t) q[t] = xs[u64(qc) * DIM + t];
-    // the range of the wave's first lane, so candidates stay wave-uniform
-    const u32 h0 = __builtin_amdgcn_readfirstlane(qc) / SEED * SEED;
+    // the wave's first lane decides both ranges, so candidates stay
+    // wave-uniform: the wave's own 64 positions give a first bound (the k-th
+    // distance within them, +inf if they hold fewer than k others), and the
+    // SEED range is then scanned admitting only distances up to it. That
+    // range contains the 64, so its k-th distance is at most the bound, and
+    // the list's k-th is exactly the range's.
+    const u32 first = __builtin_amdgcn_readfirstlane(qc);
+    const u32 w0 = first / 64 * 64, h0 = first / SEED * SEED;
     KnnList<KMAX> L;
     L.init(k);
-    knn_scan_static<V, DIM, KMAX>(xs, h0, min(n, h0 + SEED), qc, q, ma, k, L);
+    knn_scan_static<V, DIM, KMAX>(xs, w0, min(n, w0 + 64), qc, q, ma, k, L);
+    const float bound = L.kth;
+    L.init(k);
+    knn_scan_sorted<V, DIM, KMAX>(xs, perm, h0, min(n, h0 + SEED), qc, q, ma, k, bound, L);
     if (qp < n) kth0[qp] = L.kth;
 }
 
@@ -448,7 +457,7 @@ template <int V, int DIM, int KMAX>
 void knn_one(u32 blocks, hipStream_t s, const float* x, u32 n, u32 dim, MetricArgs ma, u32 k, u32* ids,
              const KnnChunks& kc) {
     if constexpr (DIM > 0) {
-        knn_seed<V, DIM, KMAX><<<blocks, GB, 0, s>>>(kc.xs, n, ma, k, kc.kth0);
+        knn_seed<V, DIM, KMAX><<<blocks, GB, 0, s>>>(kc.xs, kc.perm, n, ma, k, kc.kth0);
         knn_scan_s<V, DIM, KMAX><<<dim3(blocks, kc.chunks), GB, 0, s>>>(kc.xs, kc.perm, n, ma, k, kc.kth0, ids,
                                                                         kc.pd, kc.pi);
         if (kc.chunks > 1) knn_merge<<<blocks, GB, 0, s>>>(kc.pd, kc.pi, n, k, kc.chunks, kc.perm, ids);
