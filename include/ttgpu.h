@@ -142,8 +142,8 @@ tt_status tt_mst_deaths_dev(const float* d_pts, uint32_t n, uint32_t dim, int ki
  * reference: it is for callers that cannot allocate device memory (code
  * under HIP graph capture, serving loops, harnesses that forbid hipMalloc).
  * Every temporary lives in one static 128 MiB scratch region per device,
- * held until the process ends; n <= 1048576 and about n (52.6 + 4.125 dim)
- * bytes of temporaries (dim <= 19 at n = 2^20; TT_ELENGTH beyond). The stream must
+ * held until the process ends; n <= 1048576 and about n (56.6 + 4.125 dim)
+ * bytes of temporaries (dim <= 17 at n = 2^20; TT_ELENGTH beyond). The stream must
  * belong to the current device. Calls that overlap on one device race on
  * that region: serialize them. Same results as tt_mst_deaths_dev. */
 tt_status tt_mst_deaths_dev_noalloc(const float* d_pts, uint32_t n, uint32_t dim, int kind, float p,

@@ -98,7 +98,8 @@ __global__ __launch_bounds__(MB) void mst_nearest(const float* __restrict__ xs, 
                                                   const u32* __restrict__ sbcomp, u32 nblk,
                                                   float* __restrict__ best_key, u32* __restrict__ best_j,
                                                   unsigned long long* __restrict__ pairs,
-                                                  const u32* __restrict__ bound, int local) {
+                                                  const u32* __restrict__ bound, int local,
+                                                  float* __restrict__ lower) {
     constexpr u32 DS = DIM > 0 ? DIM : 1;
     const u32 dim = DIM > 0 ? u32(DIM) : dim_rt;
     const u32 ntodo = *d_ntodo;   // the grid covers every vertex; the todo count is read here
@@ -108,6 +109,11 @@ __global__ __launch_bounds__(MB) void mst_nearest(const float* __restrict__ xs, 
     const bool live = t < ntodo;
     const u32 i = live ? todo[t] : todo[ntodo - 1];
     const u32 ci = comp[i];
+    // bounded pass: lower[i] never exceeds i's smallest outside key (that key
+    // only grows from round to round, as the outside set shrinks), so a vertex
+    // whose lower[i] is above its component's bound cannot hold the
+    // component's smallest edge and sits this search out
+    const bool search = live && !(bound && lower[i] > __uint_as_float(bound[ci]));
     float qr[DS];
     const float* __restrict__ q = xs + u64(i) * dim;
     if constexpr (DIM > 0) {
@@ -122,7 +128,7 @@ __global__ __launch_bounds__(MB) void mst_nearest(const float* __restrict__ xs, 
     const u32 nsb = (nblk + SBB - 1) / SBB;
     const u32 home = __builtin_amdgcn_readfirstlane(i) / PB;   // the block of the wave's first vertex
     const u32 hsb = home / SBB;
-    const u32 nlive = u32(__popcll(__ballot(live)));
+    const u32 nlive = u32(__popcll(__ballot(search)));
     u64 measured = 0;   // pairs this wave evaluates
     const u32 nsu = local ? 1u : nsb;   // local: the wave's own super-block only
     for (u32 su = 0; su < nsu; ++su) {
@@ -131,7 +137,7 @@ __global__ __launch_bounds__(MB) void mst_nearest(const float* __restrict__ xs, 
         {
             const float* bx = sbox + u64(sb) * 2 * dim;
             const float lb = box_key<V, DIM>(q, bx, bx + dim, dim);
-            if (!__ballot(live && sbcomp[sb] != ci && lb <= bk)) continue;
+            if (!__ballot(search && sbcomp[sb] != ci && lb <= bk)) continue;
         }
         const u32 b0 = sb * SBB, len = min(SBB, nblk - b0);
         const u32 start = sb == hsb ? home - b0 : 0u;
@@ -142,7 +148,7 @@ __global__ __launch_bounds__(MB) void mst_nearest(const float* __restrict__ xs, 
             {
                 const float* bx = box + u64(b) * 2 * dim;
                 const float lb = box_key<V, DIM>(q, bx, bx + dim, dim);
-                if (!__ballot(live && bcomp[b] != ci && lb <= bk)) continue;
+                if (!__ballot(search && bcomp[b] != ci && lb <= bk)) continue;
             }
             const u32 j1 = u32(min<u64>(n, u64(b + 1) * PB));
             measured += u64(nlive) * (j1 - b * PB);
@@ -166,10 +172,14 @@ __global__ __launch_bounds__(MB) void mst_nearest(const float* __restrict__ xs, 
         }
     }
     if (live) {
-        // nothing at or below the bound: this vertex cannot hold its
-        // component's smallest edge, and it is searched again next round
-        best_key[i] = bj == 0xffffffffu ? INFINITY : bk;
-        best_j[i] = bj;
+        // nothing at or below the bound (or sat out): this vertex cannot hold
+        // its component's smallest edge, and it is searched again next round
+        const bool found = search && bj != 0xffffffffu;
+        best_key[i] = found ? bk : INFINITY;
+        best_j[i] = found ? bj : 0xffffffffu;
+        // its next smallest outside key is at least the one found, and above
+        // the bound it searched against
+        if (bound && search) lower[i] = found ? bk : __uint_as_float(bound[ci]);
     }
     if ((threadIdx.x & 63) == 0 && measured) atomicAdd(pairs, (unsigned long long)measured);
 }
@@ -179,9 +189,9 @@ struct NearestLaunch {
     static void run(u32 blocks, hipStream_t s, const float* xs, u32 n, u32 dim, MetricArgs ma, const u32* comp,
                     const u32* todo, const u32* ntodo, const float* box, const u32* bcomp, const float* sbox,
                     const u32* sbcomp, u32 nblk, float* bk, u32* bj, unsigned long long* pairs,
-                    const u32* bound, int local) {
+                    const u32* bound, int local, float* lower) {
         mst_nearest<V, DIM><<<blocks, MB, 0, s>>>(xs, n, dim, ma, comp, todo, ntodo, box, bcomp, sbox, sbcomp,
-                                                  nblk, bk, bj, pairs, bound, local);
+                                                  nblk, bk, bj, pairs, bound, local, lower);
     }
 };
 
@@ -406,7 +416,7 @@ struct MstBuffers {
     u32 nblk, nsb;
     u32 *comp, *parent, *cw, *bj, *todo, *scal, *bcomp, *sbcomp, *ext;
     unsigned long long* cp;
-    float *bk, *keys, *xs, *box, *sbox;
+    float *bk, *keys, *xs, *box, *sbox, *lower;
     u64 *sk, *alt, *sort_scratch;
     MstBuffers(Carve& c, u32 n, u32 dim) {
         nblk = (n + PB - 1) / PB;
@@ -416,6 +426,7 @@ struct MstBuffers {
         cw = c.get<u32>(n);
         cp = c.get<unsigned long long>(n);
         bk = c.get<float>(n);
+        lower = c.get<float>(n);
         bj = c.get<u32>(n);
         todo = c.get<u32>(n);
         scal = c.get<u32>(8);      // [0] ntodo, [1] changed, [2] out count, [3] n - 1, [4] n, [6..7] pairs
@@ -458,6 +469,7 @@ void mst_deaths_in(const float* d_x, u32 n, u32 dim, int kind, float p, float* d
     fill_device_u32(cw, n, 0xffffffffu, s);
     fill_device_u32(reinterpret_cast<u32*>(cp), 2 * u64(n), 0xffffffffu, s);
     fill_device_u32(bj, n, 0xffffffffu, s);
+    fill_device_u32(reinterpret_cast<u32*>(B.lower), n, 0u, s);   // +0.0f: no bound known yet
     fill_device_u32(scal, 3, 0u, s);
     fill_device_u32(scal + 3, 1, n - 1, s);
     fill_device_u32(scal + 4, 1, n, s);
@@ -483,12 +495,12 @@ void mst_deaths_in(const float* d_x, u32 n, u32 dim, int kind, float p, float* d
         //    above (cw)
         dispatch_metric_dim<NearestLaunch>(v, dim, nblocks, s, B.xs, n, dim, ma, comp, todo,
                                            static_cast<const u32*>(scal), B.box, B.bcomp, B.sbox, B.sbcomp, B.nblk,
-                                           bk, bj, pairs, static_cast<const u32*>(nullptr), 1);
+                                           bk, bj, pairs, static_cast<const u32*>(nullptr), 1, B.lower);
         mst_comp_key<<<comp_key_blocks(n), MB, 0, s>>>(comp, bk, n, cw);
         // 2: the full search, from that bound
         dispatch_metric_dim<NearestLaunch>(v, dim, nblocks, s, B.xs, n, dim, ma, comp, todo,
                                            static_cast<const u32*>(scal), B.box, B.bcomp, B.sbox, B.sbcomp, B.nblk,
-                                           bk, bj, pairs, static_cast<const u32*>(cw), 0);
+                                           bk, bj, pairs, static_cast<const u32*>(cw), 0, B.lower);
         fill_device_u32(cw, n, 0xffffffffu, s);
         TTG_LAUNCH_CHECK();
         mst_comp_key<<<comp_key_blocks(n), MB, 0, s>>>(comp, bk, n, cw);
@@ -547,7 +559,7 @@ void mst_deaths_dev(const float* d_x, u32 n, u32 dim, int kind, float p, float* 
 // symbol); it passes the device address hipGetSymbolAddress returns, looked
 // up once per device with that device current.
 constexpr u32 kMstNoAllocMaxN = 1u << 20;
-constexpr u64 kMstScratchBytes = u64(128) << 20;   // n (52.6 + 4.125 dim) bytes: dim <= 19 at n = 2^20
+constexpr u64 kMstScratchBytes = u64(128) << 20;   // n (56.6 + 4.125 dim) bytes: dim <= 17 at n = 2^20
 
 static __device__ __align__(256) unsigned char scratch[kMstScratchBytes];
 
