@@ -146,6 +146,7 @@ void spatial_boxes(const float* src, u32 count, u32 per, u32 dim, bool from_boxe
 void spatial_order(const float* d_x, u32 n, u32 dim, const u32* d_n, u64* keys, u64* alt, u64* sort_scratch,
                    u32* ext, float* xs, u32* perm, hipStream_t s) {
     if (!n) return;
+    if (dim == 0) fail(TT_EINVAL, "spatial_order: dim must be >= 1");
     // grid cells over the extent of the first kdim coordinates, `bits` bits
     // each (code < 2^32), sorted stably (ties keep index order)
     const u32 kdim = std::min<u32>(dim, 32);
