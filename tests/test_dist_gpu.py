@@ -252,3 +252,31 @@ def test_events_variant_times_the_tile_kernel(gpu, restatement):
     assert np.array_equal(bits(out[0][0].reshape(n, n)), bits(want))
     assert np.array_equal(bits(out[1][0]), bits(out[0][0]))
     assert np.array_equal(out[1][1], out[0][1]) and out[1][2] == out[0][2]
+
+
+@pytest.mark.parametrize("metric", EXACT)
+@pytest.mark.parametrize("offset", [0, 1, 3])
+def test_full_layout_into_offset_views(gpu, restatement, metric, offset):
+    """The full layout for d <= 4 is streamed with 16-byte stores where the
+    matrix is 16-byte aligned and element stores where it is not: a matrix
+    view starting 0, 1 or 3 floats into a buffer must hold the same bits, and
+    the floats around it must stay untouched. n = 1001: rows do not divide the
+    1024-entry workgroups, so waves straddle rows."""
+    import torch
+    from turbotops_amd import dist_matrix_dev
+    n = 1001
+    x = _cloud(n, 3, seed=77)
+    xt = torch.from_numpy(x).cuda()
+    buf = torch.full((n * n + 8,), -7.0, dtype=torch.float32, device="cuda")
+    rowmax = torch.zeros(n, dtype=torch.int32, device="cuda")
+    radius = torch.zeros(1, dtype=torch.float32, device="cuda")
+    view = buf[offset:offset + n * n]
+    dist_matrix_dev(xt, metric, "full", out=view, rowmax=rowmax, radius=radius)
+    torch.cuda.synchronize()
+    got = buf.cpu().numpy()
+    want = restatement.make_dense(x, metric, True)
+    assert np.array_equal(bits(got[offset:offset + n * n].reshape(n, n)), bits(want))
+    assert np.all(got[:offset] == -7.0) and np.all(got[offset + n * n:] == -7.0)
+    rm = rowmax.cpu().numpy().view(np.float32)
+    assert np.array_equal(bits(rm), bits(want.max(axis=1)))
+    assert float(radius.cpu()[0]) == restatement.enclosing_radius(want, n)
