@@ -257,11 +257,11 @@ def test_events_variant_times_the_tile_kernel(gpu, restatement):
 @pytest.mark.parametrize("metric", EXACT)
 @pytest.mark.parametrize("offset", [0, 1, 3])
 def test_full_layout_into_offset_views(gpu, restatement, metric, offset):
-    """The full layout for d <= 4 is streamed with 16-byte stores where the
-    matrix is 16-byte aligned and element stores where it is not: a matrix
-    view starting 0, 1 or 3 floats into a buffer must hold the same bits, and
-    the floats around it must stay untouched. n = 1001: rows do not divide the
-    1024-entry workgroups, so waves straddle rows."""
+    """A matrix view starting 0, 1 or 3 floats into a larger buffer (16-byte
+    aligned or not): the tiles' row stores, whose buffer resources range-check
+    each row segment, must write exactly the view's bits and leave the floats
+    around it untouched, with the row maxima and radius unchanged. n = 1001:
+    partial tiles at the edges."""
     import torch
     from turbotops_amd import dist_matrix_dev
     n = 1001
