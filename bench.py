@@ -26,6 +26,8 @@ Prints ONE JSON line. Besides the contract keys it carries:
   differentiable  forward + backward of a top-8 H1 persistence loss on a
                   noisy circle (n = 2048), the backward kernel alone, and the
                   64-circle batch
+  graph_stages    seconds per call of the exact kNN-15 graph and the radius
+                  graph (n = 2e5, d = 8) and of Boruvka H0 (n = 1e6, d = 3)
 """
 import argparse
 import json
@@ -556,6 +558,8 @@ def main():
                     help="skip the 512-cloud set under L-inf")
     ap.add_argument("--no-c4-knn", dest="c4_knn", action="store_false",
                     help="skip the 512-cloud set through the batch's kNN-graph stage")
+    ap.add_argument("--no-graph-stages", dest="graph_stages", action="store_false",
+                    help="skip the compact C3a kNN / C3b radius / C2c Boruvka timings of the default line")
     ap.add_argument("--c5-n", type=int, default=2048, help="points of the differentiable circle")
     ap.add_argument("--c5-batch", type=int, default=64, help="clouds in the differentiable batch")
     ap.add_argument("--c5-batch-nmax", type=int, default=4096, help="largest n_i taken into that batch")
@@ -584,6 +588,20 @@ def main():
     if not args.no_diff:
         print("bench: differentiable leg", file=sys.stderr, flush=True)
         out["differentiable"] = run_differentiable(args)
+    if args.graph_stages:
+        # the point-space stages on their own configurations, host to host
+        # (seconds per call over 3 calls after a warm-up; no CPU baseline
+        # or whole run here: bench.py --workload NAME gives those)
+        print("bench: graph stages", file=sys.stderr, flush=True)
+        stages = {}
+        for w in ("c3a-knn", "c3b-radius", "c2c-mst"):
+            g = run_graph(argparse.Namespace(workload=w, warmup=1, steps=3, whole_run=False, no_cpu=True))
+            entry = {"seconds_per_call": g["value"], "config": g["config"]}
+            if "roofline" in g:
+                entry["pairs"] = g["roofline"]["pairs"]
+                entry["pairs_note"] = g["roofline"]["note"]
+            stages[w] = entry
+        out["graph_stages"] = stages
     print(json.dumps(out), flush=True)
 
 
