@@ -7,9 +7,17 @@
 // once into a HIP graph. Each cloud's graph is what persistence(cloud,
 // graph="knn", k) reduces.
 //
+//   spatial     per cloud: coordinate extents (bknn_extent), a grid code per
+//   order       point (bknn_codes), one stable sort of (cloud, code, index)
+//               keys over every cloud, the points copied in that order
+//               (bknn_gather) with each position's original index
 //   bknn_scan   one query per lane, one cloud per workgroup of 256 queries;
 //               one launch per dimension group (2, 3, 8, other), so each
-//               runs the list code specialised for its dimension (knn.hpp)
+//               runs the list code specialised for its dimension (knn.hpp).
+//               Compile-time dimensions follow the cloud's spatial order: a
+//               wave's 64 neighbouring positions bound each query's k-th
+//               distance, and the cloud is scanned admitting only distances up
+//               to it (every pair still measured; ids are the original ones)
 //   bknn_keys   both directions of every list entry as (cloud, row, col) keys
 //   radix sort  one sort over every cloud (cloud-major, rows ascending, then
 //               neighbour ids ascending: from_rows' order)
@@ -52,6 +60,7 @@ __device__ __forceinline__ u32 list_len(u32 k, u32 n) { return n >= 2 ? min(k, n
 
 template <int V, int DIM, int KMAX>
 __global__ __launch_bounds__(QB) void bknn_scan(KnnBatchShape b, MetricArgs ma, const float* __restrict__ pts,
+                                                const float* __restrict__ xs, const u32* __restrict__ perm,
                                                 const u32* __restrict__ qb_cloud, const u32* __restrict__ qb_first,
                                                 u64 qb0, u32* __restrict__ ids) {
     const u64 blk = qb0 + blockIdx.x;
@@ -60,22 +69,138 @@ __global__ __launch_bounds__(QB) void bknn_scan(KnnBatchShape b, MetricArgs ma, 
     const u32 kc = list_len(b.k, n);   // >= 1: only clouds with n >= 2 have query blocks
     const u32 i = qb_first[blk] + threadIdx.x;
     const u32 qi = i < n ? i : n - 1;
-    const float* __restrict__ x = pts + b.pt_off[c];
     KnnList<KMAX> L;
     L.init(kc);
     if constexpr (DIM > 0) {
+        // the cloud in its spatial order (xs; perm = original index per
+        // position): lane = position, so a wave's queries are neighbours. The
+        // wave's own 64 positions give a first bound of each query's k-th
+        // distance (+inf if they hold fewer than k others); the whole cloud is
+        // then scanned admitting only distances up to it (knn_scan_sorted), the
+        // list's ids and ties being the original ids.
+        const float* __restrict__ x = xs + b.pt_off[c];
+        const u32* __restrict__ pc = perm + b.row_base[c];
         float q[DIM];
 #pragma unroll
         for (int t = 0; t < DIM; ++t) q[t] = x[u64(qi) * DIM + t];
-        knn_scan_static<V, DIM, KMAX>(x, 0, n, qi, q, ma, kc, L);
-    } else {
-        knn_scan_runtime<V, KMAX>(x, b.dim[c], 0, n, qi, ma, kc, L);
-    }
-    if (i < n) {
-        u32* out = ids + b.list_off[c] + u64(i) * kc;
+        const u32 w0 = __builtin_amdgcn_readfirstlane(qi) / 64 * 64;
+        knn_scan_static<V, DIM, KMAX>(x, w0, min(n, w0 + 64), qi, q, ma, kc, L);
+        const float bound = L.kth;
+        L.init(kc);
+        knn_scan_sorted<V, DIM, KMAX>(x, pc, 0, n, qi, q, ma, kc, bound, L);
+        if (i < n) {
+            u32* out = ids + b.list_off[c] + u64(pc[i]) * kc;
 #pragma unroll
-        for (int t = 0; t < KMAX; ++t)
-            if (u32(t) < kc) out[t] = L.bid[t];
+            for (int t = 0; t < KMAX; ++t)
+                if (u32(t) < kc) out[t] = L.bid[t];
+        }
+    } else {
+        const float* __restrict__ x = pts + b.pt_off[c];
+        knn_scan_runtime<V, KMAX>(x, b.dim[c], 0, n, qi, ma, kc, L);
+        if (i < n) {
+            u32* out = ids + b.list_off[c] + u64(i) * kc;
+#pragma unroll
+            for (int t = 0; t < KMAX; ++t)
+                if (u32(t) < kc) out[t] = L.bid[t];
+        }
+    }
+}
+
+// ---- each cloud's spatial order (compile-time dimension groups) -------------
+// Order-preserving map of float bits to unsigned (finite values).
+__device__ __forceinline__ u32 bk_ord(float f) {
+    const u32 u = __float_as_uint(f);
+    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+__device__ __forceinline__ float bk_unord(u32 u) {
+    return __uint_as_float((u & 0x80000000u) ? (u & 0x7fffffffu) : ~u);
+}
+
+constexpr u32 EXT_STRIDE = 64;   // ext entries per cloud: 2 per coordinate, at most 32 coordinates
+
+// One workgroup per cloud: the order bits of each coordinate's smallest and
+// largest finite value over the cloud's points.
+__global__ __launch_bounds__(256) void bknn_extent(KnnBatchShape b, const float* __restrict__ pts,
+                                                   u32* __restrict__ ext) {
+    __shared__ u32 slo[4], shi[4];
+    const u32 c = blockIdx.x, n = b.n[c], dim = b.dim[c];
+    const u32 kdim = min(dim, 32u);
+    const float* __restrict__ x = pts + b.pt_off[c];
+    const u32 lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    for (u32 k = 0; k < kdim; ++k) {
+        u32 lo = 0xffffffffu, hi = 0u;
+        for (u32 v = threadIdx.x; v < n; v += 256) {
+            const float f = x[u64(v) * dim + k];
+            if (__builtin_isfinite(f)) {
+                lo = min(lo, bk_ord(f));
+                hi = max(hi, bk_ord(f));
+            }
+        }
+        for (int off = 32; off > 0; off >>= 1) {
+            lo = min(lo, u32(__shfl_xor(int(lo), off)));
+            hi = max(hi, u32(__shfl_xor(int(hi), off)));
+        }
+        if (lane == 0) {
+            slo[w] = lo;
+            shi[w] = hi;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            for (u32 q = 1; q < 4; ++q) {
+                lo = min(lo, slo[q]);
+                hi = max(hi, shi[q]);
+            }
+            ext[u64(c) * EXT_STRIDE + 2 * k] = lo;
+            ext[u64(c) * EXT_STRIDE + 2 * k + 1] = hi;
+        }
+        __syncthreads();
+    }
+}
+
+// key of point r: (cloud << (32 + codebits)) | (grid code << 32) | local index,
+// the code interleaving `bits` bits of the cell of each of the first kuse
+// coordinates over the cloud's extent.
+__global__ void bknn_codes(KnnBatchShape b, const float* __restrict__ pts, const u32* __restrict__ ext,
+                           u32 codebits, u64* __restrict__ keys) {
+    for (u64 r = blockIdx.x * u64(blockDim.x) + threadIdx.x; r < b.total_rows; r += u64(gridDim.x) * blockDim.x) {
+        const u32 c = b.row_cloud[r];
+        const u32 v = u32(r - b.row_base[c]);
+        const u32 dim = b.dim[c];
+        const u32 kuse = min(min(dim, 32u), codebits);
+        const u32 bits = max(1u, min(10u, codebits / kuse));
+        const float cells = float(1u << bits);
+        const float* __restrict__ p = pts + b.pt_off[c] + u64(v) * dim;
+        u32 cell[32];
+        for (u32 k = 0; k < kuse; ++k) {
+            const float lo = bk_unord(ext[u64(c) * EXT_STRIDE + 2 * k]);
+            const float hi = bk_unord(ext[u64(c) * EXT_STRIDE + 2 * k + 1]);
+            const float f = p[k];
+            u32 cl = 0;
+            if (hi > lo && f > lo) {   // false for NaN and for an empty extent
+                const float t = (f - lo) / (hi - lo) * cells;
+                cl = t >= cells ? (1u << bits) - 1u : u32(t);
+            }
+            cell[k] = cl;
+        }
+        u64 code = 0;
+        for (int bt = int(bits) - 1; bt >= 0; --bt)
+            for (u32 k = 0; k < kuse; ++k) code = (code << 1) | ((cell[k] >> bt) & 1u);
+        keys[r] = (u64(c) << (32 + codebits)) | (code << 32) | v;
+    }
+}
+
+// Sorted position r (global; the cloud's positions stay within its own rows,
+// the sort being stable with the cloud in the top bits): the point's copy in
+// xs and its local original index.
+__global__ void bknn_gather(KnnBatchShape b, const float* __restrict__ pts, const u64* __restrict__ sorted,
+                            float* __restrict__ xs, u32* __restrict__ perm) {
+    for (u64 r = blockIdx.x * u64(blockDim.x) + threadIdx.x; r < b.total_rows; r += u64(gridDim.x) * blockDim.x) {
+        const u32 c = b.row_cloud[r];
+        const u32 v = u32(sorted[r]);
+        const u32 dim = b.dim[c];
+        const u64 lp = r - b.row_base[c];
+        for (u32 k = 0; k < dim; ++k) xs[b.pt_off[c] + lp * dim + k] = pts[b.pt_off[c] + u64(v) * dim + k];
+        perm[r] = v;
     }
 }
 
@@ -128,39 +253,40 @@ inline int dim_group(u32 d) { return d == 2 ? 0 : d == 3 ? 1 : d == 8 ? 2 : 3; }
 
 template <int V, int KMAX>
 void launch_scan_groups(const u64* gq, hipStream_t s, const KnnBatchShape& b, MetricArgs ma, const float* pts,
-                        const u32* qb_cloud, const u32* qb_first, u32* ids) {
+                        const float* xs, const u32* perm, const u32* qb_cloud, const u32* qb_first, u32* ids) {
     for (int g = 0; g < kDimGroups; ++g) {
         const u64 q0 = gq[g], q1 = gq[g + 1];
         if (q1 <= q0) continue;
         const u32 grid = u32(q1 - q0);
         switch (kGroupDim[g]) {
-            case 2: bknn_scan<V, 2, KMAX><<<grid, QB, 0, s>>>(b, ma, pts, qb_cloud, qb_first, q0, ids); break;
-            case 3: bknn_scan<V, 3, KMAX><<<grid, QB, 0, s>>>(b, ma, pts, qb_cloud, qb_first, q0, ids); break;
-            case 8: bknn_scan<V, 8, KMAX><<<grid, QB, 0, s>>>(b, ma, pts, qb_cloud, qb_first, q0, ids); break;
-            default: bknn_scan<V, 0, KMAX><<<grid, QB, 0, s>>>(b, ma, pts, qb_cloud, qb_first, q0, ids); break;
+            case 2: bknn_scan<V, 2, KMAX><<<grid, QB, 0, s>>>(b, ma, pts, xs, perm, qb_cloud, qb_first, q0, ids); break;
+            case 3: bknn_scan<V, 3, KMAX><<<grid, QB, 0, s>>>(b, ma, pts, xs, perm, qb_cloud, qb_first, q0, ids); break;
+            case 8: bknn_scan<V, 8, KMAX><<<grid, QB, 0, s>>>(b, ma, pts, xs, perm, qb_cloud, qb_first, q0, ids); break;
+            default: bknn_scan<V, 0, KMAX><<<grid, QB, 0, s>>>(b, ma, pts, xs, perm, qb_cloud, qb_first, q0, ids); break;
         }
     }
 }
 
 template <int V>
 void launch_scan_k(u32 kmax, const u64* gq, hipStream_t s, const KnnBatchShape& b, MetricArgs ma, const float* pts,
-                   const u32* qb_cloud, const u32* qb_first, u32* ids) {
+                   const float* xs, const u32* perm, const u32* qb_cloud, const u32* qb_first, u32* ids) {
     switch (kmax) {
-        case 8: launch_scan_groups<V, 8>(gq, s, b, ma, pts, qb_cloud, qb_first, ids); break;
-        case 16: launch_scan_groups<V, 16>(gq, s, b, ma, pts, qb_cloud, qb_first, ids); break;
-        case 32: launch_scan_groups<V, 32>(gq, s, b, ma, pts, qb_cloud, qb_first, ids); break;
-        default: launch_scan_groups<V, 64>(gq, s, b, ma, pts, qb_cloud, qb_first, ids); break;
+        case 8: launch_scan_groups<V, 8>(gq, s, b, ma, pts, xs, perm, qb_cloud, qb_first, ids); break;
+        case 16: launch_scan_groups<V, 16>(gq, s, b, ma, pts, xs, perm, qb_cloud, qb_first, ids); break;
+        case 32: launch_scan_groups<V, 32>(gq, s, b, ma, pts, xs, perm, qb_cloud, qb_first, ids); break;
+        default: launch_scan_groups<V, 64>(gq, s, b, ma, pts, xs, perm, qb_cloud, qb_first, ids); break;
     }
 }
 
 void launch_scan(MetricVariant v, u32 kmax, const u64* gq, hipStream_t s, const KnnBatchShape& b, MetricArgs ma,
-                 const float* pts, const u32* qb_cloud, const u32* qb_first, u32* ids) {
+                 const float* pts, const float* xs, const u32* perm, const u32* qb_cloud, const u32* qb_first,
+                 u32* ids) {
     switch (v) {
-        case MV_L1: launch_scan_k<MV_L1>(kmax, gq, s, b, ma, pts, qb_cloud, qb_first, ids); break;
-        case MV_L2: launch_scan_k<MV_L2>(kmax, gq, s, b, ma, pts, qb_cloud, qb_first, ids); break;
-        case MV_LINF: launch_scan_k<MV_LINF>(kmax, gq, s, b, ma, pts, qb_cloud, qb_first, ids); break;
-        case MV_LP_INT: launch_scan_k<MV_LP_INT>(kmax, gq, s, b, ma, pts, qb_cloud, qb_first, ids); break;
-        default: launch_scan_k<MV_LP_POW>(kmax, gq, s, b, ma, pts, qb_cloud, qb_first, ids); break;
+        case MV_L1: launch_scan_k<MV_L1>(kmax, gq, s, b, ma, pts, xs, perm, qb_cloud, qb_first, ids); break;
+        case MV_L2: launch_scan_k<MV_L2>(kmax, gq, s, b, ma, pts, xs, perm, qb_cloud, qb_first, ids); break;
+        case MV_LINF: launch_scan_k<MV_LINF>(kmax, gq, s, b, ma, pts, xs, perm, qb_cloud, qb_first, ids); break;
+        case MV_LP_INT: launch_scan_k<MV_LP_INT>(kmax, gq, s, b, ma, pts, xs, perm, qb_cloud, qb_first, ids); break;
+        default: launch_scan_k<MV_LP_POW>(kmax, gq, s, b, ma, pts, xs, perm, qb_cloud, qb_first, ids); break;
     }
 }
 
@@ -203,6 +329,12 @@ struct BatchKnnState {
         *d_rp = nullptr;   // row offsets over every cloud's rows (total_rows + 1)
     u64 *d_keys = nullptr, *d_alt = nullptr, *d_pt_off = nullptr, *d_row_base = nullptr, *d_list_off = nullptr;
     u64* d_sort = nullptr;   // radix sort scratch (radix_scratch_entries)
+    // each cloud's spatial order: keys and their sort, the sorted copy of the
+    // points, the original index per position, extents, the point count
+    u64 *d_skeys = nullptr, *d_salt = nullptr, *d_ssort = nullptr;
+    float* d_xs = nullptr;
+    u32 *d_perm = nullptr, *d_ext = nullptr, *d_nrows = nullptr;
+    u32 codebits = 23;
     KnnBatchShape shape{};
     float* h_pts = nullptr;
     std::vector<u32> h_rp, h_col, lists;
@@ -229,7 +361,14 @@ struct BatchKnnState {
             fill_device_u32(d_rp, total_rows + 1, 0u, s);
             return;
         }
-        launch_scan(variant, kmax, gq.data(), s, shape, ma, d_pts, d_qb_cloud, d_qb_first, d_ids);
+        bknn_extent<<<nclouds, 256, 0, s>>>(shape, d_pts, d_ext);
+        bknn_codes<<<grid_for(total_rows), 256, 0, s>>>(shape, d_pts, d_ext, codebits, d_skeys);
+        TTG_LAUNCH_CHECK();
+        const u64* order = radix_sort_u64_counted(d_skeys, std::max<u64>(total_rows, 1), d_nrows, 32, 64, d_salt,
+                                                  d_ssort, s);
+        bknn_gather<<<grid_for(total_rows), 256, 0, s>>>(shape, d_pts, order, d_xs, d_perm);
+        TTG_LAUNCH_CHECK();
+        launch_scan(variant, kmax, gq.data(), s, shape, ma, d_pts, d_xs, d_perm, d_qb_cloud, d_qb_first, d_ids);
         TTG_LAUNCH_CHECK();
         for (u32 c : large)
             knn_large_k(d_pts + pt_off[c], n[c], dim[c], kind, p, std::min(k, n[c] - 1), d_ids + list_off[c],
@@ -337,10 +476,25 @@ void bknn_setup(BatchKnnState& b, u32 nclouds, const uint32_t* n, const uint32_t
     b.d_row_cloud = ws.get<u32>(row_cloud.size());
     b.d_qb_cloud = ws.get<u32>(std::max<size_t>(qb_cloud.size(), 1));
     b.d_qb_first = ws.get<u32>(std::max<size_t>(qb_first.size(), 1));
+    {
+        const u64 r1 = std::max<u64>(b.total_rows, 1);
+        if (b.total_rows >= (u64(1) << 32)) fail(TT_ELENGTH, "tt_batch_knn_create: 2^32 points or more in one batch");
+        b.codebits = u32(32 - b.cb);   // cloud bits and code bits share the top 32 of each key
+        if (b.codebits > 23) b.codebits = 23;
+        b.d_skeys = ws.get<u64>(r1);
+        b.d_salt = ws.get<u64>(r1);
+        b.d_ssort = ws.get<u64>(radix_scratch_entries(r1));
+        b.d_xs = ws.get<float>(std::max<u64>(b.total_pts, 1));
+        b.d_perm = ws.get<u32>(r1);
+        b.d_ext = ws.get<u32>(u64(nclouds) * EXT_STRIDE);
+        b.d_nrows = ws.get<u32>(1);
+    }
     TTG_HIP(hipStreamCreateWithFlags(&b.stream, hipStreamNonBlocking));
     hipStream_t s = b.stream;
     const u32 count = u32(b.m);
     TTG_HIP(hipMemcpyAsync(b.d_count, &count, 4, hipMemcpyHostToDevice, s));
+    const u32 nrows = u32(b.total_rows);
+    TTG_HIP(hipMemcpyAsync(b.d_nrows, &nrows, 4, hipMemcpyHostToDevice, s));
     TTG_HIP(hipMemcpyAsync(b.d_n, b.n.data(), nclouds * 4, hipMemcpyHostToDevice, s));
     TTG_HIP(hipMemcpyAsync(b.d_dim, b.dim.data(), nclouds * 4, hipMemcpyHostToDevice, s));
     TTG_HIP(hipMemcpyAsync(b.d_pt_off, b.pt_off.data(), (nclouds + 1) * 8, hipMemcpyHostToDevice, s));
