@@ -280,3 +280,31 @@ def test_full_layout_into_offset_views(gpu, restatement, metric, offset):
     rm = rowmax.cpu().numpy().view(np.float32)
     assert np.array_equal(bits(rm), bits(want.max(axis=1)))
     assert float(radius.cpu()[0]) == restatement.enclosing_radius(want, n)
+
+
+@pytest.mark.parametrize("shape", [(1032, 3), (130, 4), (97, 8), (90, 21)])
+def test_nan_and_inf_coordinates_dense(gpu, restatement, shape):
+    """NaN (both signs), +inf and -inf coordinates, full and lower layouts:
+    the matrix is the oracle's bit for bit, with a zero diagonal (the
+    reference never computes it) and, for L2, the reference's NaN (that of
+    the last NaN term in its x86 arithmetic); the full and lower layouts
+    give the same radius, and the row maxima skip NaN distances as IEEE
+    maxNum does."""
+    n, d = shape
+    rng = np.random.default_rng(91 + d)
+    x = rng.normal(size=(n, d)).astype(np.float32)
+    x[rng.integers(0, n, 20), rng.integers(0, d, 20)] = np.nan
+    x[rng.integers(0, n, 8), rng.integers(0, d, 8)] = np.inf
+    x[rng.integers(0, n, 8), rng.integers(0, d, 8)] = -np.inf
+    x[rng.integers(0, n, 4), rng.integers(0, d, 4)] = -np.float32(np.nan)
+    print(f"case nan/inf n={n} d={d}", flush=True)
+    for metric in EXACT:
+        want = restatement.make_dense(x, metric, True)
+        want_low = restatement.make_dense(x, metric, False)
+        got, rowmax, radius = _dev(metric, x, "full")
+        low, _, radius_l = _dev(metric, x, "lower")
+        assert np.array_equal(bits(got.reshape(n, n)), bits(want)), metric
+        assert np.array_equal(bits(low), bits(want_low)), metric
+        assert bits(radius_l) == bits(radius), metric
+        finite = np.where(np.isnan(want), np.float32(0), want)
+        assert np.array_equal(bits(rowmax), bits(finite.max(axis=1))), metric

@@ -77,7 +77,6 @@ __device__ __forceinline__ void tile_out(const float (&val)[RPW], u32 n, u32 bi,
     sm.part_col[wave][lane] = cmax;
     __syncthreads();
 
-    const u32 i = bi * DTILE + lane;
     float rmax = 0.0f;
 #pragma unroll
     for (u32 c = 0; c < RPW; ++c) {
@@ -92,6 +91,51 @@ __device__ __forceinline__ void tile_out(const float (&val)[RPW], u32 n, u32 bi,
     }
     sm.part_row[wave][lane] = rmax;
     __syncthreads();
+}
+
+// The reference's value for a dense-matrix cell whose L2 sum is NaN: it
+// computes d(x_a, x_b) with a the larger index (rips.hpp:397-399) on x86,
+// where a - b returns a NaN operand quieted (a first, sign kept) and inf - inf
+// the default NaN 0xffc00000, fma(d, d, s) returns d's NaN before s's, and
+// the square root passes its NaN on: the cell holds the NaN of the last NaN
+// term. The GPU's arithmetic passes NaNs on by other rules (its subtract
+// flips the sign of a NaN in b), so NaN results are rebuilt here (the sum is
+// NaN exactly when some term is).
+__device__ __noinline__ float l2_nan_ref(const float* __restrict__ a, const float* __restrict__ b, u32 dim) {
+    u32 r = 0xffc00000u;
+    for (u32 k = 0; k < dim; ++k) {
+        const u32 ab = __float_as_uint(a[k]), bb = __float_as_uint(b[k]);
+        if ((ab & 0x7fffffffu) > 0x7f800000u) r = ab | 0x00400000u;
+        else if ((bb & 0x7fffffffu) > 0x7f800000u) r = bb | 0x00400000u;
+        else if ((ab & 0x7fffffffu) == 0x7f800000u && ab == bb) r = 0xffc00000u;
+    }
+    return __uint_as_float(r);
+}
+
+// Cells as the reference holds them: 0 on the diagonal, which it never
+// computes (a point with a NaN or infinite coordinate gives NaN against
+// itself), and for L2 the reference's NaN. Only diagonal tiles test the
+// diagonal, and the NaN repair runs when some value of the wave is NaN.
+template <int V, int RPW>
+__device__ __forceinline__ void dense_cells(float (&val)[RPW], u32 ibase, u32 jc, u32 n, bool diag,
+                                            const float* __restrict__ x, u32 dim) {
+    if (diag) {
+#pragma unroll
+        for (u32 r = 0; r < RPW; ++r)
+            if (min(ibase + r, n - 1) == jc) val[r] = 0.0f;
+    }
+    if constexpr (V == MV_L2) {
+        bool nan = false;
+#pragma unroll
+        for (u32 r = 0; r < RPW; ++r) nan |= val[r] != val[r];
+        if (nan) {
+#pragma unroll
+            for (u32 r = 0; r < RPW; ++r) {
+                const u32 i = min(ibase + r, n - 1);
+                if (val[r] != val[r]) val[r] = l2_nan_ref(x + u64(max(i, jc)) * dim, x + u64(min(i, jc)) * dim, dim);
+            }
+        }
+    }
 }
 
 // ld: row stride of the full layout in floats (n, or n rounded up for
@@ -111,6 +155,7 @@ __device__ __forceinline__ void dist_tile(const float* __restrict__ x, u32 n, u3
     float val[RPW];
     if constexpr (DIM > 0) rows_static<V, DIM, RPW>(x, n, ma, ibase, jc, val);
     else rows_runtime<V, RPW>(x, n, dim_rt, ma, ibase, jc, val);
+    dense_cells<V, RPW>(val, ibase, jc, n, bi == bj, x, DIM > 0 ? u32(DIM) : dim_rt);
 
     // A tile with every row and column below n (bj <= bi, so (bi + 1) * 64 <= n
     // suffices) stores without per-element guards; each store addresses a
